@@ -1,0 +1,861 @@
+// HIP kernels of the COMCOT shallow-water step for gfx950 (MI355X).
+//
+// One layer-step is one kernel (kFused): each 256-thread workgroup owns a
+// TX x TY tile of cells, stages Z1, M1, N1, H of the tile plus a two-cell
+// halo in LDS (each HBM byte read once; the halo comes from neighbouring
+// tiles, which the XCD-aware tile order keeps in the same L2), computes
+// the new elevation Z2 (continuity, with the radiation boundary on the
+// outer ring of the outermost layer) for the tile plus the halo it needs,
+// then the new fluxes M2, N2 and the wet maximum / first arrival, and
+// writes Z2, M2, N2 once.  28 B/cell-update of HBM traffic (36 with the
+// maximum), against 68-80 for the reference's 8 launches.
+//
+// Layers with nested children need the parent's Z2 between continuity and
+// momentum (the child writes the footprint, JNZ), so they run kMassOnly,
+// the child sub-steps, then kMomentOnly.
+//
+// Arithmetic follows the reference Fortran operation by operation (no FMA
+// contraction: built with -ffp-contract=off; IEEE division and sqrt), so
+// results equal the FP32 Fortran bit for bit; pow/exp use detmath.h.
+#include <hip/hip_runtime.h>
+
+#include "comcot_consts.h"
+#include "detmath.h"
+#include "kernels.h"
+
+namespace gc {
+
+namespace {
+
+constexpr int kThreads = 256;
+
+template <int P>
+struct Lds {
+  float *z1, *m1, *n1, *h, *z2, *dz;
+  int gi0, gj0;  // global (1-based) index of LDS element 0
+  __device__ __forceinline__ int o(int i, int j) const { return (j - gj0) * P + (i - gi0); }
+  __device__ __forceinline__ float Z1(int i, int j) const { return z1[o(i, j)]; }
+  __device__ __forceinline__ float Z2(int i, int j) const { return z2[o(i, j)]; }
+  __device__ __forceinline__ float M(int i, int j) const { return m1[o(i, j)]; }
+  __device__ __forceinline__ float N(int i, int j) const { return n1[o(i, j)]; }
+  __device__ __forceinline__ float H(int i, int j) const { return h[o(i, j)]; }
+  __device__ __forceinline__ float DZ(int i, int j) const { return dz[o(i, j)]; }
+};
+
+__device__ __forceinline__ float cf(const DevLayer &L, int k, int j) {
+  return L.coef[k * L.ny + (j - 1)];
+}
+
+// PQ_DEPTH (initialization.f90:2512-2523)
+template <int P>
+__device__ __forceinline__ float hp_at(const DevLayer &L, const Lds<P> &s, int i, int j) {
+  const float h = s.H(i, j);
+  if (h > kGx) {
+    const int ip1 = i + 1 >= L.nx ? L.nx : i + 1;
+    return 0.5f * (h + s.H(ip1, j));
+  }
+  return 0.0f;
+}
+template <int P>
+__device__ __forceinline__ float hq_at(const DevLayer &L, const Lds<P> &s, int i, int j) {
+  const float h = s.H(i, j);
+  if (h > kGx) {
+    const int jp1 = j + 1 >= L.ny ? L.ny : j + 1;
+    return 0.5f * (h + s.H(i, jp1));
+  }
+  return 0.0f;
+}
+
+__device__ __forceinline__ size_t gidx(const DevLayer &L, int i, int j) {
+  return (size_t)(j - 1) * L.nx + (i - 1);
+}
+
+// OPEN (boundaries.f90:22-202) for one ring cell of the outermost layer.
+// The Fortran reads level-2 fluxes, which equal level 1 at that point of
+// the step (CHANGE copied them); the ring is written once per cell.
+template <int P>
+__device__ float open_cell(const DevLayer &L, const Lds<P> &s, int i, int j) {
+  const int nx = L.nx, ny = L.ny;
+  if ((i == 1 || i == nx) && (j == 1 || j == ny)) {
+    float qx, qy;
+    bool neg;
+    if (i == 1 && j == 1) {
+      qx = s.M(1, 1); qy = s.N(1, 1); neg = qx > 0.0f || qy > 0.0f;
+    } else if (i == nx && j == 1) {
+      qx = s.M(nx - 1, 1); qy = s.N(nx, 1); neg = qx < 0.0f || qy > 0.0f;
+    } else if (i == 1) {
+      qx = s.M(1, ny); qy = s.N(1, ny - 1); neg = qx > 0.0f || qy < 0.0f;
+    } else {
+      qx = s.M(nx - 1, ny); qy = s.N(nx, ny - 1); neg = qx < 0.0f || qy < 0.0f;
+    }
+    if (s.H(i, j) > kGx) {
+      float cc = sqrtf(kGrav * s.H(i, j));
+      float uh = sqrtf(qx * qx + qy * qy);
+      float zz = uh / cc;
+      if (neg) zz = -zz;
+      if (fabsf(zz) > kUB) zz = 0.0f;
+      return zz;
+    }
+    return 0.0f;
+  }
+  if (!(s.H(i, j) > kGx)) return 0.0f;
+  const float cc = sqrtf(kGrav * s.H(i, j));
+  float uh, uu;
+  bool neg;
+  if (j == 1) {
+    uh = 0.5f * (s.M(i, j) + s.M(i - 1, j));
+    float v = s.N(i, j);
+    uu = sqrtf(uh * uh + v * v);
+    neg = v > 0.0f;
+  } else if (j == ny) {
+    uh = 0.5f * (s.M(i, j) + s.M(i - 1, j));
+    float v = s.N(i, j - 1);
+    uu = sqrtf(uh * uh + v * v);
+    neg = v < 0.0f;
+  } else if (i == 1) {
+    if (s.H(i, j - 1) > kGx) uh = 0.5f * (s.N(i, j) + s.N(i, j - 1));
+    else uh = s.N(i, j);
+    float u = s.M(i, j);
+    uu = sqrtf(uh * uh + u * u);
+    neg = u > 0.0f;
+  } else {
+    uh = 0.5f * (s.N(i, j) + s.N(i, j - 1));
+    float u = s.M(i - 1, j);
+    uu = sqrtf(uh * uh + u * u);
+    neg = u < 0.0f;
+  }
+  float zz = uu / cc;
+  if (neg) zz = -zz;
+  if (fabsf(zz) > kUB) zz = 0.0f;
+  return zz;
+}
+
+// Continuity for one cell: MASS_S (mass.f90:61-100) / MASS_C (:103-157),
+// CONMASS_S (:227-277) / CONMASS (:162-224) when NL.  dz2 is the total
+// depth DZ(:,:,2) of the nonlinear forms (0 where not computed).
+template <bool NL, bool SPH, int P>
+__device__ __forceinline__ void mass_cell(const DevLayer &L, const Lds<P> &s, int i, int j,
+                                          float &z2, float &dz2) {
+  if (L.is_root && (i == 1 || j == 1 || i == L.nx || j == L.ny)) {
+    z2 = open_cell(L, s, i, j);
+    dz2 = 0.0f;
+    return;
+  }
+  if (i < 2 || j < 2) {  // ghost row/column of an inner layer
+    z2 = 0.0f;
+    dz2 = 0.0f;
+    return;
+  }
+  const float h = s.H(i, j);
+  float zz;
+  if (SPH) {
+    const float r1 = cf(L, kR1, j), r11 = cf(L, kR11, j);
+    const float r6 = cf(L, kR6, j), r6m = cf(L, kR6, j - 1);
+    zz = s.Z1(i, j) - r1 * (s.M(i, j) - s.M(i - 1, j)) -
+         r11 * (s.N(i, j) * r6 - s.N(i, j - 1) * r6m);
+  } else {
+    zz = s.Z1(i, j) - L.rx * (s.M(i, j) - s.M(i - 1, j)) - L.ry * (s.N(i, j) - s.N(i, j - 1));
+  }
+  if (fabsf(zz) < kEps) zz = 0.0f;
+  if (!NL) {
+    if (h > kGx) {
+      if ((zz + h) <= kEps) zz = -h;
+      z2 = zz;
+    } else {
+      z2 = 0.0f;
+    }
+    dz2 = 0.0f;
+  } else {
+    const bool active = SPH ? (h >= kElmax) : (h > kElmax);
+    if (active) {
+      float dd = zz + h;
+      if (dd >= kGx) { dz2 = dd; z2 = zz; }
+      else { dz2 = 0.0f; z2 = -h; }
+    } else {
+      dz2 = 0.0f;
+      z2 = -h;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- linear
+// MOMT_S x (moment.f90:86-115) / MOMT_C x (moment.f90:190-230)
+template <bool SPH, int P>
+__device__ float lin_x(const DevLayer &L, const Lds<P> &s, int i, int j) {
+  if (!(s.H(i, j) > kGx && s.H(i + 1, j) > kGx)) return 0.0f;
+  if (SPH) {
+    int jm1 = j - 1;
+    if (jm1 <= 1) jm1 = 1;
+    float tot_n = s.N(i, j) + s.N(i + 1, j) + s.N(i, jm1) + s.N(i + 1, jm1);
+    float r2 = L.r24 ? L.r24[gidx(L, i, j)] : cf(L, kF2, j) * fmaxf(hp_at(L, s, i, j), 0.0f);
+    float xm = s.M(i, j) - r2 * (s.Z2(i + 1, j) - s.Z2(i, j)) + cf(L, kR3, j) * tot_n;
+    if (fabsf(xm) < kEps) xm = 0.0f;
+    return xm;
+  } else {
+    int jm1 = j - 1, jp1 = j + 1;
+    if (jm1 < 1) jm1 = 1;
+    if (jp1 > L.ny) jp1 = L.ny;
+    float hm = hp_at(L, s, i, j) + 0.5f * (s.Z2(i, j) + s.Z2(i + 1, j));
+    float xm = s.M(i, j) - L.grx * hm * (s.Z2(i + 1, j) - s.Z2(i, j));
+    float ff = 0.0f;
+    if (L.fric) {
+      float xqq = 0.25f * (s.N(i, j) + s.N(i + 1, j) + s.N(i, jm1) + s.N(i + 1, jm1));
+      if (hm >= 1.0e-3f)
+        ff = L.kfric * sqrtf(s.M(i, j) * s.M(i, j) + xqq * xqq) / gcdet::powf_det(hm, kManExp);
+      else
+        ff = 0.0f;
+      xm = xm - ff * s.M(i, j);
+    }
+    if (L.modscm == 0) {
+      float scm = L.grx * kTwlvth * hm *
+                  ((s.Z2(i + 1, jp1) - 2.0f * s.Z2(i + 1, j) + s.Z2(i + 1, jm1)) -
+                   (s.Z2(i, jp1) - 2.0f * s.Z2(i, j) + s.Z2(i, jm1)));
+      xm = xm - scm;
+    }
+    if (L.fric) xm = xm / (1.0f + ff);
+    if (fabsf(xm) < kEps) xm = 0.0f;
+    return xm;
+  }
+}
+
+// MOMT_S y (moment.f90:117-146) / MOMT_C y (moment.f90:233-273)
+template <bool SPH, int P>
+__device__ float lin_y(const DevLayer &L, const Lds<P> &s, int i, int j) {
+  if (!(s.H(i, j) > kGx && s.H(i, j + 1) > kGx)) return 0.0f;
+  if (SPH) {
+    int im1 = i - 1;
+    if (im1 <= 1) im1 = 1;
+    float tot_m = s.M(im1, j) + s.M(im1, j + 1) + s.M(i, j) + s.M(i, j + 1);
+    float r4 = L.r24 ? L.r24[(size_t)L.nx * L.ny + gidx(L, i, j)]
+                     : cf(L, kF4, j) * fmaxf(hq_at(L, s, i, j), 0.0f);
+    float xn = s.N(i, j) - r4 * (s.Z2(i, j + 1) - s.Z2(i, j)) - cf(L, kR5, j) * tot_m;
+    if (fabsf(xn) < kEps) xn = 0.0f;
+    return xn;
+  } else {
+    int im1 = i - 1, ip1 = i + 1;
+    if (im1 < 1) im1 = 1;
+    if (ip1 >= L.nx) ip1 = L.nx;
+    float hn = hq_at(L, s, i, j) + 0.5f * (s.Z2(i, j) + s.Z2(i, j + 1));
+    float xn = s.N(i, j) - L.gry * hn * (s.Z2(i, j + 1) - s.Z2(i, j));
+    float ff = 0.0f;
+    if (L.fric) {
+      float xpp = 0.25f * (s.M(i, j) + s.M(i, j + 1) + s.M(im1, j) + s.M(im1, j + 1));
+      if (hn >= 1.0e-5f)
+        ff = L.kfric * sqrtf(s.N(i, j) * s.N(i, j) + xpp * xpp) / gcdet::powf_det(hn, kManExp);
+      else
+        ff = 0.0f;
+      xn = xn - ff * s.N(i, j);
+    }
+    if (L.modscm == 0) {
+      float scm = L.gry * kTwlvth * hn *
+                  ((s.Z2(ip1, j + 1) - 2.0f * s.Z2(i, j + 1) + s.Z2(im1, j + 1)) -
+                   (s.Z2(ip1, j) - 2.0f * s.Z2(i, j) + s.Z2(im1, j)));
+      xn = xn - scm;
+    }
+    if (L.fric) xn = xn / (1.0f + ff);
+    if (fabsf(xn) < kEps) xn = 0.0f;
+    return xn;
+  }
+}
+
+// ------------------------------------------------------------- nonlinear
+// Total depths at the flux points (moment.f90:345-366, 849-870); DZ at
+// level 1 is identically zero in the reference (never refreshed after
+// ALLOC), so DP(I,J,1) = 0.25*(DZ2(I)+DZ2(I+1)).
+template <int P>
+__device__ __forceinline__ float dp1_at(const DevLayer &L, const Lds<P> &s, int i, int j) {
+  const int ip1 = i + 1 >= L.nx ? L.nx : i + 1;
+  float v = 0.25f * (s.DZ(i, j) + s.DZ(ip1, j));
+  return v < kGx ? 0.0f : v;
+}
+template <int P>
+__device__ __forceinline__ float dp2_at(const DevLayer &L, const Lds<P> &s, int i, int j) {
+  const int ip1 = i + 1 >= L.nx ? L.nx : i + 1;
+  float v = 0.5f * (s.DZ(i, j) + s.DZ(ip1, j));
+  return v < kGx ? 0.0f : v;
+}
+template <int P>
+__device__ __forceinline__ float dq1_at(const DevLayer &L, const Lds<P> &s, int i, int j) {
+  const int jp1 = j + 1 >= L.ny ? L.ny : j + 1;
+  float v = 0.25f * (s.DZ(i, j) + s.DZ(i, jp1));
+  return v < kGx ? 0.0f : v;
+}
+template <int P>
+__device__ __forceinline__ float dq2_at(const DevLayer &L, const Lds<P> &s, int i, int j) {
+  const int jp1 = j + 1 >= L.ny ? L.ny : j + 1;
+  float v = 0.5f * (s.DZ(i, j) + s.DZ(i, jp1));
+  return v < kGx ? 0.0f : v;
+}
+
+// WAVE_BREAKING (moment.f90:1272-1384): damp the convection terms where
+// the surface rises faster than 0.08*sqrt(g*D).
+template <int P>
+__device__ __forceinline__ void wave_breaking(const DevLayer &L, const Lds<P> &s, int i, int j,
+                                              float &advx, float &advy) {
+  float c = sqrtf(kGrav * s.DZ(i, j));
+  float dzdt_f = 0.08f * c;
+  float dzdt_i = 0.65f * c;
+  float dzdt = fabsf(s.Z2(i, j) - s.Z1(i, j)) / L.dt;
+  if (dzdt > dzdt_f) {
+    float ref = (dzdt - dzdt_f) / (dzdt_i - dzdt_f);
+    float coef = gcdet::expf_det(-(1.0f * ref));
+    advx = coef * advx;
+    advy = coef * advy;
+  }
+}
+
+// X flux of CONMOME_S (moment.f90:889-1066, SPH) / CONMOME (:393-577)
+template <bool SPH, int P>
+__device__ float nl_x(const DevLayer &L, const Lds<P> &s, int i, int j) {
+  const int IX = L.nx, JY = L.ny;
+  int ip1 = i + 1, im1 = i - 1;
+  if (im1 <= 1) im1 = 1;
+  if (ip1 >= IX) ip1 = IX;
+  int jp1 = j + 1, jm1 = j - 1;
+  if (jm1 <= 1) jm1 = 1;
+  if (jp1 >= JY) jp1 = JY;
+  if (s.H(i, j) <= kElmax || hp_at(L, s, i, j) <= kElmax) return 0.0f;
+  float DD, DF;
+  const float dzi = s.DZ(i, j), dzp = s.DZ(ip1, j);
+  if (dzi > kGx && dzp > kGx) {
+    DD = dp2_at(L, s, i, j);
+    DF = dp1_at(L, s, i, j);
+  } else if (dzi > kGx && dzp <= kGx && s.H(ip1, j) + s.Z2(i, j) > kGx) {
+    DD = SPH ? hp_at(L, s, i, j) + 0.5f * (s.Z2(i, j) + s.Z2(ip1, j)) : 0.5f * dzi;
+    DF = DD;
+  } else if (dzi <= kGx && dzp > kGx && s.H(i, j) + s.Z2(ip1, j) > kGx) {
+    DD = SPH ? hp_at(L, s, i, j) + 0.5f * (s.Z2(i, j) + s.Z2(ip1, j)) : 0.5f * dzp;
+    DF = DD;
+  } else {
+    return 0.0f;
+  }
+  if (DD < kGx) return 0.0f;
+  if (DF < 1.0e-3f) DF = 1.0e-3f;
+  const float p = s.M(i, j);
+  const float xqq = 0.25f * (s.N(i, j) + s.N(ip1, j) + s.N(i, jm1) + s.N(ip1, jm1));
+  float FF = 0.0f;
+  if (L.fric) FF = L.kfric * sqrtf(p * p + xqq * xqq) / gcdet::powf_det(DF, kManExp);
+  float XP;
+  if (SPH) {
+    const float r2 = cf(L, kF2, j);
+    if (L.modscm == 0 && j < JY) {
+      XP = p - r2 * DD * (s.Z2(ip1, j) - s.Z2(i, j)) -
+           r2 * DD / 12.0f *
+               ((s.Z2(ip1, jp1) - 2.0f * s.Z2(ip1, j) + s.Z2(ip1, jm1)) -
+                (s.Z2(i, jp1) - 2.0f * s.Z2(i, j) + s.Z2(i, jm1)));
+    } else {
+      XP = p - r2 * DD * (s.Z2(ip1, j) - s.Z2(i, j));
+    }
+    if (L.fric) XP = XP - FF * p;
+    XP = XP + 4.0f * cf(L, kR3, j) * xqq;
+  } else {
+    XP = p - L.grx * DD * (s.Z2(ip1, j) - s.Z2(i, j));
+    if (L.modscm == 0 && j < JY) {
+      float scm = L.grx * DD / 12.0f *
+                  ((s.Z2(ip1, jp1) - 2.0f * s.Z2(ip1, j) + s.Z2(ip1, jm1)) -
+                   (s.Z2(i, jp1) - 2.0f * s.Z2(i, j) + s.Z2(i, jm1)));
+      XP = XP - scm;
+    }
+    if (L.fric) XP = XP - FF * p;
+  }
+  if (L.linchk == 1 && dp1_at(L, s, i, j) >= kGx &&
+      (i > kSpan && i < IX - kSpan && j > kSpan && j < JY - kSpan)) {
+    const float fx = SPH ? cf(L, kR21, j) : L.rx;
+    const float fy = SPH ? cf(L, kR0, j) : L.ry;
+    const float d = dp1_at(L, s, i, j);
+    float advx, advy;
+    if (p < 0.0f) {
+      if (dp1_at(L, s, ip1, j) < kGx || s.DZ(ip1, j) < kGx) {
+        advx = fx * (-(p * p / d));
+      } else {
+        float pp = s.M(ip1, j);
+        advx = fx * (pp * pp / dp1_at(L, s, ip1, j) - p * p / d);
+      }
+    } else {
+      if (dp1_at(L, s, im1, j) < kGx || s.DZ(i, j) < kGx) {
+        advx = fx * (p * p / d);
+      } else {
+        float pm = s.M(im1, j);
+        advx = fx * (p * p / d - pm * pm / dp1_at(L, s, im1, j));
+      }
+    }
+    if (xqq < 0.0f) {
+      if (s.DZ(i, jp1) < kGx || s.DZ(ip1, jp1) < kGx) {
+        advy = fy * (-(p * xqq / d));
+      } else if (dp1_at(L, s, i, jp1) < kGx) {
+        advy = fy * (-(p * xqq / d));
+      } else {
+        float xqe = 0.25f * (s.N(i, jp1) + s.N(ip1, jp1) + s.N(i, j) + s.N(ip1, j));
+        advy = fy * (s.M(i, jp1) * xqe / dp1_at(L, s, i, jp1) - p * xqq / d);
+      }
+    } else {
+      if (s.DZ(i, jm1) < kGx || s.DZ(ip1, jm1) < kGx) {
+        advy = fy * (p * xqq / d);
+      } else if (dp1_at(L, s, i, jm1) < kGx) {
+        advy = fy * (p * xqq / d);
+      } else {
+        float xqe = 0.25f * (s.N(i, jm1) + s.N(ip1, jm1) + s.N(i, j - 2) + s.N(ip1, j - 2));
+        advy = fy * (p * xqq / d - s.M(i, jm1) * xqe / dp1_at(L, s, i, jm1));
+      }
+    }
+    wave_breaking(L, s, i, j, advx, advy);
+    XP = XP - advx - advy;
+  }
+  if (L.fric) XP = XP / (1.0f + FF);
+  if (fabsf(XP) < kEps) XP = 0.0f;
+  const float lim = kVLimit * DD;
+  if (XP > lim) XP = lim;
+  if (XP < -lim) XP = -lim;
+  return XP;
+}
+
+// Y flux of CONMOME_S (moment.f90:1071-1258, SPH) / CONMOME (:582-770).
+// CONMOME's Y equation subtracts its friction term from the X variable
+// (moment.f90:672), so only the 1/(1+FF) factor reaches XQ there.
+template <bool SPH, int P>
+__device__ float nl_y(const DevLayer &L, const Lds<P> &s, int i, int j) {
+  const int IX = L.nx, JY = L.ny;
+  int ip1 = i + 1, im1 = i - 1;
+  if (im1 <= 1) im1 = 1;
+  if (ip1 >= IX) ip1 = IX;
+  int jp1 = j + 1, jm1 = j - 1;
+  if (jm1 <= 1) jm1 = 1;
+  if (jp1 >= JY) jp1 = JY;
+  if (s.H(i, j) <= kElmax || hq_at(L, s, i, j) <= kElmax) return 0.0f;
+  float DD, DF;
+  const float dzi = s.DZ(i, j), dzp = s.DZ(i, jp1);
+  if (dzi > kGx && dzp > kGx) {
+    DD = dq2_at(L, s, i, j);
+    DF = dq1_at(L, s, i, j);
+  } else if (dzi > kGx && dzp <= kGx && s.H(i, jp1) + s.Z2(i, j) > kGx) {
+    DD = SPH ? hq_at(L, s, i, j) + 0.5f * (s.Z2(i, j) + s.Z2(i, jp1)) : 0.5f * dzi;
+    DF = DD;
+  } else if (dzi <= kGx && dzp > kGx && s.H(i, j) + s.Z2(i, jp1) > kGx) {
+    DD = SPH ? hq_at(L, s, i, j) + 0.5f * (s.Z2(i, j) + s.Z2(i, jp1)) : 0.5f * dzp;
+    DF = DD;
+  } else {
+    return 0.0f;
+  }
+  if (DD < kGx) return 0.0f;
+  if (DF < 1.0e-3f) DF = 1.0e-3f;
+  const float q = s.N(i, j);
+  const float xpp = 0.25f * (s.M(i, j) + s.M(i, jp1) + s.M(im1, j) + s.M(im1, jp1));
+  float FF = 0.0f;
+  if (L.fric) FF = L.kfric * sqrtf(q * q + xpp * xpp) / gcdet::powf_det(DF, kManExp);
+  float XQ;
+  if (SPH) {
+    const float r4 = cf(L, kF4, j);
+    if (L.modscm == 0 && i < IX) {
+      XQ = q - r4 * DD * (s.Z2(i, jp1) - s.Z2(i, j)) -
+           r4 * DD / 12.0f *
+               ((s.Z2(ip1, jp1) - 2.0f * s.Z2(i, jp1) + s.Z2(im1, jp1)) -
+                (s.Z2(ip1, j) - 2.0f * s.Z2(i, j) + s.Z2(im1, j)));
+    } else {
+      XQ = q - r4 * DD * (s.Z2(i, jp1) - s.Z2(i, j));
+    }
+    if (L.fric) XQ = XQ - FF * q;
+    XQ = XQ - 4.0f * cf(L, kR5, j) * xpp;
+  } else {
+    XQ = q - L.gry * DD * (s.Z2(i, jp1) - s.Z2(i, j));
+    if (L.modscm == 0 && i < IX) {
+      float scm = L.gry * DD / 12.0f *
+                  ((s.Z2(ip1, jp1) - 2.0f * s.Z2(i, jp1) + s.Z2(im1, jp1)) -
+                   (s.Z2(ip1, j) - 2.0f * s.Z2(i, j) + s.Z2(im1, j)));
+      XQ = XQ - scm;
+    }
+  }
+  if (L.linchk == 1 && dq1_at(L, s, i, j) >= kGx &&
+      (i > kSpan && i < IX - kSpan && j > kSpan && j < JY - kSpan)) {
+    const float fy = SPH ? cf(L, kR0, j) : L.ry;
+    const float fx = SPH ? cf(L, kR22, j) : L.rx;
+    const float d = dq1_at(L, s, i, j);
+    float advx, advy;
+    if (q < 0.0f) {
+      if (dq1_at(L, s, i, jp1) < kGx || s.DZ(i, jp1) < kGx) {
+        advy = fy * (-(q * q / d));
+      } else {
+        float qq = s.N(i, jp1);
+        advy = fy * (qq * qq / dq1_at(L, s, i, jp1) - q * q / d);
+      }
+    } else {
+      if (dq1_at(L, s, i, jm1) < kGx || s.DZ(i, j) < kGx) {
+        advy = fy * (q * q / d);
+      } else {
+        float qm = s.N(i, jm1);
+        advy = fy * (q * q / d - qm * qm / dq1_at(L, s, i, jm1));
+      }
+    }
+    if (xpp < 0.0f) {
+      if (s.DZ(ip1, j) < kGx || s.DZ(ip1, jp1) < kGx) {
+        advx = fx * (-(q * xpp / d));
+      } else if (dq1_at(L, s, ip1, j) < kGx) {
+        advx = fx * (-(q * xpp / d));
+      } else {
+        float xpe = 0.25f * (s.M(ip1, j) + s.M(ip1, jp1) + s.M(i, j) + s.M(i, jp1));
+        advx = fx * (s.N(ip1, j) * xpe / dq1_at(L, s, ip1, j) - q * xpp / d);
+      }
+    } else {
+      if (s.DZ(im1, j) < kGx || s.DZ(im1, jp1) < kGx) {
+        advx = fx * (q * xpp / d);
+      } else if (dq1_at(L, s, i - 1, j) < kGx) {
+        advx = fx * (q * xpp / d);
+      } else {
+        float xpe = 0.25f * (s.M(im1, j) + s.M(im1, jp1) + s.M(i - 2, j) + s.M(i - 2, jp1));
+        advx = fx * (q * xpp / d - s.N(im1, j) * xpe / dq1_at(L, s, im1, j));
+      }
+    }
+    wave_breaking(L, s, i, j, advx, advy);
+    XQ = XQ - advx - advy;
+  }
+  if (L.fric) XQ = XQ / (1.0f + FF);
+  if (fabsf(XQ) < kEps) XQ = 0.0f;
+  const float lim = kVLimit * DD;
+  if (XQ > lim) XQ = lim;
+  if (XQ < -lim) XQ = -lim;
+  return XQ;
+}
+
+// Tile -> workgroup order: workgroup b runs on XCD b%8; XCD x gets the
+// contiguous run of tiles [x*per, (x+1)*per), so vertically adjacent tiles
+// (whose halos overlap) share that XCD's L2.
+__device__ __forceinline__ int tile_of_block(int per_xcd) {
+  const int b = blockIdx.x;
+  return (b & 7) * per_xcd + (b >> 3);
+}
+
+template <int MODE, bool NL, bool SPH, int TX, int TY>
+__global__ void __launch_bounds__(kThreads)
+    k_step(DevLayer L, int tiles_x, int ntiles, int per_xcd) {
+  constexpr int P = TX + 4, R = TY + 4, A = P * R;
+  __shared__ float sm[6 * A];
+  const int t = tile_of_block(per_xcd);
+  if (t >= ntiles) return;
+  const int ti0 = (t % tiles_x) * TX + 1, tj0 = (t / tiles_x) * TY + 1;
+  Lds<P> s{sm, sm + A, sm + 2 * A, sm + 3 * A, sm + 4 * A, sm + 5 * A, ti0 - 2, tj0 - 2};
+  const int nx = L.nx, ny = L.ny, tid = threadIdx.x;
+
+  // ---- stage inputs with a two-cell halo
+  for (int idx = tid; idx < A; idx += kThreads) {
+    const int gi = ti0 - 2 + idx % P, gj = tj0 - 2 + idx / P;
+    const bool in = gi >= 1 && gi <= nx && gj >= 1 && gj <= ny;
+    const size_t g = in ? (size_t)(gj - 1) * nx + (gi - 1) : 0;
+    const float h = in ? L.h[g] : 0.0f;
+    s.h[idx] = h;
+    s.m1[idx] = in ? L.m1[g] : 0.0f;
+    s.n1[idx] = in ? L.n1[g] : 0.0f;
+    if (MODE != kMomentOnly || NL) s.z1[idx] = in ? L.z1[g] : 0.0f;
+    if (MODE == kMomentOnly) {
+      const float z2 = in ? L.z2[g] : 0.0f;
+      s.z2[idx] = z2;
+      if (NL) {
+        float dz = 0.0f;
+        if (in) {
+          if (L.dzp) {
+            dz = L.dzp[g];
+          } else {
+            const bool massed = L.is_root ? (gi >= 2 && gi <= nx - 1 && gj >= 2 && gj <= ny - 1)
+                                          : (gi >= 2 && gj >= 2);
+            dz = massed ? z2 + h : 0.0f;
+          }
+        }
+        s.dz[idx] = dz;
+      }
+    } else {
+      s.z2[idx] = 0.0f;
+      s.dz[idx] = 0.0f;
+    }
+  }
+  __syncthreads();
+
+  // ---- continuity (+ open boundary) on the tile and the halo it needs
+  if (MODE != kMomentOnly) {
+    constexpr int CX = MODE == kFused ? TX + 3 : TX;
+    constexpr int CY = MODE == kFused ? TY + 3 : TY;
+    const int ci0 = MODE == kFused ? ti0 - 1 : ti0;
+    const int cj0 = MODE == kFused ? tj0 - 1 : tj0;
+    for (int idx = tid; idx < CX * CY; idx += kThreads) {
+      const int i = ci0 + idx % CX, j = cj0 + idx / CX;
+      if (i < 1 || i > nx || j < 1 || j > ny) continue;
+      float z2, dz2;
+      mass_cell<NL, SPH, P>(L, s, i, j, z2, dz2);
+      s.z2[s.o(i, j)] = z2;
+      if (NL) s.dz[s.o(i, j)] = dz2;
+    }
+    __syncthreads();
+  }
+
+  // ---- outputs of the tile
+  int step = 0;
+  if (MODE != kMassOnly && L.zmax) {
+    step = L.ctl[L.ctl_in] + 1;
+    if (L.advance && t == 0 && tid == 0) L.ctl[1 - L.ctl_in] = step;
+  }
+  for (int idx = tid; idx < TX * TY; idx += kThreads) {
+    const int i = ti0 + idx % TX, j = tj0 + idx / TX;
+    if (i > nx || j > ny) continue;
+    const size_t g = (size_t)(j - 1) * nx + (i - 1);
+    if (MODE != kMomentOnly) {
+      float zo;
+      if (L.is_root || (i >= 2 && j >= 2)) zo = s.Z2(i, j);
+      else if (i == 1 && j == 1) zo = s.Z2(2, 2);  // UPDATE ghosts, all_grids.f90:323-324
+      else if (i == 1) zo = s.Z2(2, j);
+      else zo = s.Z2(i, 2);
+      L.z2[g] = zo;
+      if (MODE == kMassOnly && NL && L.dzp) L.dzp[g] = s.DZ(i, j);
+    }
+    if (MODE != kMassOnly) {
+      const bool r = L.is_root;
+      if (NL) {
+        // CONMOME(_S) ranges (moment.f90:377-387, 583-593, 873-883, 1072-1082)
+        const bool mx = r ? (i <= nx - 1 && j <= ny - 1) : (i >= 2 && i <= nx - 1 && j >= 2);
+        const bool my = r ? (i <= nx - 1 && j <= ny - 1) : (i >= 2 && j >= 2 && j <= ny - 1);
+        if (mx) L.m2[g] = nl_x<SPH, P>(L, s, i, j);
+        if (my) L.n2[g] = nl_y<SPH, P>(L, s, i, j);
+      } else {
+        // MOMT_S / MOMT_C ranges (moment.f90:78-88, 117-119, 174-192, 233-235)
+        const int lo = r ? 1 : 2;
+        if (i >= lo && i <= nx - 1 && j >= lo) L.m2[g] = lin_x<SPH, P>(L, s, i, j);
+        if (j >= lo && j <= ny - 1 && i >= lo) L.n2[g] = lin_y<SPH, P>(L, s, i, j);
+      }
+      if (L.zmax && (r || (i >= 2 && j >= 2))) {
+        const float z2 = s.Z2(i, j);
+        if (s.H(i, j) + z2 > kGx) {
+          const float zm = L.zmax[g];
+          if (z2 >= L.arr_thr && !(zm >= L.arr_thr)) L.arrival[g] = step;
+          if (z2 > zm) L.zmax[g] = z2;
+        }
+      }
+    }
+  }
+}
+
+// Reference maximum recorder (GPUOutput.cu:46-53 semantics: no wet test)
+// plus the wet maximum / arrival of the extension, for maxamp_launch_.
+__global__ void k_max(DevLayer L, float *zmax_raw) {
+  const size_t n = (size_t)L.nx * L.ny;
+  const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int step = L.ctl[L.ctl_in] + 1;
+  if (g == 0 && L.advance) L.ctl[1 - L.ctl_in] = step;
+  if (g >= n) return;
+  const int i = (int)(g % L.nx) + 1, j = (int)(g / L.nx) + 1;
+  const float z2 = L.z2[g];
+  if (zmax_raw[g] < z2) zmax_raw[g] = z2;
+  if (L.zmax && (L.is_root || (i >= 2 && j >= 2)) && L.h[g] + z2 > kGx) {
+    const float zm = L.zmax[g];
+    if (z2 >= L.arr_thr && !(zm >= L.arr_thr)) L.arrival[g] = step;
+    if (z2 > zm) L.zmax[g] = z2;
+  }
+}
+
+// ---------------------------------------------------------------- nesting
+__device__ __forceinline__ float pidx(const DevNest &N, const float *a, int i, int j) {
+  return a[(size_t)(j - 1) * N.pnx + (i - 1)];
+}
+
+// Parent-side forecast of the interface flux at line index k (NEWQ at
+// sub-step 2, all_grids.f90:706-720 and 778-790): one step of the
+// parent's linear momentum, spherical (R2/R3 or R4/R5) or Cartesian
+// (GRX*HM or GRY*HN).  side 0/1: lines I = IS-1 / IE (M); 2/3: J = JS-1 /
+// JE (N).
+__device__ float iface_forecast(const DevNest &N, int side, int k) {
+  const bool vert = side < 2;
+  float x;
+  if (vert) {
+    const int i = side == 0 ? N.is - 1 : N.ie, j = k, ip1 = i + 1, jm1 = j - 1;
+    const float hpc = pidx(N, N.ph, i, j) > kGx
+                          ? 0.5f * (pidx(N, N.ph, i, j) + pidx(N, N.ph, ip1 >= N.pnx ? N.pnx : ip1, j))
+                          : 0.0f;
+    if (N.p_sph) {
+      float tot_n = pidx(N, N.pn1, i, j) + pidx(N, N.pn1, ip1, j) + pidx(N, N.pn1, i, jm1) +
+                    pidx(N, N.pn1, ip1, jm1);
+      const float f2 = N.pcoef[kF2 * N.pny + (j - 1)];
+      float r2 = N.p_r24 ? pidx(N, N.pr24, i, j) : (N.p_nl ? f2 : f2 * fmaxf(hpc, 0.0f));
+      x = pidx(N, N.pm1, i, j) - r2 * (pidx(N, N.pz2, ip1, j) - pidx(N, N.pz2, i, j)) +
+          N.pcoef[kR3 * N.pny + (j - 1)] * tot_n;
+    } else {
+      float hm = hpc + 0.5f * (pidx(N, N.pz2, i, j) + pidx(N, N.pz2, ip1, j));
+      x = pidx(N, N.pm1, i, j) - N.pgrx * hm * (pidx(N, N.pz2, ip1, j) - pidx(N, N.pz2, i, j));
+    }
+  } else {
+    const int i = k, j = side == 2 ? N.js - 1 : N.je, jp1 = j + 1, im1 = i - 1;
+    const float hqc = pidx(N, N.ph, i, j) > kGx
+                          ? 0.5f * (pidx(N, N.ph, i, j) + pidx(N, N.ph, i, jp1 >= N.pny ? N.pny : jp1))
+                          : 0.0f;
+    if (N.p_sph) {
+      float tot_m = pidx(N, N.pm1, im1, j) + pidx(N, N.pm1, im1, jp1) + pidx(N, N.pm1, i, j) +
+                    pidx(N, N.pm1, i, jp1);
+      const float f4 = N.pcoef[kF4 * N.pny + (j - 1)];
+      float r4 = N.p_r24 ? N.pr24[(size_t)N.pnx * N.pny + (size_t)(j - 1) * N.pnx + (i - 1)]
+                         : (N.p_nl ? f4 : f4 * fmaxf(hqc, 0.0f));
+      x = pidx(N, N.pn1, i, j) - r4 * (pidx(N, N.pz2, i, jp1) - pidx(N, N.pz2, i, j)) -
+          N.pcoef[kR5 * N.pny + (j - 1)] * tot_m;
+    } else {
+      float hn = hqc + 0.5f * (pidx(N, N.pz2, i, j) + pidx(N, N.pz2, i, jp1));
+      x = pidx(N, N.pn1, i, j) - N.pgry * hn * (pidx(N, N.pz2, i, jp1) - pidx(N, N.pz2, i, j));
+    }
+  }
+  if (fabsf(x) < kEps) x = 0.0f;
+  return x;
+}
+
+// Both parent cells of the interface face at line index k wet in Z2?
+__device__ __forceinline__ bool iface_wet(const DevNest &N, int side, int k) {
+  const bool vert = side < 2;
+  const int fi = vert ? (side == 0 ? N.is - 1 : N.ie) : k;
+  const int fj = vert ? k : (side == 2 ? N.js - 1 : N.je);
+  const int ai = vert ? fi + 1 : fi, aj = vert ? fj : fj + 1;
+  return (pidx(N, N.ph, fi, fj) + pidx(N, N.pz2, fi, fj) > kGx) &&
+         (pidx(N, N.ph, ai, aj) + pidx(N, N.pz2, ai, aj) > kGx);
+}
+
+// Parent flux on the interface line at index k for sub-step t: JNQ takes
+// the parent's level-1 flux as is (all_grids.f90:484-487); NEWQ blends the
+// cached forecast with it where both parent cells are wet and is 0
+// elsewhere (all_grids.f90:703-727, 774-796).
+__device__ float iface_flux(const DevNest &N, int side, int k, int t) {
+  const bool vert = side < 2;
+  const int fi = vert ? (side == 0 ? N.is - 1 : N.ie) : k;
+  const int fj = vert ? k : (side == 2 ? N.js - 1 : N.je);
+  const float f1 = vert ? pidx(N, N.pm1, fi, fj) : pidx(N, N.pn1, fi, fj);
+  if (t == 1) return f1;
+  if (!iface_wet(N, side, k)) return 0.0f;
+  const float c1 = ((float)t - 1.0f) / (float)N.rel_time;
+  const float c2 = 1.0f - c1;
+  const float cached = t == 2 ? iface_forecast(N, side, k)
+                              : (vert ? N.yflux[(side & 1) * N.pny + (k - 1)]
+                                      : N.xflux[(side & 1) * N.pnx + (k - 1)]);
+  return c1 * cached + c2 * f1;
+}
+
+// JNQ / NEWQ for one child: threads over the four interface lines.  A
+// thread owns parent index k of one side: it writes the forecast cache at
+// k (t == 2, k in the line's full range JS-2..JE+2 / IS-2..IE+2) and, for
+// k in the footprint, the R child boundary fluxes of EDGEINTERP_VERT/HORI
+// (all_grids.f90:502-604), from the interface flux at k-1, k, k+1.
+__global__ void k_jnq(DevNest N, int t) {
+  const int lv = N.je - N.js + 5, lh = N.ie - N.is + 5;  // k ranges incl. +-2
+  int g = blockIdx.x * blockDim.x + threadIdx.x;
+  int side;
+  if (g < 2 * lv) { side = g / lv; g -= side * lv; }
+  else { g -= 2 * lv; if (g >= 2 * lh) return; side = 2 + g / lh; g -= (side - 2) * lh; }
+  const bool vert = side < 2;
+  const int k = (vert ? N.js : N.is) - 2 + g;
+  if (t == 2 && iface_wet(N, side, k)) {
+    const float x = iface_forecast(N, side, k);
+    if (vert) N.yflux[(side & 1) * N.pny + (k - 1)] = x;
+    else N.xflux[(side & 1) * N.pnx + (k - 1)] = x;
+  }
+  const int lo = vert ? N.js : N.is, hi = vert ? N.je : N.ie;
+  if (k < lo || k > hi) return;
+  const float fm = iface_flux(N, side, k - 1, t);
+  const float f0 = iface_flux(N, side, k, t);
+  const float fp = iface_flux(N, side, k + 1, t);
+  const float fs = 0.5f * (fm + f0);
+  const float fe = 0.5f * (fp + f0);
+  const int nn = 2 * N.rel_size;
+  const float c1 = (fe - fs) / (float)nn;  // == REAL((fe-fs)/DBLE(nn)), all_grids.f90:538
+  const float c2 = -c1 + fs;
+  const int base = (k - lo) * N.rel_size + 1;
+  for (int kk = 1; kk <= N.rel_size; ++kk) {
+    const int c = base + kk;
+    int ci, cj;
+    if (vert) { ci = side == 0 ? 1 : N.cnx; cj = c; }
+    else { ci = c; cj = side == 2 ? 1 : N.cny; }
+    const size_t cg = (size_t)(cj - 1) * N.cnx + (ci - 1);
+    float v = 2.0f * (float)kk * c1 + c2;
+    if (N.ch[cg] + N.cz1[cg] <= kGx) v = 0.0f;
+    if (vert) N.cm1[cg] = v;
+    else N.cn1[cg] = v;
+  }
+}
+
+// JNZ (all_grids.f90:342-421): parent Z2 over the footprint = mean of the
+// wet child cells, summed in the Fortran order (KI outer, KJ inner).
+__global__ void k_jnz(DevNest N, float *pz2_out) {
+  const int w = N.ie - N.is + 1, hgt = N.je - N.js + 1;
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= w * hgt) return;
+  const int i = N.is + g % w, j = N.js + g / w;
+  const int ir = N.rel_size;
+  const float half = (float)(ir * ir) / 2.0f;
+  const int i0 = (i - N.is) * ir + 1, j0 = (j - N.js) * ir + 1;
+  const bool even = (N.rel_time % 2) == 0;
+  float sum = 0.0f, count = 0.0f;
+  for (int ki = 1; ki <= ir; ++ki) {
+    for (int kj = 1; kj <= ir; ++kj) {
+      const size_t cg = (size_t)(j0 + kj - 1) * N.cnx + (i0 + ki - 1);
+      const float z2 = N.cz2[cg];
+      if (N.ch[cg] + z2 > kGx) {
+        if (even) sum = sum + 0.5f * (N.cz1[cg] + z2);
+        else sum = sum + z2;
+        count = count + 1.0f;
+      }
+    }
+  }
+  pz2_out[(size_t)(j - 1) * N.pnx + (i - 1)] = count > half ? sum / count : 0.0f;
+}
+
+template <int MODE, bool NL, bool SPH, int TX, int TY>
+hipError_t launch_t(const DevLayer &L, hipStream_t st) {
+  const int tx = (L.nx + TX - 1) / TX, ty = (L.ny + TY - 1) / TY;
+  const int nt = tx * ty, per = (nt + 7) / 8;
+  hipLaunchKernelGGL((k_step<MODE, NL, SPH, TX, TY>), dim3(8 * per), dim3(kThreads), 0, st, L,
+                     tx, nt, per);
+  return hipGetLastError();
+}
+
+template <int TX, int TY>
+hipError_t launch_shape(const DevLayer &L, int mode, bool nl, bool sph, hipStream_t st) {
+#define GC_CASE(MD, A, B)                                  \
+  if (mode == MD && nl == A && sph == B) return launch_t<MD, A, B, TX, TY>(L, st);
+  GC_CASE(kFused, false, true) GC_CASE(kFused, false, false)
+  GC_CASE(kFused, true, true) GC_CASE(kFused, true, false)
+  GC_CASE(kMassOnly, false, true) GC_CASE(kMassOnly, false, false)
+  GC_CASE(kMassOnly, true, true) GC_CASE(kMassOnly, true, false)
+  GC_CASE(kMomentOnly, false, true) GC_CASE(kMomentOnly, false, false)
+  GC_CASE(kMomentOnly, true, true) GC_CASE(kMomentOnly, true, false)
+#undef GC_CASE
+  return hipErrorInvalidValue;
+}
+
+const int kVariants[][2] = {{64, 16}, {64, 8}, {128, 8}, {32, 32}, {64, 32}};
+
+}  // namespace
+
+int num_tile_variants() { return (int)(sizeof(kVariants) / sizeof(kVariants[0])); }
+void tile_variant_shape(int v, int *tx, int *ty) {
+  *tx = kVariants[v][0];
+  *ty = kVariants[v][1];
+}
+
+hipError_t launch_step(const DevLayer &L, int mode, bool nl, bool sph, int variant,
+                       hipStream_t st) {
+  switch (variant) {
+    case 0: return launch_shape<64, 16>(L, mode, nl, sph, st);
+    case 1: return launch_shape<64, 8>(L, mode, nl, sph, st);
+    case 2: return launch_shape<128, 8>(L, mode, nl, sph, st);
+    case 3: return launch_shape<32, 32>(L, mode, nl, sph, st);
+    case 4: return launch_shape<64, 32>(L, mode, nl, sph, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_max(const DevLayer &L, float *zmax_raw, hipStream_t st) {
+  const size_t n = (size_t)L.nx * L.ny;
+  hipLaunchKernelGGL(k_max, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, L, zmax_raw);
+  return hipGetLastError();
+}
+
+hipError_t launch_jnq(const DevNest &N, int t, hipStream_t st) {
+  const int total = 2 * (N.je - N.js + 5) + 2 * (N.ie - N.is + 5);
+  hipLaunchKernelGGL(k_jnq, dim3((total + 127) / 128), dim3(128), 0, st, N, t);
+  return hipGetLastError();
+}
+
+hipError_t launch_jnz(const DevNest &N, float *pz2_out, hipStream_t st) {
+  const int total = (N.ie - N.is + 1) * (N.je - N.js + 1);
+  hipLaunchKernelGGL(k_jnz, dim3((total + 127) / 128), dim3(128), 0, st, N, pz2_out);
+  return hipGetLastError();
+}
+
+}  // namespace gc

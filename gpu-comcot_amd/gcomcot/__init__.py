@@ -1,0 +1,13 @@
+"""gcomcot — COMCOT's shallow-water step on MI355X through a C ABI.
+
+The compute lives in libgcomcot_hip.so (HIP kernels for gfx950 plus the
+reference's C entry points); this package is the host mirror used by the
+tests and the benchmark: scenario generation, layer registration, the
+step loop and state access.
+"""
+from ._lib import GcomcotError, LIB_PATH, lib
+from .scenarios import LayerSpec, Scenario, config0, config1, config2
+from .engine import DeviceRun
+
+__all__ = ["GcomcotError", "LIB_PATH", "lib", "LayerSpec", "Scenario", "config0", "config1",
+           "config2", "DeviceRun"]

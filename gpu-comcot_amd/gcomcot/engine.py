@@ -1,0 +1,150 @@
+"""Host mirror of the COMCOT driver's layer loop over the HIP C ABI.
+
+DeviceRun registers a Scenario's layers with gcomcot_init_layer_ex_ (or,
+for spherical linear layers, the reference entry gcomcot_init_layer_ with
+full PARA planes) and drives the step either as the whole-run device loop
+(gcomcot_run_steps_) or call by call through the reference entry points in
+the order of the Fortran main loop (comcot.f90:359-439), which is what the
+unmodified Fortran driver does when linked against libgcomcot_hip.so.
+"""
+import ctypes
+
+import numpy as np
+
+from . import scenarios
+from ._lib import GcomcotError, arr, check, fp, ip, lib
+
+
+def _plane(ny, nx):
+    return np.empty((ny, nx), dtype=np.float32)
+
+
+def para_planes(spec):
+    """The (NX,NY) planes PARA hands to gcomcot_init_layer_ (R1..R5, R6, R11)."""
+    c = scenarios.para_rows(spec.laygov, spec.dt, spec.dx, spec.y, spec.dely).reshape(10, spec.ny)
+    h = spec.h
+    gx = np.float32(1.0e-5)
+    hp = np.zeros_like(h)
+    hq = np.zeros_like(h)
+    wet = h > gx
+    hn = np.concatenate([h[:, 1:], h[:, -1:]], axis=1)   # IP1 clamped at NX
+    hu = np.concatenate([h[1:, :], h[-1:, :]], axis=0)   # JP1 clamped at NY
+    hp[wet] = (np.float32(0.5) * (h + hn))[wet]
+    hq[wet] = (np.float32(0.5) * (h + hu))[wet]
+    row = lambda k: np.repeat(c[k][:, None], spec.nx, axis=1).astype(np.float32)
+    r2 = (row(3) * np.maximum(hp, np.float32(0))).astype(np.float32)
+    r4 = (row(7) * np.maximum(hq, np.float32(0))).astype(np.float32)
+    return dict(R1=row(1), R2=r2, R3=row(6), R4=r4, R5=row(8), R6=c[9].astype(np.float32),
+                R11=row(2))
+
+
+class DeviceRun:
+    """One scenario resident on the GPU, driven through the C ABI."""
+
+    def __init__(self, scenario, reference_init=False, arrival_threshold=0.01):
+        self.scenario = scenario
+        L = lib()
+        L.cuda_shutdown_()
+        L.gcomcot_init_gpu_()
+        L.gcomcot_set_arrival_threshold_(fp(arrival_threshold))
+        self.specs = {}
+        for s in scenario.layers:
+            self.specs[s.id] = s
+            corners = np.array(s.corners, dtype=np.int32)
+            ha, hp = arr(s.h)
+            za, zp = arr(s.z0)
+            if reference_init:
+                if s.laycord != 0 or s.laygov != 0:
+                    raise GcomcotError("the reference init entry only takes spherical linear layers")
+                P = para_planes(s)
+                keep = {k: arr(v) for k, v in P.items()}
+                L.gcomcot_init_layer_(ip(s.id), ip(s.parent), ip(s.level), ip(s.rel_size),
+                                      ip(s.rel_time), corners.ctypes.data_as(ctypes.c_void_p),
+                                      fp(0), fp(0), keep["R1"][1], keep["R2"][1], keep["R3"][1],
+                                      keep["R4"][1], keep["R5"][1], keep["R6"][1],
+                                      keep["R11"][1], hp, zp, ip(s.nx), ip(s.ny))
+                continue
+            if s.laycord == 0:
+                coef = scenarios.para_rows(s.laygov, s.dt, s.dx, s.y, s.dely)
+                cptr = coef.ctypes.data_as(ctypes.c_void_p)
+            else:
+                coef = None
+                cptr = None
+            rc = L.gcomcot_init_layer_ex_(ip(s.id), ip(s.parent), ip(s.level), ip(s.laycord),
+                                          ip(s.laygov), ip(s.rel_size), ip(s.rel_time),
+                                          corners.ctypes.data_as(ctypes.c_void_p),
+                                          ip(s.fric_switch), fp(s.fric_coef), fp(s.dt),
+                                          fp(s.dx), fp(s.dy), cptr, hp, zp, ip(s.nx), ip(s.ny))
+            check(rc, f"init layer {s.id}")
+
+    # ------------------------------------------------------------ running
+    def run(self, steps, graph=True):
+        rc = lib().gcomcot_run_steps_(ip(steps), ip(1 if graph else 0))
+        check(rc, "run_steps")
+
+    def timed(self, steps, per_step=True):
+        total = ctypes.c_float(0)
+        each = np.zeros(steps, dtype=np.float32) if per_step else None
+        rc = lib().gcomcot_time_steps_(ip(steps), ctypes.byref(total),
+                                       each.ctypes.data_as(ctypes.c_void_p) if per_step else None)
+        check(rc, "time_steps")
+        return total.value, each
+
+    def dropin_step(self):
+        """One root step through the reference entry points, in the order
+        of comcot.f90:362-439 (MAX_AMP after momentum, CHANGE last)."""
+        L = lib()
+        one = ctypes.c_int(1)
+        L.mass_launch_(None, None, None, ctypes.byref(one))
+        L.openbd_launch_(None)
+        if len(self.scenario.layers) > 1:
+            L.all_grid_launch_()
+        L.momt_launch_(None, None, None, ctypes.byref(one))
+        L.maxamp_launch_(ctypes.byref(one))
+        L.cuda_update_layer_(ctypes.byref(one))
+
+    # ------------------------------------------------------------ state
+    def state(self, lid=1):
+        """Level-1 and level-2 planes of a layer, as (ny, nx) float32."""
+        s = self.specs[lid]
+        L = lib()
+        out = {k: _plane(s.ny, s.nx) for k in ("z1", "z2", "m1", "m2", "n1", "n2")}
+        p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+        L.cuda_getz1_(p(out["z1"]), ip(lid))
+        L.cuda_getz_(p(out["z2"]), ip(lid))
+        L.cuda_getmn1_(p(out["m1"]), p(out["n1"]), ip(lid))
+        L.cuda_getmn_(p(out["m2"]), p(out["n2"]), ip(lid))
+        return out
+
+    def set_level1(self, lid, z=None, m=None, n=None):
+        s = self.specs[lid]
+        L = lib()
+        if z is not None:
+            za, zp = arr(z)
+            L.cuda_setz1_(zp, ip(lid))
+        if m is not None or n is not None:
+            cur = self.state(lid)
+            ma, mp = arr(cur["m1"] if m is None else m)
+            na, np_ = arr(cur["n1"] if n is None else n)
+            L.cuda_setmn1_(mp, np_, ip(lid))
+
+    def maxima(self, lid=1):
+        s = self.specs[lid]
+        zw = _plane(s.ny, s.nx)
+        ar = np.empty((s.ny, s.nx), dtype=np.int32)
+        zr = _plane(s.ny, s.nx)
+        p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+        lib().gcomcot_get_max_(ip(lid), p(zw), p(ar))
+        lib().cuda_getzmax_(p(zr), ip(lid))
+        return {"zmax_wet": zw, "arrival": ar, "zmax_raw": zr}
+
+    def flux_cache(self, lid=1):
+        s = self.specs[lid]
+        xf = np.empty(2 * s.nx, dtype=np.float32)
+        yf = np.empty(2 * s.ny, dtype=np.float32)
+        lib().gcomcot_get_flux_cache_(ip(lid), xf.ctypes.data_as(ctypes.c_void_p),
+                                      yf.ctypes.data_as(ctypes.c_void_p))
+        return xf.reshape(2, s.nx), yf.reshape(2, s.ny)
+
+    def close(self):
+        lib().cuda_shutdown_()

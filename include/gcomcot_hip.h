@@ -1,0 +1,169 @@
+/* gcomcot_hip — MI355X (gfx950) HIP implementation of COMCOT's staggered
+ * leapfrog shallow-water step behind the C ABI that the COMCOT Fortran
+ * driver already calls.
+ *
+ * Part 1 is the drop-in boundary: the 19 external symbols that the
+ * reference Fortran objects leave undefined for the GPU half
+ * (comcot.f90:281-298,362,367,390,399,421,509; output.f90:777-778,918,
+ * 1085,1140; all_grids.f90:429-446), with the reference's signatures:
+ * lowercase with a trailing underscore, every argument by reference,
+ * arrays as base addresses of contiguous column-major (I fastest) data,
+ * layer id 1 = outermost layer, ids 1..19.  Every call returns after the
+ * device work it issued is complete, as the reference's calls do.
+ * Errors print a message and exit(EXIT_FAILURE), as the reference does
+ * (GPUHeader.h:29-44), unless gcomcot_set_error_mode_(0) selected status
+ * mode (then calls return early and gcomcot_last_error_ reports).
+ *
+ * Part 2 extends the boundary with what the reference cannot express:
+ * nonlinear and Cartesian layers, a whole-run device loop, the wet-only
+ * maximum and first-arrival fields, and the scenario batch engine.
+ */
+#ifndef GCOMCOT_HIP_H
+#define GCOMCOT_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ */
+/* Part 1: reference entry points                                      */
+/* ------------------------------------------------------------------ */
+
+/* replaces gcomcot_init_gpu_ (GPUcomcot.cu:192-211): select device 0,
+ * create the library's stream, clear the layer table */
+void gcomcot_init_gpu_(void);
+
+/* replaces gcomcot_init_layer_ (GPUcomcot.cu:214-251 + cuda_alloc_layer
+ * :36-142).  Spherical linear layer (the only kind the reference's GPU
+ * path runs, GPUall_grids.cu:683,687).  R1..R5, R11 are the (NX,NY)
+ * planes of PARA, R6 its (NY) vector; corners = IS,IE,JS,JE in the parent
+ * (1-based).  Row factors are taken from column I=1 as the reference does
+ * (GPUcomcot.cu:62-68); R2/R4 are reduced to row factors of max(HP,0) when
+ * every cell of the row reproduces bit for bit, otherwise kept as planes. */
+void gcomcot_init_layer_(int *layerid, int *parent, int *level, int *rel_size, int *rel_time,
+                         int *corners, float *grx, float *gry, float *R1, float *R2, float *R3,
+                         float *R4, float *R5, float *R6, float *R11, float *H, float *Z,
+                         int *nx, int *ny);
+
+/* replaces gcomcot_show_grids_ (GPUall_grids.cu:26-30): print the tree */
+void gcomcot_show_grids_(void);
+
+/* replaces mass_launch_ (GPUMass_s.cu:9-37): continuity for layer *lid
+ * into level 2.  For the outermost layer the radiation boundary of OPEN
+ * (boundaries.f90:22-202) is computed in the same kernel.  Z1/Z2/H are
+ * ignored (the reference uses them only in debug builds). */
+void mass_launch_(const float *Z1, float *Z2, const float *H, const int *lid);
+
+/* replaces openbd_launch_ (GPUOpen_BD.cu:13-37): nothing left to do — the
+ * boundary ring was produced by mass_launch_ (same inputs, same values) */
+void openbd_launch_(float *Z2);
+
+/* replaces momt_launch_ (GPUMoment_s.cu:10-56): momentum for layer *lid
+ * into level 2.  Arrays ignored, as in the reference. */
+void momt_launch_(float *M2, float *N2, float *Z2, const int *lid);
+
+/* replaces all_grid_launch_ (GPUall_grids.cu:701-705): every nested layer
+ * below the root for one root step (JNQ/NEWQ, mass, recursion, momentum,
+ * JNZ at the half step, level rotation), all_grids.f90:4-312 order */
+void all_grid_launch_(void);
+
+/* replaces maxamp_launch_ (GPUOutput.cu:11-27): Zmax = max(Zmax, Z2) with
+ * no wet test (reference semantics); also advances the wet-only maximum
+ * and the first-arrival field of the extension */
+void maxamp_launch_(const int *lid);
+
+/* replaces cuda_update_layer_ (GPUcomcot.cu:144-155): swap time levels */
+void cuda_update_layer_(int *lid);
+
+/* replace the accessors of GPUOutput.cu:55-103: level 2 (z, mn), level 1
+ * (z1, mn1) and the raw maximum, whole planes, host <-> device */
+void cuda_getz_(float *Z, int *lid);
+void cuda_setz_(float *Z, int *lid);
+void cuda_getz1_(float *Z, int *lid);
+void cuda_setz1_(float *Z, int *lid);
+void cuda_getmn_(float *M, float *N, int *lid);
+void cuda_setmn_(float *M, float *N, int *lid);
+void cuda_getmn1_(float *M, float *N, int *lid);
+void cuda_setmn1_(float *M, float *N, int *lid);
+void cuda_getzmax_(float *Zmax, int *lid);
+
+/* replaces gcomcot_get_z_ (GPUOutput.cu:29-44): level-2 Z at 1-based
+ * (i,j),(i+1,j),(i,j+1),(i+1,j+1) for gauge interpolation */
+void gcomcot_get_z_(int *lid, float *l1, float *l2, float *l3, float *l4, int *i, int *j);
+
+/* replaces cuda_shutdown_ (GPUcomcot.cu:253-256, empty there): frees every
+ * device buffer of the library */
+void cuda_shutdown_(void);
+
+/* ------------------------------------------------------------------ */
+/* Part 2: extensions (no reference counterpart)                       */
+/* ------------------------------------------------------------------ */
+
+/* Per-row coefficient table of PARA (comcot.f90:694-804) for a spherical
+ * layer: coef[k*ny + j], k = R0,R1,R11,F2,R21,R22,R3,F4,R5,R6 (10 rows of
+ * ny).  R2 = F2*max(HP,0) and R4 = F4*max(HQ,0) for linear layers; F2, F4
+ * for nonlinear ones. */
+void gcomcot_para_rows_(const int *laygov, const int *ny, const float *dt, const float *dx,
+                        const float *y, const float *dely, float *coef);
+
+/* CR_CHECK (initialization.f90:1163-1196): dt of a first-level layer */
+float gcomcot_cr_dt_(const int *laycord, const int *laygov, const int *nx, const int *ny,
+                     const float *h, const float *dx, const float *dy, const float *lat_first,
+                     const float *lat_last, const float *dt_in);
+
+/* CR_CHECK child part (initialization.f90:1206-1247): REL_TIME and DT */
+void gcomcot_child_dt_(const int *laycord, const int *laygov, const int *nx, const int *ny,
+                       const float *h, const float *dx, const float *dy, const float *lat_first,
+                       const float *lat_last, const float *dt_parent, int *rel_time,
+                       float *dt_child);
+
+/* Any layer: laycord 0 spherical / 1 Cartesian; laygov 0 linear / 1
+ * nonlinear (9 and, for Cartesian, 5 accepted as nonlinear, as MASS and
+ * MOMENT dispatch them, mass.f90:19-52); fric_switch 0 Manning (constant
+ * fric_coef) / 1 none.  coef: per-row table of gcomcot_para_rows_ for
+ * spherical layers (ignored for Cartesian).  dx, dy: arc minutes
+ * (spherical) or metres (Cartesian).  Returns 0 or an error code. */
+int gcomcot_init_layer_ex_(int *layerid, int *parent, int *level, int *laycord, int *laygov,
+                           int *rel_size, int *rel_time, int *corners, int *fric_switch,
+                           float *fric_coef, float *dt, float *dx, float *dy, float *coef,
+                           float *H, float *Z, int *nx, int *ny);
+
+/* Run *nsteps complete root steps on the device (mass+open, nested
+ * layers, momentum, wet-only max and arrival, level rotation) as one
+ * enqueued sequence (captured in a HIP graph when *use_graph != 0).
+ * Returns 0 or an error code. */
+int gcomcot_run_steps_(int *nsteps, int *use_graph);
+
+/* Same work as gcomcot_run_steps_ without a graph, timed with HIP events
+ * recorded on the library's stream: *ms_total from the first launch to
+ * the end of the last step; ms_each[k] (if not NULL) the duration of root
+ * step k, from the event before its first kernel to the event after its
+ * last one. */
+int gcomcot_time_steps_(int *nsteps, float *ms_total, float *ms_each);
+
+/* Wet-only maximum (starts at 0; cells with H+Z > GX, as output.f90:
+ * 876-885) and first-arrival root step (-1 = never) of a layer */
+void gcomcot_get_max_(int *lid, float *zmax_wet, int *arrival);
+void gcomcot_set_arrival_threshold_(float *thr);
+
+/* Interface flux cache of a parent layer: XFLUX(NX,2), YFLUX(NY,2)
+ * (all_grids.f90:720,790) */
+void gcomcot_get_flux_cache_(int *lid, float *xflux, float *yflux);
+
+/* Layer size query: returns 0 if the id exists */
+int gcomcot_layer_dims_(int *lid, int *nx, int *ny);
+
+/* Error handling: mode 1 = print and exit (reference), 0 = record and
+ * return; last error text copied into buf (returns its code, 0 = none) */
+void gcomcot_set_error_mode_(int *fatal);
+int gcomcot_last_error_(char *buf, int *len);
+
+/* Launch-shape override for the step kernels (autotuner hook): tile
+ * variant index, see gcomcot_tile_variants_.  Returns 0 if valid. */
+int gcomcot_set_tile_variant_(int *variant);
+int gcomcot_tile_variants_(int *count, int *tx, int *ty);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

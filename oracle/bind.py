@@ -1,0 +1,229 @@
+"""Python bindings of the two CPU checkers (test infrastructure only).
+
+  Oracle     oracle/_build/liboracle.so   C++ restatement (oracle.cpp),
+             FP32 or FP64, libm or device pow/exp
+  Reference  oracle/_ref/libcomcot_ref.so the reference's own Fortran,
+             flang-built from its sources by oracle/Makefile
+
+Both take a gcomcot Scenario (root first, children in id order 2, 3, ...)
+and expose the same calls: step, call (one routine), state, set_state.
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use
+this module.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "_build", "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libcomcot_ref.so")
+REFERENCE_DIR = "/root/reference"
+
+OP = dict(MASS=1, OPEN=2, MOMENT=3, JNQ=4, NEWQ=5, JNZ=6, UPDATE=7, ALL_GRID=8, CHANGE=9, MAX=10)
+
+
+def build(ref=True):
+    """Build liboracle.so, and the reference library when its sources exist."""
+    subprocess.run(["make", "-s", "-C", HERE, "liboracle"], check=True)
+    if ref and os.path.isdir(REFERENCE_DIR):
+        subprocess.run(["make", "-s", "-C", HERE, "ref", f"REF={REFERENCE_DIR}"], check=True)
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _i(v):
+    return ctypes.byref(ctypes.c_int(int(v)))
+
+
+def _f(v):
+    return ctypes.byref(ctypes.c_float(float(v)))
+
+
+class Oracle:
+    """The C++ restatement.  precision 32 or 64; pow_mode 0 libm, 1 device."""
+
+    _lib = None
+
+    @classmethod
+    def lib(cls):
+        if cls._lib is None:
+            if not os.path.exists(ORACLE_SO):
+                build(ref=False)
+            L = ctypes.CDLL(ORACLE_SO)
+            L.orc_new.restype = ctypes.c_void_p
+            L.orc_new.argtypes = [ctypes.c_int, ctypes.c_int]
+            L.orc_free.argtypes = [ctypes.c_void_p]
+            L.orc_add_layer.restype = ctypes.c_int
+            L.orc_add_layer.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 6 + \
+                [ctypes.c_double] * 3 + [ctypes.c_void_p] * 4 + [ctypes.c_int, ctypes.c_double,
+                                                                  ctypes.c_int, ctypes.c_int,
+                                                                  ctypes.c_void_p]
+            L.orc_step.argtypes = [ctypes.c_void_p, ctypes.c_int]
+            L.orc_call.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+            L.orc_state_io.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int] + \
+                [ctypes.c_void_p] * 4
+            L.orc_setup_io.argtypes = [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 5
+            L.orc_flux_io.argtypes = [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 2
+            L.orc_max_io.argtypes = [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 3
+            L.orc_set_arrival_threshold.argtypes = [ctypes.c_void_p, ctypes.c_double]
+            L.orc_cr_dt.restype = ctypes.c_double
+            L.orc_cr_dt.argtypes = [ctypes.c_int] * 4 + [ctypes.c_void_p] + [ctypes.c_double] * 5
+            cls._lib = L
+        return cls._lib
+
+    def __init__(self, scenario, precision=32, pow_mode=0, arrival_threshold=0.01):
+        L = self.lib()
+        self.h = L.orc_new(precision, pow_mode)
+        self.dtype = np.float32 if precision == 32 else np.float64
+        self.specs = {}
+        L.orc_set_arrival_threshold(self.h, arrival_threshold)
+        for s in scenario.layers:
+            self.specs[s.id] = s
+            y = np.zeros(s.ny) if s.y is None else np.asarray(s.y, dtype=np.float64)
+            dely = np.zeros(s.ny) if s.dely is None else np.asarray(s.dely, dtype=np.float64)
+            h = np.ascontiguousarray(s.h, dtype=np.float64)
+            z0 = np.ascontiguousarray(s.z0, dtype=np.float64)
+            corners = np.array(s.corners, dtype=np.int32)
+            rc = L.orc_add_layer(self.h, s.id, s.parent, s.laycord, s.laygov, s.nx, s.ny,
+                                 s.dx, s.dy, s.dt, _p(y), _p(dely), _p(h), _p(z0),
+                                 s.fric_switch, s.fric_coef, s.rel_size, s.rel_time, _p(corners))
+            if rc != 0:
+                raise ValueError(f"orc_add_layer({s.id}) = {rc}")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib().orc_free(self.h)
+            self.h = None
+
+    def step(self, n=1):
+        self.lib().orc_step(self.h, n)
+
+    def call(self, op, lid=1, t=0):
+        self.lib().orc_call(self.h, OP[op] if isinstance(op, str) else op, lid, t)
+
+    def state(self, lid=1):
+        s = self.specs[lid]
+        bufs = [np.empty(2 * s.nx * s.ny) for _ in range(4)]
+        self.lib().orc_state_io(self.h, lid, 0, *[_p(b) for b in bufs])
+        out = {}
+        for name, b in zip(("z", "m", "n", "dz"), bufs):
+            b = b.reshape(2, s.ny, s.nx).astype(self.dtype)
+            out[name + "1"], out[name + "2"] = b[0], b[1]
+        return out
+
+    def set_state(self, lid, st):
+        s = self.specs[lid]
+        bufs = []
+        for name in ("z", "m", "n", "dz"):
+            b = np.stack([np.asarray(st[name + "1"], np.float64), np.asarray(st[name + "2"], np.float64)])
+            bufs.append(np.ascontiguousarray(b.reshape(-1)))
+        self.lib().orc_state_io(self.h, lid, 1, *[_p(b) for b in bufs])
+
+    def setup(self, lid=1):
+        s = self.specs[lid]
+        n = s.nx * s.ny
+        hp, hq, rpl, r6, sc = (np.empty(n), np.empty(n), np.empty(9 * n), np.empty(s.ny),
+                               np.empty(5))
+        self.lib().orc_setup_io(self.h, lid, _p(hp), _p(hq), _p(rpl), _p(r6), _p(sc))
+        return dict(hp=hp.reshape(s.ny, s.nx), hq=hq.reshape(s.ny, s.nx),
+                    r=rpl.reshape(9, s.ny, s.nx), r6=r6, rx=sc[0], ry=sc[1], grx=sc[2],
+                    gry=sc[3], dt=sc[4])
+
+    def flux(self, lid=1):
+        s = self.specs[lid]
+        xf, yf = np.empty(2 * s.nx), np.empty(2 * s.ny)
+        self.lib().orc_flux_io(self.h, lid, _p(xf), _p(yf))
+        return xf.reshape(2, s.nx).astype(self.dtype), yf.reshape(2, s.ny).astype(self.dtype)
+
+    def maxima(self, lid=1):
+        s = self.specs[lid]
+        n = s.nx * s.ny
+        zw, zr, ar = np.empty(n), np.empty(n), np.empty(n, dtype=np.int32)
+        self.lib().orc_max_io(self.h, lid, _p(zw), _p(zr), _p(ar))
+        return {"zmax_wet": zw.reshape(s.ny, s.nx).astype(self.dtype),
+                "zmax_raw": zr.reshape(s.ny, s.nx).astype(self.dtype),
+                "arrival": ar.reshape(s.ny, s.nx)}
+
+
+class Reference:
+    """The flang-built reference Fortran (needs oracle/_ref built)."""
+
+    _lib = None
+
+    @classmethod
+    def available(cls):
+        return os.path.exists(REF_SO)
+
+    @classmethod
+    def lib(cls):
+        if cls._lib is None:
+            if not os.path.exists(REF_SO):
+                raise FileNotFoundError(REF_SO)
+            cls._lib = ctypes.CDLL(REF_SO)
+        return cls._lib
+
+    def __init__(self, scenario):
+        L = self.lib()
+        self.specs = {}
+        for k, s in enumerate(scenario.layers):
+            if s.id != k + 1:
+                raise ValueError("reference driver needs layer ids 1, 2, 3, ... in order")
+            self.specs[s.id] = s
+            y = np.zeros(s.ny, np.float32) if s.y is None else np.asarray(s.y, np.float32)
+            dely = np.zeros(s.ny, np.float32) if s.dely is None else np.asarray(s.dely, np.float32)
+            h = np.ascontiguousarray(s.h, dtype=np.float32)
+            z0 = np.ascontiguousarray(s.z0, dtype=np.float32)
+            self._keep = (y, dely, h, z0)
+            if s.parent == -1:
+                L.ref_init_root(_i(s.nx), _i(s.ny), _i(s.laycord), _i(s.laygov), _f(s.dx),
+                                _f(s.dy), _f(s.dt), _p(y), _p(dely), _p(h), _p(z0),
+                                _i(s.fric_switch), _f(s.fric_coef))
+            else:
+                corners = np.array(s.corners, dtype=np.int32)
+                L.ref_init_child(_i(s.parent), _i(s.laycord), _i(s.laygov), _i(s.rel_size),
+                                 _i(s.rel_time), _p(corners), _i(s.nx), _i(s.ny), _f(s.dx),
+                                 _f(s.dy), _f(s.dt), _p(y), _p(dely), _p(h), _p(z0),
+                                 _i(s.fric_switch), _f(s.fric_coef))
+
+    def step(self, n=1):
+        self.lib().ref_step(_i(n))
+
+    def call(self, op, lid=1, t=0):
+        self.lib().ref_call(_i(OP[op] if isinstance(op, str) else op), _i(lid), _i(t))
+
+    def state(self, lid=1):
+        s = self.specs[lid]
+        bufs = [np.empty(2 * s.nx * s.ny, np.float32) for _ in range(4)]
+        self.lib().ref_state_io(_i(lid), _i(0), *[_p(b) for b in bufs])
+        out = {}
+        for name, b in zip(("z", "m", "n", "dz"), bufs):
+            b = b.reshape(2, s.ny, s.nx)
+            out[name + "1"], out[name + "2"] = b[0].copy(), b[1].copy()
+        return out
+
+    def set_state(self, lid, st):
+        bufs = []
+        for name in ("z", "m", "n", "dz"):
+            b = np.stack([np.asarray(st[name + "1"], np.float32), np.asarray(st[name + "2"], np.float32)])
+            bufs.append(np.ascontiguousarray(b.reshape(-1)))
+        self.lib().ref_state_io(_i(lid), _i(1), *[_p(b) for b in bufs])
+
+    def setup(self, lid=1):
+        s = self.specs[lid]
+        n = s.nx * s.ny
+        hp, hq, rpl = np.empty(n, np.float32), np.empty(n, np.float32), np.empty(9 * n, np.float32)
+        r6, sc = np.empty(s.ny, np.float32), np.empty(5, np.float32)
+        self.lib().ref_setup_io(_i(lid), _p(hp), _p(hq), _p(rpl), _p(r6), _p(sc))
+        return dict(hp=hp.reshape(s.ny, s.nx), hq=hq.reshape(s.ny, s.nx),
+                    r=rpl.reshape(9, s.ny, s.nx), r6=r6, rx=sc[0], ry=sc[1], grx=sc[2],
+                    gry=sc[3], dt=sc[4])
+
+    def flux(self, lid=1):
+        s = self.specs[lid]
+        xf, yf = np.empty(2 * s.nx, np.float32), np.empty(2 * s.ny, np.float32)
+        self.lib().ref_flux_io(_i(lid), _p(xf), _p(yf))
+        return xf.reshape(2, s.nx), yf.reshape(2, s.ny)

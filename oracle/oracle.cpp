@@ -1,0 +1,1177 @@
+// ORACLE — C++ restatement of COMCOT's shallow-water step (test infrastructure).
+//
+// Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load
+// this library, as the checker (and as a timed CPU baseline).  The product
+// path never calls it.  Each routine below restates one reference routine,
+// cited file:line, in the same operation order and the same loop bounds,
+// with the same REAL (FP32) constants.  Built with -ffp-contract=off so no
+// fused multiply-adds change rounding, as in the x86-64 Fortran build.
+//
+// Parity pinning (see tests/): the FP32 instance is compared bit for bit
+// with the flang-built reference Fortran on committed golden fixtures
+// (tests/golden/) and, where oracle/_ref is built, on fresh seeded inputs.
+//
+// The only code shared with the product is the scalar pow/exp of
+// detmath.h, used in pow_mode 1; it is pinned separately against libm.
+#include "oracle.h"
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "../gpu-comcot_amd/csrc/detmath.h"
+
+namespace {
+
+// BLOCK DATA /CONS/ (comcot.f90:454-470) and PARA's DATA (comcot.f90:709)
+const float kElmax = -50.0f;
+const float kGrav = 9.807f;
+const float kREarth = 6378000.0f;
+const float kGx = 1.0e-5f;
+const float kEps = 1.0e-10f;
+const float kVLimit = 20.0f;
+const float kRadDeg = 0.01745329252f;
+const float kRadMin = 0.000290888208665721f;
+const float kCoriW = 7.2722e-5f;
+const float kOSixty = 0.016666666667f;
+const float kUB = 99.0f;  // boundaries.f90:30
+
+inline float m_cos(float x) { return cosf(x); }
+inline double m_cos(double x) { return cos(x); }
+inline float m_sin(float x) { return sinf(x); }
+inline double m_sin(double x) { return sin(x); }
+inline float m_sqrt(float x) { return sqrtf(x); }
+inline double m_sqrt(double x) { return sqrt(x); }
+inline float m_abs(float x) { return fabsf(x); }
+inline double m_abs(double x) { return fabs(x); }
+
+template <class T>
+struct Layer {
+  int id = 0, parent = -1, level = 1;
+  int nx = 0, ny = 0, laycord = 0, laygov = 0;
+  int rel_size = 1, rel_time = 1, corners[4] = {0, 0, 0, 0};
+  int fric_switch = 1, linchk = 1, modscm = 0;
+  T fric_coef = 0, dt = 0, dx = 0, dy = 0, rx = 0, ry = 0, grx = 0, gry = 0;
+  std::vector<T> h, hp, hq, z, m, n, dz;  // z/m/n/dz: 2 levels
+  std::vector<T> r0, r1, r11, r2, r21, r22, r3, r4, r5, r6;
+  std::vector<T> xflux, yflux;            // (nx,2), (ny,2)
+  std::vector<T> zmax_wet, zmax_raw;
+  std::vector<int> arrival;
+  std::vector<int> children;              // indices into State::layers
+
+  size_t nxy() const { return (size_t)nx * ny; }
+  // 1-based (I,J) of a plane; K = time level 1 or 2
+  size_t ix(int i, int j) const { return (size_t)(j - 1) * nx + (i - 1); }
+  T &Z(int i, int j, int k) { return z[(k - 1) * nxy() + ix(i, j)]; }
+  T &M(int i, int j, int k) { return m[(k - 1) * nxy() + ix(i, j)]; }
+  T &N(int i, int j, int k) { return n[(k - 1) * nxy() + ix(i, j)]; }
+  T &DZ(int i, int j, int k) { return dz[(k - 1) * nxy() + ix(i, j)]; }
+  T &H(int i, int j) { return h[ix(i, j)]; }
+  T &HP(int i, int j) { return hp[ix(i, j)]; }
+  T &HQ(int i, int j) { return hq[ix(i, j)]; }
+  T &R(std::vector<T> &r, int i, int j) { return r[ix(i, j)]; }
+};
+
+template <class T>
+struct State {
+  int pow_mode = 0;
+  T arrival_thr = T(0.01f);
+  int nstep = 0;
+  std::vector<Layer<T>> layers;
+  int find(int id) {
+    for (size_t k = 0; k < layers.size(); ++k)
+      if (layers[k].id == id) return (int)k;
+    return -1;
+  }
+
+  T pw(T x, T y) {
+    if constexpr (sizeof(T) == 4) {
+      return pow_mode ? gcdet::powf_det(x, y) : powf(x, y);
+    } else {
+      return pow(x, y);
+    }
+  }
+  T ex(T x) {
+    if constexpr (sizeof(T) == 4) {
+      return pow_mode ? gcdet::expf_det(x) : expf(x);
+    } else {
+      return exp(x);
+    }
+  }
+
+  // ---------------------------------------------------------------- setup
+  // PQ_DEPTH (initialization.f90:2501-2526)
+  void pq_depth(Layer<T> &L) {
+    const T GX = T(kGx);
+    for (int i = 1; i <= L.nx; ++i) {
+      int ip1 = i + 1;
+      if (ip1 >= L.nx) ip1 = L.nx;
+      for (int j = 1; j <= L.ny; ++j) {
+        int jp1 = j + 1;
+        if (jp1 >= L.ny) jp1 = L.ny;
+        if (L.H(i, j) > GX) {
+          L.HP(i, j) = T(0.5f) * (L.H(i, j) + L.H(ip1, j));
+          L.HQ(i, j) = T(0.5f) * (L.H(i, j) + L.H(i, jp1));
+        }
+      }
+    }
+  }
+
+  // PARA (comcot.f90:694-804), LAYGOV 0/1/9 branch
+  void para(Layer<T> &L, const std::vector<T> &y, const std::vector<T> &dely) {
+    const T RR = L.dt / T(kREarth);
+    const T RS = T(kGrav) * RR;
+    const T RT = T(0.5f) * L.dt * T(kCoriW);
+    const T RADM = T(kRadMin), RADD = T(kRadDeg);
+    for (int j = 1; j <= L.ny; ++j) {
+      T ang_m = y[j - 1] * RADD;
+      T ang_n = (y[j - 1] + T(0.5f) * dely[j - 1] * T(kOSixty)) * RADD;
+      T cos_m = m_cos(ang_m), cos_n = m_cos(ang_n);
+      T sin_m = m_sin(ang_m), sin_n = m_sin(ang_n);
+      for (int i = 1; i <= L.nx; ++i) {
+        L.R(L.r0, i, j) = RR / (dely[j - 1] * RADM);
+        L.R(L.r1, i, j) = RR / cos_m / (L.dx * RADM);
+        L.R(L.r11, i, j) = RR / cos_m / (dely[j - 1] * RADM);
+        if (L.laygov == 0)
+          L.R(L.r2, i, j) = RS / cos_m / (L.dx * RADM) * std::max(L.HP(i, j), T(0));
+        if (L.laygov == 1) L.R(L.r2, i, j) = RS / cos_m / (L.dx * RADM);
+        L.R(L.r21, i, j) = RR / cos_m / (L.dx * RADM);
+        L.R(L.r22, i, j) = RR / cos_n / (L.dx * RADM);
+        L.R(L.r3, i, j) = RT * sin_m;
+        if (L.laygov == 0)
+          L.R(L.r4, i, j) = RS / (dely[j - 1] * RADM) * std::max(L.HQ(i, j), T(0));
+        if (L.laygov == 1) L.R(L.r4, i, j) = RS / (dely[j - 1] * RADM);
+        L.R(L.r5, i, j) = RT * sin_n;
+      }
+      L.r6[j - 1] = cos_n;
+    }
+  }
+
+  // ----------------------------------------------------------------- mass
+  // MASS_S (mass.f90:61-100); CONMASS_S (mass.f90:227-277) when nl
+  void mass_s(Layer<T> &L, bool nl) {
+    const T GX = T(kGx), EPS = T(kEps), ELMAX = T(kElmax);
+    int is = 2, js = 2, ie, je;
+    if (L.id == 1) { ie = L.nx - 1; je = L.ny - 1; } else { ie = L.nx; je = L.ny; }
+    for (int j = js; j <= je; ++j) {
+      int jm1 = j - 1;
+      for (int i = is; i <= ie; ++i) {
+        T h = L.H(i, j);
+        bool active = nl ? (h >= ELMAX) : (h > GX);
+        if (active) {
+          T zz = L.Z(i, j, 1) - L.R(L.r1, i, j) * (L.M(i, j, 1) - L.M(i - 1, j, 1)) -
+                 L.R(L.r11, i, j) * (L.N(i, j, 1) * L.r6[j - 1] - L.N(i, jm1, 1) * L.r6[jm1 - 1]);
+          if (m_abs(zz) < EPS) zz = T(0);
+          if (!nl) {
+            if ((zz + h) <= EPS) zz = -h;
+            L.Z(i, j, 2) = zz;
+          } else {
+            T dd = zz + h;
+            if (dd >= GX) { L.DZ(i, j, 2) = dd; L.Z(i, j, 2) = zz; }
+            else { L.DZ(i, j, 2) = T(0); L.Z(i, j, 2) = -h; }
+          }
+        } else {
+          if (!nl) L.Z(i, j, 2) = T(0);
+          else { L.DZ(i, j, 2) = T(0); L.Z(i, j, 2) = -h; }
+        }
+      }
+    }
+  }
+
+  // MASS_C (mass.f90:103-157); CONMASS (mass.f90:162-224) when nl.  2-D only.
+  void mass_c(Layer<T> &L, bool nl) {
+    const T GX = T(kGx), EPS = T(kEps), ELMAX = T(kElmax);
+    const T RX = L.rx, RY = L.ry;
+    int is = 2, js = 2, ie, je;
+    if (L.id == 1) { ie = L.nx - 1; je = L.ny - 1; } else { ie = L.nx; je = L.ny; }
+    for (int j = js; j <= je; ++j) {
+      for (int i = is; i <= ie; ++i) {
+        T h = L.H(i, j);
+        bool active = nl ? (h > ELMAX) : (h > GX);
+        if (active) {
+          T zz = L.Z(i, j, 1) - RX * (L.M(i, j, 1) - L.M(i - 1, j, 1)) -
+                 RY * (L.N(i, j, 1) - L.N(i, j - 1, 1));
+          if (m_abs(zz) < EPS) zz = T(0);
+          if (!nl) {
+            if ((zz + h) <= EPS) zz = -h;
+            L.Z(i, j, 2) = zz;
+          } else {
+            T dd = zz + h;
+            if (dd >= GX) { L.DZ(i, j, 2) = dd; L.Z(i, j, 2) = zz; }
+            else { L.DZ(i, j, 2) = T(0); L.Z(i, j, 2) = -h; }
+          }
+        } else {
+          if (!nl) L.Z(i, j, 2) = T(0);
+          else { L.DZ(i, j, 2) = T(0); L.Z(i, j, 2) = -h; }
+        }
+      }
+    }
+  }
+
+  // MASS dispatcher (mass.f90:3-58), LAYGOV 0/1/9 (and 5 for Cartesian)
+  void mass(Layer<T> &L) {
+    if (L.laycord == 0) mass_s(L, L.laygov == 1 || L.laygov == 9);
+    else mass_c(L, L.laygov == 1 || L.laygov == 5 || L.laygov == 9);
+  }
+
+  // ---------------------------------------------------------------- open
+  // OPEN (boundaries.f90:22-202); reads level-2 fluxes like the Fortran
+  void open_bc(Layer<T> &L) {
+    const T GX = T(kGx), G = T(kGrav), UB = T(kUB), ZERO = T(0);
+    const int NX = L.nx, NY = L.ny;
+    {  // J = 1
+      int j = 1;
+      for (int i = 2; i <= NX - 1; ++i) {
+        if (L.H(i, j) > GX) {
+          T cc = m_sqrt(G * L.H(i, j));
+          T uh = T(0.5f) * (L.M(i, j, 2) + L.M(i - 1, j, 2));
+          T uu = m_sqrt(uh * uh + L.N(i, j, 2) * L.N(i, j, 2));
+          T zz = uu / cc;
+          if (L.N(i, j, 2) > ZERO) zz = -zz;
+          if (m_abs(zz) > UB) zz = T(0);
+          L.Z(i, j, 2) = zz;
+        } else L.Z(i, j, 2) = ZERO;
+      }
+    }
+    {  // J = NY
+      int j = NY;
+      for (int i = 2; i <= NX - 1; ++i) {
+        if (L.H(i, j) > GX) {
+          T cc = m_sqrt(G * L.H(i, j));
+          T uh = T(0.5f) * (L.M(i, j, 2) + L.M(i - 1, j, 2));
+          T uu = m_sqrt(uh * uh + L.N(i, j - 1, 2) * L.N(i, j - 1, 2));
+          T zz = uu / cc;
+          if (L.N(i, j - 1, 2) < ZERO) zz = -zz;
+          if (m_abs(zz) > UB) zz = T(0);
+          L.Z(i, j, 2) = zz;
+        } else L.Z(i, j, 2) = ZERO;
+      }
+    }
+    {  // I = 1
+      int i = 1;
+      for (int j = 2; j <= NY - 1; ++j) {
+        if (L.H(i, j) > GX) {
+          T cc = m_sqrt(G * L.H(i, j));
+          T uh;
+          if (L.H(i, j - 1) > GX) uh = T(0.5f) * (L.N(i, j, 2) + L.N(i, j - 1, 2));
+          else uh = L.N(i, j, 2);
+          T uu = m_sqrt(uh * uh + L.M(i, j, 2) * L.M(i, j, 2));
+          T zz = uu / cc;
+          if (L.M(i, j, 2) > ZERO) zz = -zz;
+          if (m_abs(zz) > UB) zz = T(0);
+          L.Z(i, j, 2) = zz;
+        } else L.Z(i, j, 2) = ZERO;
+      }
+    }
+    {  // I = NX
+      int i = NX;
+      for (int j = 2; j <= NY - 1; ++j) {
+        if (L.H(i, j) > GX) {
+          T cc = m_sqrt(G * L.H(i, j));
+          T uh = T(0.5f) * (L.N(i, j, 2) + L.N(i, j - 1, 2));
+          T uu = m_sqrt(uh * uh + L.M(i - 1, j, 2) * L.M(i - 1, j, 2));
+          T zz = uu / cc;
+          if (L.M(i - 1, j, 2) < ZERO) zz = -zz;
+          if (m_abs(zz) > UB) zz = T(0);
+          L.Z(i, j, 2) = zz;
+        } else L.Z(i, j, 2) = ZERO;
+      }
+    }
+    // corners (boundaries.f90:134-199)
+    auto corner = [&](int ci, int cj, T qx, T qy, bool neg) {
+      if (L.H(ci, cj) > GX) {
+        T cc = m_sqrt(G * L.H(ci, cj));
+        T uh = m_sqrt(qx * qx + qy * qy);
+        T zz = uh / cc;
+        if (neg) zz = -zz;
+        if (m_abs(zz) > UB) zz = T(0);
+        L.Z(ci, cj, 2) = zz;
+      } else L.Z(ci, cj, 2) = ZERO;
+    };
+    {
+      T qx = L.M(1, 1, 2), qy = L.N(1, 1, 2);
+      corner(1, 1, qx, qy, qx > ZERO || qy > ZERO);
+    }
+    {
+      T qx = L.M(NX - 1, 1, 2), qy = L.N(NX, 1, 2);
+      corner(NX, 1, qx, qy, qx < ZERO || qy > ZERO);
+    }
+    {
+      T qx = L.M(1, NY, 2), qy = L.N(1, NY - 1, 2);
+      corner(1, NY, qx, qy, qx > ZERO || qy < ZERO);
+    }
+    {
+      T qx = L.M(NX - 1, NY, 2), qy = L.N(NX, NY - 1, 2);
+      corner(NX, NY, qx, qy, qx < ZERO || qy < ZERO);
+    }
+  }
+
+  // ------------------------------------------------------------- momentum
+  // MOMT_S (moment.f90:65-149)
+  void momt_s(Layer<T> &L) {
+    const T GX = T(kGx), EPS = T(kEps);
+    int ie = L.nx - 1, je = L.ny - 1, is = 2, js = 2;
+    if (L.id == 1) { is = 1; js = 1; }
+    for (int i = is; i <= ie; ++i) {
+      int ip1 = i + 1;
+      for (int j = js; j <= L.ny; ++j) {
+        if (L.H(i, j) > GX && L.H(ip1, j) > GX) {
+          int jm1 = j - 1;
+          if (jm1 <= 1) jm1 = 1;
+          T tot_n = L.N(i, j, 1) + L.N(ip1, j, 1) + L.N(i, jm1, 1) + L.N(ip1, jm1, 1);
+          T xm = L.M(i, j, 1) - L.R(L.r2, i, j) * (L.Z(ip1, j, 2) - L.Z(i, j, 2)) +
+                 L.R(L.r3, i, j) * tot_n;
+          if (m_abs(xm) < EPS) xm = T(0);
+          L.M(i, j, 2) = xm;
+        } else L.M(i, j, 2) = T(0);
+      }
+    }
+    for (int j = js; j <= je; ++j) {
+      int jp1 = j + 1;
+      for (int i = is; i <= L.nx; ++i) {
+        if (L.H(i, j) > GX && L.H(i, jp1) > GX) {
+          int im1 = i - 1;
+          if (im1 <= 1) im1 = 1;
+          T tot_m = L.M(im1, j, 1) + L.M(im1, jp1, 1) + L.M(i, j, 1) + L.M(i, jp1, 1);
+          T xn = L.N(i, j, 1) - L.R(L.r4, i, j) * (L.Z(i, jp1, 2) - L.Z(i, j, 2)) -
+                 L.R(L.r5, i, j) * tot_m;
+          if (m_abs(xn) < EPS) xn = T(0);
+          L.N(i, j, 2) = xn;
+        } else L.N(i, j, 2) = T(0);
+      }
+    }
+  }
+
+  // MOMT_C (moment.f90:152-277), 2-D, constant Manning (FRIC_SWITCH 0/1)
+  void momt_c(Layer<T> &L) {
+    const T GX = T(kGx), EPS = T(kEps), TWLVTH = T(0.08333333333333f);
+    const T CONST = T(0.5f) * L.dt * T(kGrav);
+    const T FM = L.fric_coef;
+    T FF = T(0);
+    int ie = L.nx - 1, je = L.ny - 1, is, js;
+    if (L.id == 1) { is = 1; js = 1; } else { is = 2; js = 2; }
+    for (int i = is; i <= ie; ++i) {
+      int ip1 = i + 1;
+      for (int j = js; j <= je + 1; ++j) {
+        T xm;
+        if (L.H(i, j) > GX && L.H(ip1, j) > GX) {
+          int jm1 = j - 1, jp1 = j + 1;
+          if (jm1 < 1) jm1 = 1;
+          if (jp1 > L.ny) jp1 = L.ny;
+          T hm = L.HP(i, j) + T(0.5f) * (L.Z(i, j, 2) + L.Z(ip1, j, 2));
+          xm = L.M(i, j, 1) - L.grx * hm * (L.Z(ip1, j, 2) - L.Z(i, j, 2));
+          if (L.fric_switch != 1) {
+            T xqq = T(0.25f) * (L.N(i, j, 1) + L.N(ip1, j, 1) + L.N(i, jm1, 1) + L.N(ip1, jm1, 1));
+            if (hm >= T(1.0e-3f))
+              FF = CONST * FM * FM * m_sqrt(L.M(i, j, 1) * L.M(i, j, 1) + xqq * xqq) /
+                   pw(hm, T(2.3333333f));
+            else FF = T(0);
+            xm = xm - FF * L.M(i, j, 1);
+          }
+          if (L.modscm == 0) {
+            T scm = L.grx * TWLVTH * hm *
+                    ((L.Z(ip1, jp1, 2) - T(2) * L.Z(ip1, j, 2) + L.Z(ip1, jm1, 2)) -
+                     (L.Z(i, jp1, 2) - T(2) * L.Z(i, j, 2) + L.Z(i, jm1, 2)));
+            xm = xm - scm;
+          }
+          if (L.fric_switch != 1) xm = xm / (T(1) + FF);
+          if (m_abs(xm) < EPS) xm = T(0);
+          L.M(i, j, 2) = xm;
+        } else {
+          xm = T(0);
+          L.M(i, j, 2) = xm;
+        }
+      }
+    }
+    for (int j = js; j <= je; ++j) {
+      int jp1 = j + 1;
+      for (int i = is; i <= ie + 1; ++i) {
+        if (L.H(i, j) > GX && L.H(i, jp1) > GX) {
+          int im1 = i - 1, ip1 = i + 1;
+          if (im1 < 1) im1 = 1;
+          if (ip1 >= L.nx) ip1 = L.nx;
+          T hn = L.HQ(i, j) + T(0.5f) * (L.Z(i, j, 2) + L.Z(i, jp1, 2));
+          T xn = L.N(i, j, 1) - L.gry * hn * (L.Z(i, jp1, 2) - L.Z(i, j, 2));
+          if (L.fric_switch != 1) {
+            T xpp = T(0.25f) * (L.M(i, j, 1) + L.M(i, jp1, 1) + L.M(im1, j, 1) + L.M(im1, jp1, 1));
+            if (hn >= T(1.0e-5f))
+              FF = CONST * FM * FM * m_sqrt(L.N(i, j, 1) * L.N(i, j, 1) + xpp * xpp) /
+                   pw(hn, T(2.3333333f));
+            else FF = T(0);
+            xn = xn - FF * L.N(i, j, 1);
+          }
+          if (L.modscm == 0) {
+            T scm = L.gry * TWLVTH * hn *
+                    ((L.Z(ip1, jp1, 2) - T(2) * L.Z(i, jp1, 2) + L.Z(im1, jp1, 2)) -
+                     (L.Z(ip1, j, 2) - T(2) * L.Z(i, j, 2) + L.Z(im1, j, 2)));
+            xn = xn - scm;
+          }
+          if (L.fric_switch != 1) xn = xn / (T(1) + FF);
+          if (m_abs(xn) < EPS) xn = T(0);
+          L.N(i, j, 2) = xn;
+        } else L.N(i, j, 2) = T(0);
+      }
+    }
+  }
+
+  // WAVE_BREAKING (moment.f90:1272-1384): damping of the convection terms
+  void wave_breaking(Layer<T> &L, T &advx, T &advy, int i, int j) {
+    const T G = T(kGrav);
+    T c = m_sqrt(G * L.DZ(i, j, 2));
+    T dzdt_f = T(0.08f) * c;
+    T dzdt_i = T(0.65f) * c;
+    T dzdt = m_abs(L.Z(i, j, 2) - L.Z(i, j, 1)) / L.dt;
+    if (dzdt > dzdt_f) {
+      T ref = (dzdt - dzdt_f) / (dzdt_i - dzdt_f);
+      T a = T(1);
+      T coef = ex(-(a * ref));
+      advx = coef * advx;
+      advy = coef * advy;
+    }
+  }
+
+  // CONMOME_S (moment.f90:783-1268) when sph, CONMOME (moment.f90:280-780)
+  // otherwise.  2-D, FRIC_SWITCH 0/1.  Differences between the two kept as
+  // in the reference: the wet/dry-edge flux depth (HP+avg Z vs DZ/2,
+  // moment.f90:912 vs :421), the Coriolis term (spherical only, :963,
+  // :1158), the advection factors (R21/R0/R22 vs RX/RY), the SCM order,
+  // and CONMOME's Y equation subtracting friction from XP (:672).
+  void conmome(Layer<T> &L, bool sph) {
+    const T GX = T(kGx), EPS = T(kEps), ELMAX = T(kElmax), VL = T(kVLimit);
+    const int IX = L.nx, JY = L.ny;
+    const int IFRIC = L.fric_switch;
+    const T FM = L.fric_coef;
+    const T CONST = (IFRIC != 1) ? T(0.5f) * L.dt * T(kGrav) : T(0);
+    int lin_chk = L.linchk;
+    if (L.laygov == 9) { L.linchk = 0; lin_chk = 0; }
+    const int MOD_SCM = L.modscm;
+    const T RX = L.rx, RY = L.ry, GRX = L.grx, GRY = L.gry;
+    const size_t nxy = L.nxy();
+    // DP, DQ (moment.f90:345-366 / 849-870)
+    std::vector<T> DP(2 * nxy), DQ(2 * nxy);
+    for (int i = 1; i <= IX; ++i) {
+      for (int j = 1; j <= JY; ++j) {
+        int ip1 = i + 1;
+        if (ip1 >= IX) ip1 = IX;
+        int jp1 = j + 1;
+        if (jp1 >= JY) jp1 = JY;
+        T dp1 = T(0.25f) * (L.DZ(i, j, 2) + L.DZ(i, j, 1) + L.DZ(ip1, j, 2) + L.DZ(ip1, j, 1));
+        T dp2 = T(0.5f) * (L.DZ(i, j, 2) + L.DZ(ip1, j, 2));
+        T dq1 = T(0.25f) * (L.DZ(i, j, 2) + L.DZ(i, j, 1) + L.DZ(i, jp1, 2) + L.DZ(i, jp1, 1));
+        T dq2 = T(0.5f) * (L.DZ(i, j, 2) + L.DZ(i, jp1, 2));
+        if (dp1 < GX) dp1 = T(0);
+        if (dp2 < GX) dp2 = T(0);
+        if (dq1 < GX) dq1 = T(0);
+        if (dq2 < GX) dq2 = T(0);
+        DP[L.ix(i, j)] = dp1;
+        DP[nxy + L.ix(i, j)] = dp2;
+        DQ[L.ix(i, j)] = dq1;
+        DQ[nxy + L.ix(i, j)] = dq2;
+      }
+    }
+    auto dp1 = [&](int i, int j) { return DP[L.ix(i, j)]; };
+    auto dq1 = [&](int i, int j) { return DQ[L.ix(i, j)]; };
+    auto dz2 = [&](int i, int j) { return L.DZ(i, j, 2); };
+    auto z2 = [&](int i, int j) { return L.Z(i, j, 2); };
+    auto p1 = [&](int i, int j) { return L.M(i, j, 1); };
+    auto q1 = [&](int i, int j) { return L.N(i, j, 1); };
+    const int ISPANS = 10, ISPANE = IX - 10, JSPANS = 10, JSPANE = JY - 10;
+
+    T DD = T(0), DF = T(0), FF = T(0), XP = T(0), XQ = T(0);
+    // ---- X direction
+    {
+      int is = 2, js = 2, ie = IX - 1, je = JY;
+      if (L.id == 1) { is = 1; js = 1; ie = IX - 1; je = JY - 1; }
+      for (int i = is; i <= ie; ++i) {
+        int ip1 = i + 1, im1 = i - 1;
+        if (im1 <= 1) im1 = 1;
+        if (ip1 >= IX) ip1 = IX;
+        for (int j = js; j <= je; ++j) {
+          int noflux = 0;
+          int jp1 = j + 1, jm1 = j - 1;
+          if (jm1 <= 1) jm1 = 1;
+          if (jp1 >= JY) jp1 = JY;
+          if (L.H(i, j) <= ELMAX || L.HP(i, j) <= ELMAX) {
+            noflux = 1;
+          } else {
+            if (dz2(i, j) > GX && dz2(ip1, j) > GX) {
+              DD = DP[nxy + L.ix(i, j)];
+              DF = dp1(i, j);
+            } else if (dz2(i, j) > GX && dz2(ip1, j) <= GX && L.H(ip1, j) + z2(i, j) > GX) {
+              DD = sph ? L.HP(i, j) + T(0.5f) * (z2(i, j) + z2(ip1, j)) : T(0.5f) * dz2(i, j);
+              DF = DD;
+            } else if (dz2(i, j) <= GX && dz2(ip1, j) > GX && L.H(i, j) + z2(ip1, j) > GX) {
+              DD = sph ? L.HP(i, j) + T(0.5f) * (z2(i, j) + z2(ip1, j)) : T(0.5f) * dz2(ip1, j);
+              DF = DD;
+            } else {
+              noflux = 1;
+            }
+            if (DD < GX) noflux = 1;
+          }
+          if (noflux == 1) { L.M(i, j, 2) = T(0); continue; }
+          if (DF < T(1.0e-3f)) DF = T(1.0e-3f);
+          T xqq = T(0.25f) * (q1(i, j) + q1(ip1, j) + q1(i, jm1) + q1(ip1, jm1));
+          if (IFRIC != 1)
+            FF = CONST * FM * FM * m_sqrt(p1(i, j) * p1(i, j) + xqq * xqq) / pw(DF, T(2.3333333f));
+          else FF = T(0);
+          if (sph) {
+            if (MOD_SCM == 0 && j < JY) {
+              T r2 = L.R(L.r2, i, j);
+              XP = p1(i, j) - r2 * DD * (z2(ip1, j) - z2(i, j)) -
+                   r2 * DD / T(12.0f) *
+                       ((z2(ip1, jp1) - T(2) * z2(ip1, j) + z2(ip1, jm1)) -
+                        (z2(i, jp1) - T(2) * z2(i, j) + z2(i, jm1)));
+            } else {
+              XP = p1(i, j) - L.R(L.r2, i, j) * DD * (z2(ip1, j) - z2(i, j));
+            }
+            if (IFRIC != 1) XP = XP - FF * p1(i, j);
+            XP = XP + T(4.0f) * L.R(L.r3, i, j) * xqq;
+          } else {
+            XP = p1(i, j) - GRX * DD * (z2(ip1, j) - z2(i, j));
+            if (MOD_SCM == 0 && j < JY) {
+              T scm = GRX * DD / T(12.0f) *
+                      ((z2(ip1, jp1) - T(2) * z2(ip1, j) + z2(ip1, jm1)) -
+                       (z2(i, jp1) - T(2) * z2(i, j) + z2(i, jm1)));
+              XP = XP - scm;
+            }
+            if (IFRIC != 1) XP = XP - FF * p1(i, j);
+          }
+          if (lin_chk == 1 && dp1(i, j) >= GX &&
+              (i > ISPANS && i < ISPANE && j > JSPANS && j < JSPANE)) {
+            T fx = sph ? L.R(L.r21, i, j) : RX;
+            T fy = sph ? L.R(L.r0, i, j) : RY;
+            T advx = T(0), advy = T(0);
+            if (p1(i, j) < T(0)) {
+              if (dp1(ip1, j) < GX || dz2(ip1, j) < GX)
+                advx = fx * (-(p1(i, j) * p1(i, j) / dp1(i, j)));
+              else
+                advx = fx * (p1(ip1, j) * p1(ip1, j) / dp1(ip1, j) - p1(i, j) * p1(i, j) / dp1(i, j));
+            } else {
+              if (dp1(im1, j) < GX || dz2(i, j) < GX)
+                advx = fx * (p1(i, j) * p1(i, j) / dp1(i, j));
+              else
+                advx = fx * (p1(i, j) * p1(i, j) / dp1(i, j) - p1(im1, j) * p1(im1, j) / dp1(im1, j));
+            }
+            if (xqq < T(0)) {
+              if (dz2(i, jp1) < GX || dz2(ip1, jp1) < GX) {
+                advy = fy * (-(p1(i, j) * xqq / dp1(i, j)));
+              } else if (dp1(i, jp1) < GX) {
+                advy = fy * (-(p1(i, j) * xqq / dp1(i, j)));
+              } else {
+                T xqe = T(0.25f) * (q1(i, jp1) + q1(ip1, jp1) + q1(i, j) + q1(ip1, j));
+                advy = fy * (p1(i, jp1) * xqe / dp1(i, jp1) - p1(i, j) * xqq / dp1(i, j));
+              }
+            } else {
+              if (dz2(i, jm1) < GX || dz2(ip1, jm1) < GX) {
+                advy = fy * (p1(i, j) * xqq / dp1(i, j));
+              } else if (dp1(i, jm1) < GX) {
+                advy = fy * (p1(i, j) * xqq / dp1(i, j));
+              } else {
+                T xqe = T(0.25f) * (q1(i, jm1) + q1(ip1, jm1) + q1(i, j - 2) + q1(ip1, j - 2));
+                advy = fy * (p1(i, j) * xqq / dp1(i, j) - p1(i, jm1) * xqe / dp1(i, jm1));
+              }
+            }
+            wave_breaking(L, advx, advy, i, j);
+            XP = XP - advx - advy;
+          }
+          if (IFRIC != 1) XP = XP / (T(1) + FF);
+          if (m_abs(XP) < EPS) XP = T(0);
+          T lim = VL * DD;
+          if (XP > lim) XP = lim;
+          if (XP < -lim) XP = -lim;
+          L.M(i, j, 2) = XP;
+        }
+      }
+    }
+    // ---- Y direction
+    {
+      int is = 2, js = 2, ie = IX, je = JY - 1;
+      if (L.id == 1) { is = 1; js = 1; ie = IX - 1; je = JY - 1; }
+      for (int i = is; i <= ie; ++i) {
+        int ip1 = i + 1, im1 = i - 1;
+        if (im1 <= 1) im1 = 1;
+        if (ip1 >= IX) ip1 = IX;
+        for (int j = js; j <= je; ++j) {
+          int noflux = 0;
+          int jp1 = j + 1, jm1 = j - 1;
+          if (jm1 <= 1) jm1 = 1;
+          if (jp1 >= JY) jp1 = JY;
+          if (L.H(i, j) <= ELMAX || L.HQ(i, j) <= ELMAX) {
+            noflux = 1;
+          } else {
+            if (dz2(i, j) > GX && dz2(i, jp1) > GX) {
+              DD = DQ[nxy + L.ix(i, j)];
+              DF = dq1(i, j);
+            } else if (dz2(i, j) > GX && dz2(i, jp1) <= GX && L.H(i, jp1) + z2(i, j) > GX) {
+              DD = sph ? L.HQ(i, j) + T(0.5f) * (z2(i, j) + z2(i, jp1)) : T(0.5f) * dz2(i, j);
+              DF = DD;
+            } else if (dz2(i, j) <= GX && dz2(i, jp1) > GX && L.H(i, j) + z2(i, jp1) > GX) {
+              DD = sph ? L.HQ(i, j) + T(0.5f) * (z2(i, j) + z2(i, jp1)) : T(0.5f) * dz2(i, jp1);
+              DF = DD;
+            } else {
+              noflux = 1;
+            }
+            if (DD < GX) noflux = 1;
+          }
+          if (noflux == 1) { L.N(i, j, 2) = T(0); continue; }
+          if (DF < T(1.0e-3f)) DF = T(1.0e-3f);
+          T xpp = T(0.25f) * (p1(i, j) + p1(i, jp1) + p1(im1, j) + p1(im1, jp1));
+          if (IFRIC != 1)
+            FF = CONST * FM * FM * m_sqrt(q1(i, j) * q1(i, j) + xpp * xpp) / pw(DF, T(2.3333333f));
+          else FF = T(0);
+          if (sph) {
+            if (MOD_SCM == 0 && i < IX) {
+              T r4 = L.R(L.r4, i, j);
+              XQ = q1(i, j) - r4 * DD * (z2(i, jp1) - z2(i, j)) -
+                   r4 * DD / T(12.0f) *
+                       ((z2(ip1, jp1) - T(2) * z2(i, jp1) + z2(im1, jp1)) -
+                        (z2(ip1, j) - T(2) * z2(i, j) + z2(im1, j)));
+            } else {
+              XQ = q1(i, j) - L.R(L.r4, i, j) * DD * (z2(i, jp1) - z2(i, j));
+            }
+            if (IFRIC != 1) XQ = XQ - FF * q1(i, j);
+            XQ = XQ - T(4.0f) * L.R(L.r5, i, j) * xpp;
+          } else {
+            XQ = q1(i, j) - GRY * DD * (z2(i, jp1) - z2(i, j));
+            if (MOD_SCM == 0 && i < IX) {
+              T scm = GRY * DD / T(12.0f) *
+                      ((z2(ip1, jp1) - T(2) * z2(i, jp1) + z2(im1, jp1)) -
+                       (z2(ip1, j) - T(2) * z2(i, j) + z2(im1, j)));
+              XQ = XQ - scm;
+            }
+            // moment.f90:672 subtracts the friction from XP, not XQ
+            if (IFRIC != 1) XP = XP - FF * q1(i, j);
+          }
+          if (lin_chk == 1 && dq1(i, j) >= GX &&
+              (i > ISPANS && i < ISPANE && j > JSPANS && j < JSPANE)) {
+            T fy = sph ? L.R(L.r0, i, j) : RY;
+            T fx = sph ? L.R(L.r22, i, j) : RX;
+            T advx = T(0), advy = T(0);
+            if (q1(i, j) < T(0)) {
+              if (dq1(i, jp1) < GX || dz2(i, jp1) < GX)
+                advy = fy * (-(q1(i, j) * q1(i, j) / dq1(i, j)));
+              else
+                advy = fy * (q1(i, jp1) * q1(i, jp1) / dq1(i, jp1) - q1(i, j) * q1(i, j) / dq1(i, j));
+            } else {
+              if (dq1(i, jm1) < GX || dz2(i, j) < GX)
+                advy = fy * (q1(i, j) * q1(i, j) / dq1(i, j));
+              else
+                advy = fy * (q1(i, j) * q1(i, j) / dq1(i, j) - q1(i, jm1) * q1(i, jm1) / dq1(i, jm1));
+            }
+            if (xpp < T(0)) {
+              if (dz2(ip1, j) < GX || dz2(ip1, jp1) < GX) {
+                advx = fx * (-(q1(i, j) * xpp / dq1(i, j)));
+              } else if (dq1(ip1, j) < GX) {
+                advx = fx * (-(q1(i, j) * xpp / dq1(i, j)));
+              } else {
+                T xpe = T(0.25f) * (p1(ip1, j) + p1(ip1, jp1) + p1(i, j) + p1(i, jp1));
+                advx = fx * (q1(ip1, j) * xpe / dq1(ip1, j) - q1(i, j) * xpp / dq1(i, j));
+              }
+            } else {
+              if (dz2(im1, j) < GX || dz2(im1, jp1) < GX) {
+                advx = fx * (q1(i, j) * xpp / dq1(i, j));
+              } else if (dq1(i - 1, j) < GX) {
+                advx = fx * (q1(i, j) * xpp / dq1(i, j));
+              } else {
+                T xpe = T(0.25f) * (p1(im1, j) + p1(im1, jp1) + p1(i - 2, j) + p1(i - 2, jp1));
+                advx = fx * (q1(i, j) * xpp / dq1(i, j) - q1(im1, j) * xpe / dq1(im1, j));
+              }
+            }
+            wave_breaking(L, advx, advy, i, j);
+            XQ = XQ - advx - advy;
+          }
+          if (IFRIC != 1) XQ = XQ / (T(1) + FF);
+          if (m_abs(XQ) < EPS) XQ = T(0);
+          T lim = VL * DD;
+          if (XQ > lim) XQ = lim;
+          if (XQ < -lim) XQ = -lim;
+          L.N(i, j, 2) = XQ;
+        }
+      }
+    }
+  }
+
+  // MOMENT dispatcher (moment.f90:3-60)
+  void moment(Layer<T> &L) {
+    if (L.laycord == 0) {
+      if (L.laygov == 1 || L.laygov == 9) conmome(L, true);
+      else momt_s(L);
+    } else {
+      if (L.laygov == 1 || L.laygov == 5 || L.laygov == 9) conmome(L, false);
+      else momt_c(L);
+    }
+  }
+
+  // --------------------------------------------------------------- nesting
+  // EDGEINTERP_VERT (all_grids.f90:502-553); side 0 left (I=1), 1 right
+  void edgeinterp_vert(const std::vector<T> &flux, Layer<T> &C, int side) {
+    const T GX = T(kGx);
+    const int nn = 2 * C.rel_size, ir = C.rel_size;
+    const int ys = C.corners[2], ye = C.corners[3];
+    const int jshift = -ys * ir + 1;
+    const int ii = side == 0 ? 1 : C.nx;
+    for (int j = ys; j <= ye; ++j) {
+      T fs = T(0.5f) * (flux[j - 2] + flux[j - 1]);
+      T fe = T(0.5f) * (flux[j] + flux[j - 1]);
+      T c1 = (T)((double)(fe - fs) / (double)nn);
+      T c2 = -c1 + fs;
+      int js = j * ir + jshift;
+      for (int k = 1; k <= ir; ++k) {
+        int jj = js + k;
+        C.M(ii, jj, 1) = T(2.0f) * T(k) * c1 + c2;
+        if (C.H(ii, jj) + C.Z(ii, jj, 1) <= GX) C.M(ii, jj, 1) = T(0);
+      }
+    }
+  }
+
+  // EDGEINTERP_HORI (all_grids.f90:556-604); side 0 bottom (J=1), 1 top
+  void edgeinterp_hori(const std::vector<T> &flux, Layer<T> &C, int side) {
+    const T GX = T(kGx);
+    const int nn = 2 * C.rel_size, ir = C.rel_size;
+    const int xs = C.corners[0], xe = C.corners[1];
+    const int ishift = -xs * ir + 1;
+    const int jj = side == 0 ? 1 : C.ny;
+    for (int i = xs; i <= xe; ++i) {
+      T fs = T(0.5f) * (flux[i - 2] + flux[i - 1]);
+      T fe = T(0.5f) * (flux[i] + flux[i - 1]);
+      T c1 = (T)((double)(fe - fs) / (double)nn);
+      T c2 = -c1 + fs;
+      int is = i * ir + ishift;
+      for (int k = 1; k <= ir; ++k) {
+        int ii = is + k;
+        C.N(ii, jj, 1) = T(2.0f) * T(k) * c1 + c2;
+        if (C.H(ii, jj) + C.Z(ii, jj, 1) <= GX) C.N(ii, jj, 1) = T(0);
+      }
+    }
+  }
+
+  // JNQ (all_grids.f90:452-498), SC_OPTION = 0
+  void jnq(Layer<T> &P, Layer<T> &C) {
+    const int is = C.corners[0], ie = C.corners[1], js = C.corners[2], je = C.corners[3];
+    std::vector<T> f(P.ny);
+    for (int j = 1; j <= P.ny; ++j) f[j - 1] = P.M(is - 1, j, 1);
+    edgeinterp_vert(f, C, 0);
+    for (int j = 1; j <= P.ny; ++j) f[j - 1] = P.M(ie, j, 1);
+    edgeinterp_vert(f, C, 1);
+    std::vector<T> g(P.nx);
+    for (int i = 1; i <= P.nx; ++i) g[i - 1] = P.N(i, js - 1, 1);
+    edgeinterp_hori(g, C, 0);
+    for (int i = 1; i <= P.nx; ++i) g[i - 1] = P.N(i, je, 1);
+    edgeinterp_hori(g, C, 1);
+  }
+
+  // NEWQ_VERT (all_grids.f90:663-731)
+  void newq_vert(Layer<T> &P, Layer<T> &C, int t, int side) {
+    const T GX = T(kGx), EPS = T(kEps);
+    std::vector<T> yf(P.ny, T(0));
+    T xm = T(0);
+    const int is = C.corners[0], ie = C.corners[1], js = C.corners[2], je = C.corners[3];
+    const int i = side == 0 ? is - 1 : ie;
+    const T c1 = (T(t) - T(1.0f)) / T(C.rel_time);
+    const T c2 = T(1.0f) - c1;
+    const int ip1 = i + 1;
+    for (int j = js - 2; j <= je + 2; ++j) {
+      if ((P.H(i, j) + P.Z(i, j, 2) > GX) && (P.H(ip1, j) + P.Z(ip1, j, 2) > GX)) {
+        if (t == 2) {
+          if (P.laycord == 0) {
+            int jm1 = j - 1;
+            T tot_n = P.N(i, j, 1) + P.N(ip1, j, 1) + P.N(i, jm1, 1) + P.N(ip1, jm1, 1);
+            xm = P.M(i, j, 1) - P.R(P.r2, i, j) * (P.Z(ip1, j, 2) - P.Z(i, j, 2)) +
+                 P.R(P.r3, i, j) * tot_n;
+          } else {
+            T hm = P.HP(i, j) + T(0.5f) * (P.Z(i, j, 2) + P.Z(ip1, j, 2));
+            xm = P.M(i, j, 1) - P.grx * hm * (P.Z(ip1, j, 2) - P.Z(i, j, 2));
+          }
+          if (m_abs(xm) < EPS) xm = T(0);
+          P.yflux[side * P.ny + (j - 1)] = xm;
+        }
+        yf[j - 1] = c1 * P.yflux[side * P.ny + (j - 1)] + c2 * P.M(i, j, 1);
+      }
+    }
+    edgeinterp_vert(yf, C, side);
+  }
+
+  // NEWQ_HORI (all_grids.f90:734-801)
+  void newq_hori(Layer<T> &P, Layer<T> &C, int t, int side) {
+    const T GX = T(kGx), EPS = T(kEps);
+    std::vector<T> xf(P.nx, T(0));
+    T xn = T(0);
+    const int is = C.corners[0], ie = C.corners[1], js = C.corners[2], je = C.corners[3];
+    const int j = side == 0 ? js - 1 : je;
+    const T c1 = (T(t) - T(1.0f)) / T(C.rel_time);
+    const T c2 = T(1) - c1;
+    const int jp1 = j + 1;
+    for (int i = is - 2; i <= ie + 2; ++i) {
+      if ((P.H(i, j) + P.Z(i, j, 2) > GX) && (P.H(i, jp1) + P.Z(i, jp1, 2) > GX)) {
+        if (t == 2) {
+          int im1 = i - 1;
+          if (P.laycord == 0) {
+            T tot_m = P.M(im1, j, 1) + P.M(im1, jp1, 1) + P.M(i, j, 1) + P.M(i, jp1, 1);
+            xn = P.N(i, j, 1) - P.R(P.r4, i, j) * (P.Z(i, jp1, 2) - P.Z(i, j, 2)) -
+                 P.R(P.r5, i, j) * tot_m;
+          } else {
+            T hn = P.HQ(i, j) + T(0.5f) * (P.Z(i, j, 2) + P.Z(i, jp1, 2));
+            xn = P.N(i, j, 1) - P.gry * hn * (P.Z(i, jp1, 2) - P.Z(i, j, 2));
+          }
+          if (m_abs(xn) < EPS) xn = T(0);
+          P.xflux[side * P.nx + (i - 1)] = xn;
+        }
+        xf[i - 1] = c1 * P.xflux[side * P.nx + (i - 1)] + c2 * P.N(i, j, 1);
+      }
+    }
+    edgeinterp_hori(xf, C, side);
+  }
+
+  // NEWQ (all_grids.f90:608-659), SC_OPTION = 0
+  void newq(Layer<T> &P, Layer<T> &C, int t) {
+    newq_vert(P, C, t, 0);
+    newq_vert(P, C, t, 1);
+    newq_hori(P, C, t, 0);
+    newq_hori(P, C, t, 1);
+  }
+
+  // JNZ (all_grids.f90:342-421), SC_OPTION = 0
+  void jnz(Layer<T> &P, Layer<T> &C) {
+    const T GX = T(kGx);
+    const T half = T((float)(C.rel_size * C.rel_size) / 2.0f);
+    const int ir = C.rel_size;
+    const int is = C.corners[0], ie = C.corners[1], js = C.corners[2], je = C.corners[3];
+    const int ishift = -is * ir + 1, jshift = -js * ir + 1;
+    for (int i = is; i <= ie; ++i) {
+      for (int j = js; j <= je; ++j) {
+        T sum = T(0), count = T(0);
+        int i0 = i * ir + ishift, j0 = j * ir + jshift;
+        for (int ki = 1; ki <= ir; ++ki) {
+          for (int kj = 1; kj <= ir; ++kj) {
+            int ii = i0 + ki, jj = j0 + kj;
+            if (C.H(ii, jj) + C.Z(ii, jj, 2) > GX) {
+              if (C.rel_time % 2 == 0) sum = sum + T(0.5f) * (C.Z(ii, jj, 1) + C.Z(ii, jj, 2));
+              else sum = sum + C.Z(ii, jj, 2);
+              count = count + T(1.0f);
+            }
+          }
+        }
+        if (count > half) P.Z(i, j, 2) = sum / count;
+        else P.Z(i, j, 2) = T(0);
+      }
+    }
+  }
+
+  // UPDATE (all_grids.f90:316-339)
+  void update(Layer<T> &C) {
+    for (int j = 1; j <= C.ny; ++j) C.Z(1, j, 2) = C.Z(2, j, 2);
+    for (int i = 1; i <= C.nx; ++i) C.Z(i, 1, 2) = C.Z(i, 2, 2);
+    const size_t n = C.nxy();
+    std::memcpy(&C.z[0], &C.z[n], n * sizeof(T));
+    std::memcpy(&C.m[0], &C.m[n], n * sizeof(T));
+    std::memcpy(&C.n[0], &C.n[n], n * sizeof(T));
+  }
+
+  // ------------------------------------------------------- max and arrival
+  // Max elevation after a layer step.  Wet-only form: the CPU MAX_AMP
+  // (output.f90:876-885, starts at 0 per ALLOC comcot.f90:631).  Raw form:
+  // the GPU recorder (GPUOutput.cu:46-53, starts at the initial Z per
+  // GPUcomcot.cu:79).  First arrival (no reference counterpart): the root
+  // step index at whose end a wet cell first reaches the threshold.
+  // Child layers skip their ghost row I=1 and column J=1 (copies of the
+  // neighbours made by UPDATE, all_grids.f90:323-324).
+  void max_arrival(Layer<T> &L) {
+    const T GX = T(kGx);
+    const size_t n = L.nxy();
+    for (size_t k = 0; k < n; ++k) {
+      if (L.parent != -1 && ((int)(k % L.nx) == 0 || (int)(k / L.nx) == 0)) continue;
+      T z2 = L.z[n + k];
+      bool wet = L.h[k] + z2 > GX;
+      if (L.zmax_raw[k] < z2) L.zmax_raw[k] = z2;
+      if (wet) {
+        if (z2 >= arrival_thr && !(L.zmax_wet[k] >= arrival_thr)) L.arrival[k] = nstep;
+        if (z2 > L.zmax_wet[k]) L.zmax_wet[k] = z2;
+      }
+    }
+  }
+
+  // ALL_GRID (all_grids.f90:4-312), recursive form of the unrolled levels
+  void all_grid_children(Layer<T> &P) {
+    for (int ci : P.children) {
+      Layer<T> &C = layers[ci];
+      int nhalf = (int)std::floor(C.rel_time / 2.0f) + 1;
+      for (int k = 1; k <= C.rel_time; ++k) {
+        if (k == 1) jnq(P, C);
+        else newq(P, C, k);
+        mass(C);
+        all_grid_children(C);
+        moment(C);
+        max_arrival(C);
+        if (k == nhalf) jnz(P, C);
+        update(C);
+      }
+    }
+  }
+
+  // one root step: comcot.f90:359-439 with CHANGE's copies (:487-489)
+  void step() {
+    Layer<T> &R = layers[0];
+    nstep += 1;
+    mass(R);
+    open_bc(R);
+    if (!R.children.empty()) all_grid_children(R);
+    moment(R);
+    max_arrival(R);
+    const size_t n = R.nxy();
+    std::memcpy(&R.z[0], &R.z[n], n * sizeof(T));
+    std::memcpy(&R.m[0], &R.m[n], n * sizeof(T));
+    std::memcpy(&R.n[0], &R.n[n], n * sizeof(T));
+  }
+};
+
+struct Handle {
+  int precision;
+  std::unique_ptr<State<float>> f;
+  std::unique_ptr<State<double>> d;
+};
+
+template <class T>
+int add_layer(State<T> &S, int id, int parent, int laycord, int laygov, int nx, int ny,
+              double dx, double dy, double dt, const double *y, const double *dely,
+              const double *h, const double *z0, int fric_switch, double fric_coef,
+              int rel_size, int rel_time, const int *corners) {
+  if (nx < 3 || ny < 3) return 1;
+  if (parent == -1 && !S.layers.empty()) return 2;
+  if (parent != -1 && S.find(parent) < 0) return 3;
+  Layer<T> L;
+  L.id = id;
+  L.parent = parent;
+  L.nx = nx;
+  L.ny = ny;
+  L.laycord = laycord;
+  L.laygov = laygov;
+  L.fric_switch = fric_switch;
+  L.fric_coef = T((float)fric_coef);
+  L.dt = T((float)dt);
+  L.dx = T((float)dx);
+  L.dy = T((float)dy);
+  L.rel_size = rel_size;
+  L.rel_time = rel_time;
+  // ALLOC (comcot.f90:537-552, 602)
+  L.rx = L.dt / L.dx;
+  L.ry = L.dt / L.dy;
+  L.grx = T(kGrav) * L.rx;
+  L.gry = T(kGrav) * L.ry;
+  L.linchk = 1;
+  L.modscm = (parent == -1) ? 0 : 1;
+  if (laygov % 2 != 0) L.modscm = 1;
+  if (parent == -1) {
+    L.corners[0] = 1; L.corners[1] = nx; L.corners[2] = 1; L.corners[3] = ny;
+  } else {
+    for (int k = 0; k < 4; ++k) L.corners[k] = corners[k];
+  }
+  const size_t n = (size_t)nx * ny;
+  L.h.assign(n, T(0));
+  L.hp.assign(n, T(0));
+  L.hq.assign(n, T(0));
+  L.z.assign(2 * n, T(0));
+  L.m.assign(2 * n, T(0));
+  L.n.assign(2 * n, T(0));
+  L.dz.assign(2 * n, T(0));
+  for (size_t k = 0; k < n; ++k) {
+    L.h[k] = T((float)h[k]);
+    L.z[k] = T((float)z0[k]);
+  }
+  L.xflux.assign(2 * (size_t)nx, T(0));
+  L.yflux.assign(2 * (size_t)ny, T(0));
+  L.zmax_wet.assign(n, T(0));
+  L.zmax_raw.assign(L.z.begin(), L.z.begin() + n);
+  L.arrival.assign(n, -1);
+  for (auto *r : {&L.r0, &L.r1, &L.r11, &L.r2, &L.r21, &L.r22, &L.r3, &L.r4, &L.r5})
+    r->assign(laycord == 0 ? n : 0, T(0));
+  L.r6.assign(laycord == 0 ? ny : 0, T(0));
+  S.pq_depth(L);
+  if (laycord == 0) {
+    std::vector<T> yy(ny), dd(ny);
+    for (int j = 0; j < ny; ++j) { yy[j] = T((float)y[j]); dd[j] = T((float)dely[j]); }
+    S.para(L, yy, dd);
+  }
+  S.layers.push_back(std::move(L));
+  if (parent != -1) {
+    int pk = S.find(parent);
+    S.layers[pk].children.push_back((int)S.layers.size() - 1);
+    S.layers.back().level = S.layers[pk].level + 1;
+  }
+  return 0;
+}
+
+template <class T>
+void state_io(State<T> &S, int id, int which, double *z, double *m, double *n, double *dz) {
+  int k = S.find(id);
+  if (k < 0) return;
+  Layer<T> &L = S.layers[k];
+  const size_t nn = 2 * L.nxy();
+  for (size_t q = 0; q < nn; ++q) {
+    if (which == 0) {
+      z[q] = L.z[q]; m[q] = L.m[q]; n[q] = L.n[q]; dz[q] = L.dz[q];
+    } else {
+      L.z[q] = T(z[q]); L.m[q] = T(m[q]); L.n[q] = T(n[q]); L.dz[q] = T(dz[q]);
+    }
+  }
+}
+
+template <class T>
+void call_op(State<T> &S, int op, int id, int t) {
+  int k = S.find(id);
+  if (k < 0) return;
+  Layer<T> &L = S.layers[k];
+  Layer<T> *P = nullptr;
+  if (L.parent != -1) P = &S.layers[S.find(L.parent)];
+  switch (op) {
+    case 1: S.mass(L); break;
+    case 2: S.open_bc(L); break;
+    case 3: S.moment(L); break;
+    case 4: if (P) S.jnq(*P, L); break;
+    case 5: if (P) S.newq(*P, L, t); break;
+    case 6: if (P) S.jnz(*P, L); break;
+    case 7: S.update(L); break;
+    case 8: S.all_grid_children(S.layers[0]); break;
+    case 9: {
+      const size_t n = L.nxy();
+      std::memcpy(&L.z[0], &L.z[n], n * sizeof(T));
+      std::memcpy(&L.m[0], &L.m[n], n * sizeof(T));
+      std::memcpy(&L.n[0], &L.n[n], n * sizeof(T));
+      break;
+    }
+    case 10: S.max_arrival(L); break;
+    default: break;
+  }
+}
+
+template <class T>
+void setup_io_t(State<T> &S, int id, double *hp, double *hq, double *rpl, double *r6,
+                       double *sc) {
+  int k = S.find(id);
+  if (k < 0) return;
+  auto &L = S.layers[k];
+  const size_t n = L.nxy();
+  for (size_t q = 0; q < n; ++q) { hp[q] = L.hp[q]; hq[q] = L.hq[q]; }
+  const std::vector<T> *planes[9] = {&L.r0, &L.r1, &L.r11, &L.r2, &L.r21,
+                                     &L.r22, &L.r3, &L.r4, &L.r5};
+  for (int p = 0; p < 9; ++p)
+    for (size_t q = 0; q < n; ++q) rpl[p * n + q] = planes[p]->empty() ? 0.0 : (*planes[p])[q];
+  for (int j = 0; j < L.ny; ++j) r6[j] = L.r6.empty() ? 0.0 : L.r6[j];
+  sc[0] = L.rx; sc[1] = L.ry; sc[2] = L.grx; sc[3] = L.gry; sc[4] = L.dt;
+}
+
+template <class T>
+void flux_io_t(State<T> &S, int id, double *xf, double *yf) {
+  int k = S.find(id);
+  if (k < 0) return;
+  auto &L = S.layers[k];
+  for (size_t q = 0; q < L.xflux.size(); ++q) xf[q] = L.xflux[q];
+  for (size_t q = 0; q < L.yflux.size(); ++q) yf[q] = L.yflux[q];
+}
+
+template <class T>
+void max_io_t(State<T> &S, int id, double *zw, double *zr, int *arr) {
+  int k = S.find(id);
+  if (k < 0) return;
+  auto &L = S.layers[k];
+  for (size_t q = 0; q < L.nxy(); ++q) { zw[q] = L.zmax_wet[q]; zr[q] = L.zmax_raw[q]; arr[q] = L.arrival[q]; }
+}
+
+}  // namespace
+
+extern "C" {
+
+void *orc_new(int precision, int pow_mode) {
+  Handle *h = new Handle();
+  h->precision = precision == 64 ? 64 : 32;
+  if (h->precision == 32) { h->f.reset(new State<float>()); h->f->pow_mode = pow_mode; }
+  else { h->d.reset(new State<double>()); h->d->pow_mode = pow_mode; }
+  return h;
+}
+
+void orc_free(void *s) { delete (Handle *)s; }
+
+int orc_add_layer(void *s, int id, int parent, int laycord, int laygov, int nx, int ny,
+                  double dx, double dy, double dt, const double *y, const double *dely,
+                  const double *h, const double *z0, int fric_switch, double fric_coef,
+                  int rel_size, int rel_time, const int *corners) {
+  Handle *H = (Handle *)s;
+  if (H->precision == 32)
+    return add_layer(*H->f, id, parent, laycord, laygov, nx, ny, dx, dy, dt, y, dely, h, z0,
+                     fric_switch, fric_coef, rel_size, rel_time, corners);
+  return add_layer(*H->d, id, parent, laycord, laygov, nx, ny, dx, dy, dt, y, dely, h, z0,
+                   fric_switch, fric_coef, rel_size, rel_time, corners);
+}
+
+void orc_step(void *s, int nsteps) {
+  Handle *H = (Handle *)s;
+  for (int k = 0; k < nsteps; ++k) {
+    if (H->precision == 32) H->f->step(); else H->d->step();
+  }
+}
+
+void orc_call(void *s, int op, int id, int t) {
+  Handle *H = (Handle *)s;
+  if (H->precision == 32) call_op(*H->f, op, id, t); else call_op(*H->d, op, id, t);
+}
+
+void orc_state_io(void *s, int id, int which, double *z, double *m, double *n, double *dz) {
+  Handle *H = (Handle *)s;
+  if (H->precision == 32) state_io(*H->f, id, which, z, m, n, dz);
+  else state_io(*H->d, id, which, z, m, n, dz);
+}
+
+void orc_setup_io(void *s, int id, double *hp, double *hq, double *rpl, double *r6, double *sc) {
+  Handle *H = (Handle *)s;
+  if (H->precision == 32) setup_io_t(*H->f, id, hp, hq, rpl, r6, sc);
+  else setup_io_t(*H->d, id, hp, hq, rpl, r6, sc);
+}
+
+void orc_flux_io(void *s, int id, double *xflux, double *yflux) {
+  Handle *H = (Handle *)s;
+  if (H->precision == 32) flux_io_t(*H->f, id, xflux, yflux);
+  else flux_io_t(*H->d, id, xflux, yflux);
+}
+
+void orc_max_io(void *s, int id, double *zmax_wet, double *zmax_raw, int *arrival) {
+  Handle *H = (Handle *)s;
+  if (H->precision == 32) max_io_t(*H->f, id, zmax_wet, zmax_raw, arrival);
+  else max_io_t(*H->d, id, zmax_wet, zmax_raw, arrival);
+}
+
+void orc_set_arrival_threshold(void *s, double thr) {
+  Handle *H = (Handle *)s;
+  if (H->precision == 32) H->f->arrival_thr = (float)thr; else H->d->arrival_thr = thr;
+}
+
+// CR_CHECK (initialization.f90:1163-1196) for one root layer, FP32
+double orc_cr_dt(int laycord, int laygov, int nx, int ny, const double *h, double dx, double dy,
+                 double lat_first, double lat_last, double dt_in) {
+  float h_max = kGx;
+  for (int i = 0; i < nx; ++i)
+    for (int j = 0; j < ny; ++j) {
+      float v = (float)h[(size_t)j * nx + i];
+      if (v > h_max) h_max = v;
+    }
+  float DX, DY;
+  if (laycord == 0) {
+    float lat_max = std::max(fabsf((float)lat_first), fabsf((float)lat_last)) * kRadDeg;
+    DX = kREarth * cosf(lat_max) * ((float)dx * kRadMin);
+    DY = kREarth * ((float)dy * kRadMin);
+  } else {
+    DX = (float)dx;
+    DY = (float)dy;
+  }
+  float del_x = std::min(DX, DY);
+  float dt = (float)dt_in;
+  float cr = (float)dt_in / (del_x / sqrtf(kGrav * h_max));
+  float cr_limit = 0.5f;
+  if (laygov == 1 || laygov == 3) cr_limit = 0.35f;
+  if (cr > cr_limit) dt = cr_limit * del_x / sqrtf(kGrav * h_max);
+  float out = (float)dt_in;
+  if (dt <= out) out = dt;
+  return out;
+}
+
+}  // extern "C"

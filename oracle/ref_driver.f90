@@ -1,0 +1,330 @@
+!-----------------------------------------------------------------------
+! Test-only driver around the flang-built COMCOT reference Fortran.
+!
+! This file is part of the ORACLE (test infrastructure): it is linked
+! with objects that oracle/Makefile compiles from the reference sources
+! where they lie, into oracle/_ref/libcomcot_ref.so.  Only tests/,
+! tests/golden/make_golden.py and bench.py's cpu_baseline leg load it,
+! as a checker and as the timed CPU baseline; the product never does.
+!
+! It replays the CPU path of the reference main loop (comcot.f90:322-440
+! with the DEBUG_CORE body of CHANGE, comcot.f90:480-505) on layers it
+! builds itself, so that the reference's own MASS/OPEN/ALL_GRID/MOMENT
+! code computes every value:
+!   setup      ALLOC (comcot.f90:516), PQ_DEPTH (initialization.f90:2501),
+!              PARA (comcot.f90:694)
+!   one step   MASS (mass.f90:3) -> OPEN (boundaries.f90:22)
+!              -> ALL_GRID (all_grids.f90:4) -> MOMENT (moment.f90:3)
+!              -> level-2 to level-1 copy of Z, M, N (comcot.f90:487-489)
+! Single routines (JNQ, NEWQ, JNZ, UPDATE, ...) can be called one at a
+! time for step-local fixtures, like the reference's DEBUG_CORE mode.
+!-----------------------------------------------------------------------
+module ref_state
+  use layer_params
+  implicit none
+  type(layer), save :: lo
+  type(layer), allocatable, save :: la(:)
+  integer, save :: nla = 0
+contains
+
+
+!.....resolve a layer id (1 = root, k+1 = la(k)); stops on a bad id
+subroutine ref_pick(id, k)
+  implicit none
+  integer :: id, k
+  if (id .eq. 1) then
+     k = 0
+  else if (id .ge. 2 .and. id-1 .le. nla) then
+     k = id - 1
+  else
+     write(*,*) 'ref_driver: bad layer id', id
+     stop 1
+  end if
+end subroutine ref_pick
+
+subroutine ref_reset() bind(C, name="ref_reset")
+  implicit none
+  integer :: i
+  integer :: ngrid
+  real :: elmax, grav, pi, r_earth, gx, eps, zero, one, v_limit
+  real :: rad_deg, rad_min
+  integer :: num_flt
+  common /cons/ elmax, grav, pi, r_earth, gx, eps, zero, one, ngrid,  &
+                num_flt, v_limit, rad_deg, rad_min
+  if (allocated(la)) deallocate(la)
+  allocate(la(ngrid))
+  do i = 1, ngrid
+     la(i)%layswitch = 1
+     la(i)%parent = -99
+     la(i)%id = i + 1
+     la(i)%num_child = 0
+  end do
+  nla = 0
+  lo%num_child = 0
+end subroutine ref_reset
+
+!.....common per-layer fields; arrays Y/DEL_Y/X are only used by PARA
+subroutine ref_fill(l, nx, ny, laycord, laygov, dx, dy, dt, y, dely,    &
+                    fric_switch, fric_coef)
+  implicit none
+  type(layer) :: l
+  integer :: nx, ny, laycord, laygov, fric_switch
+  real :: dx, dy, dt, fric_coef
+  real :: y(ny), dely(ny)
+  l%nx = nx
+  l%ny = ny
+  l%laycord = laycord
+  l%laygov = laygov
+  l%layswitch = 0
+  l%dx = dx
+  l%dy = dy
+  l%dt = dt
+  l%ini_switch = 0
+  l%bc_type = 0
+  l%dim = 2
+  l%fric_switch = fric_switch
+  l%fric_coef = fric_coef
+  l%h_limit = 0.0
+  l%tide_level = 0.0
+  l%sc_option = 0
+  l%num_child = 0
+  allocate(l%y(ny))
+  allocate(l%del_y(ny))
+  allocate(l%x(nx))
+  l%y = y
+  l%del_y = dely
+  l%x = 0.0
+end subroutine ref_fill
+
+subroutine ref_init_root(nx, ny, laycord, laygov, dx, dy, dt, y, dely,  &
+                         h, z0, fric_switch, fric_coef)               &
+                         bind(C, name="ref_init_root")
+  implicit none
+  integer :: nx, ny, laycord, laygov, fric_switch
+  real :: dx, dy, dt, fric_coef
+  real :: y(ny), dely(ny), h(nx*ny), z0(nx*ny)
+  call ref_reset()
+  lo%id = 1
+  lo%parent = -1
+  lo%level = 1
+  lo%rel_size = 1
+  lo%rel_time = 1
+  call ref_fill(lo, nx, ny, laycord, laygov, dx, dy, dt, y, dely,       &
+                fric_switch, fric_coef)
+  call alloc(lo, 1)
+  lo%h = reshape(h, (/nx, ny/))
+  lo%z(:,:,1) = reshape(z0, (/nx, ny/))
+  call pq_depth(lo)
+  if (laycord .eq. 0) call para(lo)
+end subroutine ref_init_root
+
+subroutine ref_init_child(parent, laycord, laygov, rel_size, rel_time,  &
+                          corners, nx, ny, dx, dy, dt, y, dely, h, z0,  &
+                          fric_switch, fric_coef)                       &
+                          bind(C, name="ref_init_child")
+  implicit none
+  integer :: parent, laycord, laygov, rel_size, rel_time, nx, ny
+  integer :: corners(4), fric_switch
+  real :: dx, dy, dt, fric_coef
+  real :: y(ny), dely(ny), h(nx*ny), z0(nx*ny)
+  integer :: kp
+  nla = nla + 1
+  call ref_pick(parent, kp)
+  la(nla)%id = nla + 1
+  la(nla)%parent = parent
+  if (kp .eq. 0) then
+     la(nla)%level = lo%level + 1
+     lo%num_child = lo%num_child + 1
+  else
+     la(nla)%level = la(kp)%level + 1
+     la(kp)%num_child = la(kp)%num_child + 1
+  end if
+  la(nla)%rel_size = rel_size
+  la(nla)%rel_time = rel_time
+  call ref_fill(la(nla), nx, ny, laycord, laygov, dx, dy, dt, y, dely,  &
+                fric_switch, fric_coef)
+  call alloc(la(nla), 2)
+  la(nla)%corners = corners
+  la(nla)%h = reshape(h, (/nx, ny/))
+  la(nla)%z(:,:,1) = reshape(z0, (/nx, ny/))
+  call pq_depth(la(nla))
+  if (laycord .eq. 0) call para(la(nla))
+end subroutine ref_init_child
+
+!.....nsteps root steps of the reference CPU time loop
+subroutine ref_step(nsteps) bind(C, name="ref_step")
+  implicit none
+  integer :: nsteps
+  integer :: k
+  do k = 1, nsteps
+     call mass(lo)
+     call open(lo)
+     if (lo%num_child .ge. 1) call all_grid(lo, la)
+     call moment(lo)
+     lo%z(:,:,1) = lo%z(:,:,2)
+     lo%m(:,:,1) = lo%m(:,:,2)
+     lo%n(:,:,1) = lo%n(:,:,2)
+  end do
+end subroutine ref_step
+
+!.....one reference routine on one layer (step-local fixtures)
+!     op: 1 MASS, 2 OPEN, 3 MOMENT, 4 JNQ(parent,id), 5 NEWQ(parent,id,t)
+!         6 JNZ(parent,id), 7 UPDATE(id), 8 ALL_GRID(root), 9 root copy
+subroutine ref_call(op, id, t) bind(C, name="ref_call")
+  implicit none
+  integer :: op, id, t
+  integer :: k, kp
+  call ref_pick(id, k)
+  select case (op)
+  case (1)
+     if (k .eq. 0) then
+        call mass(lo)
+     else
+        call mass(la(k))
+     end if
+  case (2)
+     call open(lo)
+  case (3)
+     if (k .eq. 0) then
+        call moment(lo)
+     else
+        call moment(la(k))
+     end if
+  case (4, 5, 6)
+     call ref_pick(la(k)%parent, kp)
+     if (kp .eq. 0) then
+        if (op .eq. 4) call jnq(lo, la(k))
+        if (op .eq. 5) call newq(lo, la(k), t)
+        if (op .eq. 6) call jnz(lo, la(k))
+     else
+        if (op .eq. 4) call jnq(la(kp), la(k))
+        if (op .eq. 5) call newq(la(kp), la(k), t)
+        if (op .eq. 6) call jnz(la(kp), la(k))
+     end if
+  case (7)
+     call update(la(k))
+  case (8)
+     call all_grid(lo, la)
+  case (9)
+     lo%z(:,:,1) = lo%z(:,:,2)
+     lo%m(:,:,1) = lo%m(:,:,2)
+     lo%n(:,:,1) = lo%n(:,:,2)
+  case default
+     write(*,*) 'ref_driver: bad op', op
+     stop 1
+  end select
+end subroutine ref_call
+
+!.....copy one layer's state out (which: 0 get, 1 set)
+!     planes: z(:,:,1:2), m(:,:,1:2), n(:,:,1:2), dz(:,:,1:2), each nx*ny
+subroutine ref_state_io(id, which, z, m, n, dz) bind(C, name="ref_state_io")
+  implicit none
+  integer :: id, which
+  real :: z(*), m(*), n(*), dz(*)
+  integer :: k, nxy
+  call ref_pick(id, k)
+  if (k .eq. 0) then
+     nxy = lo%nx*lo%ny
+     if (which .eq. 0) then
+        z(1:2*nxy) = reshape(lo%z, (/2*nxy/))
+        m(1:2*nxy) = reshape(lo%m, (/2*nxy/))
+        n(1:2*nxy) = reshape(lo%n, (/2*nxy/))
+        dz(1:2*nxy) = reshape(lo%dz, (/2*nxy/))
+     else
+        lo%z = reshape(z(1:2*nxy), (/lo%nx, lo%ny, 2/))
+        lo%m = reshape(m(1:2*nxy), (/lo%nx, lo%ny, 2/))
+        lo%n = reshape(n(1:2*nxy), (/lo%nx, lo%ny, 2/))
+        lo%dz = reshape(dz(1:2*nxy), (/lo%nx, lo%ny, 2/))
+     end if
+  else
+     nxy = la(k)%nx*la(k)%ny
+     if (which .eq. 0) then
+        z(1:2*nxy) = reshape(la(k)%z, (/2*nxy/))
+        m(1:2*nxy) = reshape(la(k)%m, (/2*nxy/))
+        n(1:2*nxy) = reshape(la(k)%n, (/2*nxy/))
+        dz(1:2*nxy) = reshape(la(k)%dz, (/2*nxy/))
+     else
+        la(k)%z = reshape(z(1:2*nxy), (/la(k)%nx, la(k)%ny, 2/))
+        la(k)%m = reshape(m(1:2*nxy), (/la(k)%nx, la(k)%ny, 2/))
+        la(k)%n = reshape(n(1:2*nxy), (/la(k)%nx, la(k)%ny, 2/))
+        la(k)%dz = reshape(dz(1:2*nxy), (/la(k)%nx, la(k)%ny, 2/))
+     end if
+  end if
+end subroutine ref_state_io
+
+!.....setup products of one layer: HP, HQ (planes), the spherical
+!     coefficient planes R0,R1,R11,R2,R21,R22,R3,R4,R5 (9 planes, zero for
+!     Cartesian layers), R6 (ny), and RX, RY, GRX, GRY, DT (5 scalars)
+subroutine ref_setup_io(id, hp, hq, rpl, r6, sc) bind(C, name="ref_setup_io")
+  implicit none
+  integer :: id
+  real :: hp(*), hq(*), rpl(*), r6(*), sc(5)
+  integer :: k
+  call ref_pick(id, k)
+  if (k .eq. 0) then
+     call ref_setup_one(lo, hp, hq, rpl, r6, sc)
+  else
+     call ref_setup_one(la(k), hp, hq, rpl, r6, sc)
+  end if
+end subroutine ref_setup_io
+
+subroutine ref_setup_one(l, hp, hq, rpl, r6, sc)
+  implicit none
+  type(layer) :: l
+  real :: hp(*), hq(*), rpl(*), r6(*), sc(5)
+  integer :: nxy
+  nxy = l%nx*l%ny
+  hp(1:nxy) = reshape(l%hp, (/nxy/))
+  hq(1:nxy) = reshape(l%hq, (/nxy/))
+  rpl(1:9*nxy) = 0.0
+  r6(1:l%ny) = 0.0
+  if (l%laycord .eq. 0) then
+     rpl(1:nxy) = reshape(l%r0, (/nxy/))
+     rpl(nxy+1:2*nxy) = reshape(l%r1, (/nxy/))
+     rpl(2*nxy+1:3*nxy) = reshape(l%r11, (/nxy/))
+     if (l%laygov .le. 1) rpl(3*nxy+1:4*nxy) = reshape(l%r2, (/nxy/))
+     rpl(4*nxy+1:5*nxy) = reshape(l%r21, (/nxy/))
+     rpl(5*nxy+1:6*nxy) = reshape(l%r22, (/nxy/))
+     rpl(6*nxy+1:7*nxy) = reshape(l%r3, (/nxy/))
+     if (l%laygov .le. 1) rpl(7*nxy+1:8*nxy) = reshape(l%r4, (/nxy/))
+     rpl(8*nxy+1:9*nxy) = reshape(l%r5, (/nxy/))
+     r6(1:l%ny) = l%r6
+  end if
+  sc(1) = l%rx
+  sc(2) = l%ry
+  sc(3) = l%grx
+  sc(4) = l%gry
+  sc(5) = l%dt
+end subroutine ref_setup_one
+
+!.....parent-side interface flux cache written by NEWQ (all_grids.f90:720,790)
+subroutine ref_flux_io(id, xflux, yflux) bind(C, name="ref_flux_io")
+  implicit none
+  integer :: id
+  real :: xflux(*), yflux(*)
+  integer :: k
+  call ref_pick(id, k)
+  if (k .eq. 0) then
+     xflux(1:2*lo%nx) = reshape(lo%xflux, (/2*lo%nx/))
+     yflux(1:2*lo%ny) = reshape(lo%yflux, (/2*lo%ny/))
+  else
+     xflux(1:2*la(k)%nx) = reshape(la(k)%xflux, (/2*la(k)%nx/))
+     yflux(1:2*la(k)%ny) = reshape(la(k)%yflux, (/2*la(k)%ny/))
+  end if
+end subroutine ref_flux_io
+
+!.....time-step check of the reference (initialization.f90:1136-1307):
+!     returns the adjusted root DT and each child's REL_TIME and DT
+subroutine ref_cr_check(dt_root, rel_time, dt_child) bind(C, name="ref_cr_check")
+  implicit none
+  real :: dt_root, dt_child(*)
+  integer :: rel_time(*)
+  integer :: i
+  call cr_check(lo, la)
+  dt_root = lo%dt
+  do i = 1, nla
+     rel_time(i) = la(i)%rel_time
+     dt_child(i) = la(i)%dt
+  end do
+end subroutine ref_cr_check
+end module ref_state
