@@ -63,8 +63,13 @@ GC_HD double ln_pos(double x) {
   return (de * ln2_hi + 2.0 * s * p) + de * ln2_lo;
 }
 
-// e^t for |t| < 700
+// e^t; only ever rounded to float by the callers, so beyond |t| = 200 the
+// result is 0 or +inf there, as libm's expf/powf give (WAVE_BREAKING meets
+// t = -inf where the cell is dry: COEF = EXP(-(+inf)) = 0).  NaN passes.
 GC_HD double exp_core(double t) {
+  if (t != t) return t;
+  if (t < -200.0) return 0.0;
+  if (t > 200.0) return bits_to_double(0x7ff0000000000000ULL);
   const double inv_ln2 = 1.44269504088896338700e+00;
   const double ln2_hi = 6.93147180369123816490e-01;
   const double ln2_lo = 1.90821492927058770002e-10;
@@ -92,8 +97,13 @@ GC_HD double exp_core(double t) {
   return p * bits_to_double(scale);
 }
 
-// x^y for x > 0 (finite, normal) and moderate |y*ln x|
+// x^y for the depths the momentum routines raise to 7/3: x > 0 finite
+// (they bound it below by 1e-3 or 1e-5).  NaN, zero, negative and infinite
+// x follow C's powf for y > 0: NaN, 0, NaN, +inf.
 GC_HD float powf_det(float x, float y) {
+  if (x != x) return x;
+  if (!(x > 0.0f)) return x == 0.0f ? 0.0f : (float)bits_to_double(0x7ff8000000000000ULL);
+  if (x > 3.0e38f) return (float)bits_to_double(0x7ff0000000000000ULL);
   double t = (double)y * ln_pos((double)x);
   return (float)exp_core(t);
 }

@@ -605,17 +605,22 @@ __global__ void __launch_bounds__(kThreads)
     }
     if (MODE != kMassOnly) {
       const bool r = L.is_root;
+      // Cells outside a routine's range are never written by the Fortran,
+      // so its level 2 holds 0 there (ALLOC) and the level copy carries that
+      // 0 into level 1; writing 0 keeps the swapped buffers identical to it.
       if (NL) {
         // CONMOME(_S) ranges (moment.f90:377-387, 583-593, 873-883, 1072-1082)
         const bool mx = r ? (i <= nx - 1 && j <= ny - 1) : (i >= 2 && i <= nx - 1 && j >= 2);
         const bool my = r ? (i <= nx - 1 && j <= ny - 1) : (i >= 2 && j >= 2 && j <= ny - 1);
-        if (mx) L.m2[g] = nl_x<SPH, P>(L, s, i, j);
-        if (my) L.n2[g] = nl_y<SPH, P>(L, s, i, j);
+        L.m2[g] = mx ? nl_x<SPH, P>(L, s, i, j) : 0.0f;
+        L.n2[g] = my ? nl_y<SPH, P>(L, s, i, j) : 0.0f;
       } else {
         // MOMT_S / MOMT_C ranges (moment.f90:78-88, 117-119, 174-192, 233-235)
         const int lo = r ? 1 : 2;
-        if (i >= lo && i <= nx - 1 && j >= lo) L.m2[g] = lin_x<SPH, P>(L, s, i, j);
-        if (j >= lo && j <= ny - 1 && i >= lo) L.n2[g] = lin_y<SPH, P>(L, s, i, j);
+        const bool mx = i >= lo && i <= nx - 1 && j >= lo;
+        const bool my = j >= lo && j <= ny - 1 && i >= lo;
+        L.m2[g] = mx ? lin_x<SPH, P>(L, s, i, j) : 0.0f;
+        L.n2[g] = my ? lin_y<SPH, P>(L, s, i, j) : 0.0f;
       }
       if (L.zmax && (r || (i >= 2 && j >= 2))) {
         const float z2 = s.Z2(i, j);

@@ -12,11 +12,18 @@ import ctypes
 import numpy as np
 
 from . import scenarios
-from ._lib import GcomcotError, arr, check, fp, ip, lib
+from ._lib import GcomcotError, arr, check, fp, ip, last_error, lib
 
 
 def _plane(ny, nx):
     return np.empty((ny, nx), dtype=np.float32)
+
+
+def _ok(what):
+    """Raise if the last C ABI call recorded an error (status mode)."""
+    code, msg = last_error()
+    if code:
+        raise GcomcotError(f"{what}: {msg} (code {code})")
 
 
 def para_planes(spec):
@@ -44,8 +51,10 @@ class DeviceRun:
     def __init__(self, scenario, reference_init=False, arrival_threshold=0.01):
         self.scenario = scenario
         L = lib()
+        L.gcomcot_set_error_mode_(ip(0))   # errors raise here instead of exiting
         L.cuda_shutdown_()
         L.gcomcot_init_gpu_()
+        _ok("gcomcot_init_gpu_")
         L.gcomcot_set_arrival_threshold_(fp(arrival_threshold))
         self.specs = {}
         for s in scenario.layers:
@@ -63,6 +72,7 @@ class DeviceRun:
                                       fp(0), fp(0), keep["R1"][1], keep["R2"][1], keep["R3"][1],
                                       keep["R4"][1], keep["R5"][1], keep["R6"][1],
                                       keep["R11"][1], hp, zp, ip(s.nx), ip(s.ny))
+                _ok(f"gcomcot_init_layer_({s.id})")
                 continue
             if s.laycord == 0:
                 coef = scenarios.para_rows(s.laygov, s.dt, s.dx, s.y, s.dely)
@@ -102,6 +112,7 @@ class DeviceRun:
         L.momt_launch_(None, None, None, ctypes.byref(one))
         L.maxamp_launch_(ctypes.byref(one))
         L.cuda_update_layer_(ctypes.byref(one))
+        _ok("drop-in step")
 
     # ------------------------------------------------------------ state
     def state(self, lid=1):
@@ -114,6 +125,7 @@ class DeviceRun:
         L.cuda_getz_(p(out["z2"]), ip(lid))
         L.cuda_getmn1_(p(out["m1"]), p(out["n1"]), ip(lid))
         L.cuda_getmn_(p(out["m2"]), p(out["n2"]), ip(lid))
+        _ok("state")
         return out
 
     def set_level1(self, lid, z=None, m=None, n=None):

@@ -13,6 +13,7 @@ this module.
 import ctypes
 import os
 import subprocess
+import threading
 
 import numpy as np
 
@@ -29,6 +30,30 @@ def build(ref=True):
     subprocess.run(["make", "-s", "-C", HERE, "liboracle"], check=True)
     if ref and os.path.isdir(REFERENCE_DIR):
         subprocess.run(["make", "-s", "-C", HERE, "ref", f"REF={REFERENCE_DIR}"], check=True)
+
+
+def big_stack(fn, *args, stack_gb=4):
+    """Run fn on a thread with a large stack: the reference's momentum
+    routines hold ~11 whole-plane automatic arrays on the stack."""
+    out = {}
+
+    def target():
+        try:
+            out["r"] = fn(*args)
+        except BaseException as e:  # re-raised in the caller
+            out["e"] = e
+
+    old = threading.stack_size()
+    threading.stack_size(stack_gb * 1024 ** 3 - 4096)
+    try:
+        th = threading.Thread(target=target)
+        th.start()
+        th.join()
+    finally:
+        threading.stack_size(old)
+    if "e" in out:
+        raise out["e"]
+    return out.get("r")
 
 
 def _p(a):
@@ -190,10 +215,10 @@ class Reference:
                                  _i(s.fric_switch), _f(s.fric_coef))
 
     def step(self, n=1):
-        self.lib().ref_step(_i(n))
+        big_stack(self.lib().ref_step, _i(n))
 
     def call(self, op, lid=1, t=0):
-        self.lib().ref_call(_i(OP[op] if isinstance(op, str) else op), _i(lid), _i(t))
+        big_stack(self.lib().ref_call, _i(OP[op] if isinstance(op, str) else op), _i(lid), _i(t))
 
     def state(self, lid=1):
         s = self.specs[lid]

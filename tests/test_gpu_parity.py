@@ -1,0 +1,112 @@
+"""GPU parity: the HIP step against the oracle, bit for bit.
+
+The oracle runs in device-math mode (pow/exp of detmath.h, the only place
+the kernels may differ from libm); everything else is the restated
+reference arithmetic, so planes, wet/dry masks, maxima and arrival steps
+must match exactly.  Small grids keep the oracle side to seconds.
+"""
+import numpy as np
+import pytest
+
+from gcomcot import DeviceRun
+from gcomcot import scenarios as S
+from helpers import bits, diff_report, small_scenarios
+from oracle.bind import Oracle
+
+pytestmark = pytest.mark.gpu
+
+CASES = small_scenarios()
+
+
+def _compare(sc, dev, orc, tag):
+    for L in sc.layers:
+        g, o = dev.state(L.id), orc.state(L.id)
+        for k in ("z1", "m1", "n1"):
+            nb, mx = diff_report(g[k], o[k])
+            assert nb == 0, f"{tag} layer {L.id} {k}: {nb} cells differ (max {mx})"
+        gm, om = dev.maxima(L.id), orc.maxima(L.id)
+        nb, mx = diff_report(gm["zmax_wet"], om["zmax_wet"])
+        assert nb == 0, f"{tag} layer {L.id} zmax: {nb} cells differ (max {mx})"
+        assert np.array_equal(gm["arrival"], om["arrival"]), f"{tag} layer {L.id} arrival differs"
+    if len(sc.layers) > 1:
+        gx, gy = dev.flux_cache(1)
+        ox, oy = orc.flux(1)
+        assert np.array_equal(bits(gx), bits(ox)) and np.array_equal(bits(gy), bits(oy)), \
+            f"{tag} interface flux cache differs"
+
+
+@pytest.mark.parametrize("name,sc,steps", CASES, ids=[c[0] for c in CASES])
+@pytest.mark.parametrize("graph", [True, False], ids=["graph", "direct"])
+def test_device_loop_matches_oracle(name, sc, steps, graph):
+    print(f"[gpu] {name} graph={graph} steps={steps}", flush=True)
+    orc = Oracle(sc, 32, pow_mode=1)
+    orc.step(steps)
+    dev = DeviceRun(sc)
+    dev.run(steps, graph=graph)
+    _compare(sc, dev, orc, name)
+    dev.close()
+
+
+@pytest.mark.parametrize("name,sc,steps", CASES[:5], ids=[c[0] for c in CASES[:5]])
+def test_every_tile_shape(name, sc, steps):
+    from gcomcot._lib import lib, ip
+    import ctypes
+    n = ctypes.c_int(0)
+    lib().gcomcot_tile_variants_(ctypes.byref(n), None, None)
+    orc = Oracle(sc, 32, pow_mode=1)
+    orc.step(40)
+    try:
+        for v in range(n.value):
+            print(f"[gpu] {name} tile variant {v}", flush=True)
+            assert lib().gcomcot_set_tile_variant_(ip(v)) == 0
+            dev = DeviceRun(sc)
+            dev.run(40, graph=False)
+            _compare(sc, dev, orc, f"{name} variant {v}")
+            dev.close()
+    finally:
+        lib().gcomcot_set_tile_variant_(ip(0))
+
+
+def test_reference_entry_points_match_oracle():
+    """The Fortran driver's call sequence through the reference symbols,
+    with layers registered by gcomcot_init_layer_ from full PARA planes."""
+    sc = S.custom(laycord=0, laygov=0, nx=72, ny=64, cx=0.45,
+                  child=dict(laycord=0, laygov=0, footprint=(20, 36), rel_size=2, rel_time=2,
+                             corner=(8, 14)))
+    steps = 60
+    print("[gpu] drop-in reference entry points", flush=True)
+    orc = Oracle(sc, 32, pow_mode=0)
+    orc.step(steps)
+    dev = DeviceRun(sc, reference_init=True)
+    for _ in range(steps):
+        dev.dropin_step()
+    for L in sc.layers:
+        g, o = dev.state(L.id), orc.state(L.id)
+        for k in ("z1", "m1", "n1"):
+            nb, mx = diff_report(g[k], o[k])
+            assert nb == 0, f"layer {L.id} {k}: {nb} differ (max {mx})"
+    nb, mx = diff_report(dev.maxima(1)["zmax_raw"], orc.maxima(1)["zmax_raw"])
+    assert nb == 0, f"raw max: {nb} differ"
+    dev.close()
+
+
+def test_config0_reduced_matches_oracle():
+    sc = S.config0(nx=256, ny=192)
+    print("[gpu] config0 256x192 x 400 steps", flush=True)
+    orc = Oracle(sc, 32, pow_mode=1)
+    orc.step(400)
+    dev = DeviceRun(sc)
+    dev.run(400)
+    _compare(sc, dev, orc, "config0-256")
+    dev.close()
+
+
+def test_config1_reduced_matches_oracle():
+    sc = S.config1(nx=256, ny=192, dx=2.0)
+    print("[gpu] config1 256x192 x 300 steps", flush=True)
+    orc = Oracle(sc, 32, pow_mode=1)
+    orc.step(300)
+    dev = DeviceRun(sc)
+    dev.run(300)
+    _compare(sc, dev, orc, "config1-256")
+    dev.close()
