@@ -47,8 +47,8 @@ __device__ __forceinline__ float cf(const DevLayer &L, int k, int j) {
 }
 
 // PQ_DEPTH (initialization.f90:2512-2523)
-template <int P>
-__device__ __forceinline__ float hp_at(const DevLayer &L, const Lds<P> &s, int i, int j) {
+template <class A>
+__device__ __forceinline__ float hp_at(const DevLayer &L, const A &s, int i, int j) {
   const float h = s.H(i, j);
   if (h > kGx) {
     const int ip1 = i + 1 >= L.nx ? L.nx : i + 1;
@@ -56,8 +56,8 @@ __device__ __forceinline__ float hp_at(const DevLayer &L, const Lds<P> &s, int i
   }
   return 0.0f;
 }
-template <int P>
-__device__ __forceinline__ float hq_at(const DevLayer &L, const Lds<P> &s, int i, int j) {
+template <class A>
+__device__ __forceinline__ float hq_at(const DevLayer &L, const A &s, int i, int j) {
   const float h = s.H(i, j);
   if (h > kGx) {
     const int jp1 = j + 1 >= L.ny ? L.ny : j + 1;
@@ -73,8 +73,8 @@ __device__ __forceinline__ size_t gidx(const DevLayer &L, int i, int j) {
 // OPEN (boundaries.f90:22-202) for one ring cell of the outermost layer.
 // The Fortran reads level-2 fluxes, which equal level 1 at that point of
 // the step (CHANGE copied them); the ring is written once per cell.
-template <int P>
-__device__ float open_cell(const DevLayer &L, const Lds<P> &s, int i, int j) {
+template <class A>
+__device__ float open_cell(const DevLayer &L, const A &s, int i, int j) {
   const int nx = L.nx, ny = L.ny;
   if ((i == 1 || i == nx) && (j == 1 || j == ny)) {
     float qx, qy;
@@ -133,8 +133,8 @@ __device__ float open_cell(const DevLayer &L, const Lds<P> &s, int i, int j) {
 // Continuity for one cell: MASS_S (mass.f90:61-100) / MASS_C (:103-157),
 // CONMASS_S (:227-277) / CONMASS (:162-224) when NL.  dz2 is the total
 // depth DZ(:,:,2) of the nonlinear forms (0 where not computed).
-template <bool NL, bool SPH, int P>
-__device__ __forceinline__ void mass_cell(const DevLayer &L, const Lds<P> &s, int i, int j,
+template <bool NL, bool SPH, class A>
+__device__ __forceinline__ void mass_cell(const DevLayer &L, const A &s, int i, int j,
                                           float &z2, float &dz2) {
   if (L.is_root && (i == 1 || j == 1 || i == L.nx || j == L.ny)) {
     z2 = open_cell(L, s, i, j);
@@ -180,8 +180,8 @@ __device__ __forceinline__ void mass_cell(const DevLayer &L, const Lds<P> &s, in
 
 // ---------------------------------------------------------------- linear
 // MOMT_S x (moment.f90:86-115) / MOMT_C x (moment.f90:190-230)
-template <bool SPH, int P>
-__device__ float lin_x(const DevLayer &L, const Lds<P> &s, int i, int j) {
+template <bool SPH, class A>
+__device__ float lin_x(const DevLayer &L, const A &s, int i, int j) {
   if (!(s.H(i, j) > kGx && s.H(i + 1, j) > kGx)) return 0.0f;
   if (SPH) {
     int jm1 = j - 1;
@@ -219,8 +219,8 @@ __device__ float lin_x(const DevLayer &L, const Lds<P> &s, int i, int j) {
 }
 
 // MOMT_S y (moment.f90:117-146) / MOMT_C y (moment.f90:233-273)
-template <bool SPH, int P>
-__device__ float lin_y(const DevLayer &L, const Lds<P> &s, int i, int j) {
+template <bool SPH, class A>
+__device__ float lin_y(const DevLayer &L, const A &s, int i, int j) {
   if (!(s.H(i, j) > kGx && s.H(i, j + 1) > kGx)) return 0.0f;
   if (SPH) {
     int im1 = i - 1;
@@ -262,26 +262,26 @@ __device__ float lin_y(const DevLayer &L, const Lds<P> &s, int i, int j) {
 // Total depths at the flux points (moment.f90:345-366, 849-870); DZ at
 // level 1 is identically zero in the reference (never refreshed after
 // ALLOC), so DP(I,J,1) = 0.25*(DZ2(I)+DZ2(I+1)).
-template <int P>
-__device__ __forceinline__ float dp1_at(const DevLayer &L, const Lds<P> &s, int i, int j) {
+template <class A>
+__device__ __forceinline__ float dp1_at(const DevLayer &L, const A &s, int i, int j) {
   const int ip1 = i + 1 >= L.nx ? L.nx : i + 1;
   float v = 0.25f * (s.DZ(i, j) + s.DZ(ip1, j));
   return v < kGx ? 0.0f : v;
 }
-template <int P>
-__device__ __forceinline__ float dp2_at(const DevLayer &L, const Lds<P> &s, int i, int j) {
+template <class A>
+__device__ __forceinline__ float dp2_at(const DevLayer &L, const A &s, int i, int j) {
   const int ip1 = i + 1 >= L.nx ? L.nx : i + 1;
   float v = 0.5f * (s.DZ(i, j) + s.DZ(ip1, j));
   return v < kGx ? 0.0f : v;
 }
-template <int P>
-__device__ __forceinline__ float dq1_at(const DevLayer &L, const Lds<P> &s, int i, int j) {
+template <class A>
+__device__ __forceinline__ float dq1_at(const DevLayer &L, const A &s, int i, int j) {
   const int jp1 = j + 1 >= L.ny ? L.ny : j + 1;
   float v = 0.25f * (s.DZ(i, j) + s.DZ(i, jp1));
   return v < kGx ? 0.0f : v;
 }
-template <int P>
-__device__ __forceinline__ float dq2_at(const DevLayer &L, const Lds<P> &s, int i, int j) {
+template <class A>
+__device__ __forceinline__ float dq2_at(const DevLayer &L, const A &s, int i, int j) {
   const int jp1 = j + 1 >= L.ny ? L.ny : j + 1;
   float v = 0.5f * (s.DZ(i, j) + s.DZ(i, jp1));
   return v < kGx ? 0.0f : v;
@@ -289,8 +289,8 @@ __device__ __forceinline__ float dq2_at(const DevLayer &L, const Lds<P> &s, int 
 
 // WAVE_BREAKING (moment.f90:1272-1384): damp the convection terms where
 // the surface rises faster than 0.08*sqrt(g*D).
-template <int P>
-__device__ __forceinline__ void wave_breaking(const DevLayer &L, const Lds<P> &s, int i, int j,
+template <class A>
+__device__ __forceinline__ void wave_breaking(const DevLayer &L, const A &s, int i, int j,
                                               float &advx, float &advy) {
   float c = sqrtf(kGrav * s.DZ(i, j));
   float dzdt_f = 0.08f * c;
@@ -305,8 +305,8 @@ __device__ __forceinline__ void wave_breaking(const DevLayer &L, const Lds<P> &s
 }
 
 // X flux of CONMOME_S (moment.f90:889-1066, SPH) / CONMOME (:393-577)
-template <bool SPH, int P>
-__device__ float nl_x(const DevLayer &L, const Lds<P> &s, int i, int j) {
+template <bool SPH, class A>
+__device__ float nl_x(const DevLayer &L, const A &s, int i, int j) {
   const int IX = L.nx, JY = L.ny;
   int ip1 = i + 1, im1 = i - 1;
   if (im1 <= 1) im1 = 1;
@@ -412,8 +412,8 @@ __device__ float nl_x(const DevLayer &L, const Lds<P> &s, int i, int j) {
 // Y flux of CONMOME_S (moment.f90:1071-1258, SPH) / CONMOME (:582-770).
 // CONMOME's Y equation subtracts its friction term from the X variable
 // (moment.f90:672), so only the 1/(1+FF) factor reaches XQ there.
-template <bool SPH, int P>
-__device__ float nl_y(const DevLayer &L, const Lds<P> &s, int i, int j) {
+template <bool SPH, class A>
+__device__ float nl_y(const DevLayer &L, const A &s, int i, int j) {
   const int IX = L.nx, JY = L.ny;
   int ip1 = i + 1, im1 = i - 1;
   if (im1 <= 1) im1 = 1;
@@ -577,7 +577,7 @@ __global__ void __launch_bounds__(kThreads)
       const int i = ci0 + idx % CX, j = cj0 + idx / CX;
       if (i < 1 || i > nx || j < 1 || j > ny) continue;
       float z2, dz2;
-      mass_cell<NL, SPH, P>(L, s, i, j, z2, dz2);
+      mass_cell<NL, SPH>(L, s, i, j, z2, dz2);
       s.z2[s.o(i, j)] = z2;
       if (NL) s.dz[s.o(i, j)] = dz2;
     }
@@ -612,15 +612,15 @@ __global__ void __launch_bounds__(kThreads)
         // CONMOME(_S) ranges (moment.f90:377-387, 583-593, 873-883, 1072-1082)
         const bool mx = r ? (i <= nx - 1 && j <= ny - 1) : (i >= 2 && i <= nx - 1 && j >= 2);
         const bool my = r ? (i <= nx - 1 && j <= ny - 1) : (i >= 2 && j >= 2 && j <= ny - 1);
-        L.m2[g] = mx ? nl_x<SPH, P>(L, s, i, j) : 0.0f;
-        L.n2[g] = my ? nl_y<SPH, P>(L, s, i, j) : 0.0f;
+        L.m2[g] = mx ? nl_x<SPH>(L, s, i, j) : 0.0f;
+        L.n2[g] = my ? nl_y<SPH>(L, s, i, j) : 0.0f;
       } else {
         // MOMT_S / MOMT_C ranges (moment.f90:78-88, 117-119, 174-192, 233-235)
         const int lo = r ? 1 : 2;
         const bool mx = i >= lo && i <= nx - 1 && j >= lo;
         const bool my = j >= lo && j <= ny - 1 && i >= lo;
-        L.m2[g] = mx ? lin_x<SPH, P>(L, s, i, j) : 0.0f;
-        L.n2[g] = my ? lin_y<SPH, P>(L, s, i, j) : 0.0f;
+        L.m2[g] = mx ? lin_x<SPH>(L, s, i, j) : 0.0f;
+        L.n2[g] = my ? lin_y<SPH>(L, s, i, j) : 0.0f;
       }
       if (L.zmax && (r || (i >= 2 && j >= 2))) {
         const float z2 = s.Z2(i, j);
@@ -632,6 +632,151 @@ __global__ void __launch_bounds__(kThreads)
       }
     }
   }
+}
+
+// ---------------------------------------------------------- strip kernel
+// The fused step as a strip marching along J: a workgroup owns 252
+// interior columns (256 lanes, two halo lanes each side) and a segment of
+// rows.  Each iteration stores one prefetched input row into LDS row rings,
+// computes continuity for the row two ahead, prefetches the next input row
+// and computes the fluxes of the current row.  Halo overhead is ~1.6 % of
+// columns and (SEG+4)/SEG of rows, against (TX+4)(TY+4)/(TX*TY) for a tile.
+constexpr int kStreamW = 256;
+constexpr int kStreamI = kStreamW - 4;
+
+struct Ring {
+  float *m1, *n1, *h, *z2, *dz;  // m1/n1/h: 6 rows, z2/dz: 5 rows of kStreamW
+  int gi0;                       // global column of lane 0
+  float z1own;                   // Z1 of the calling thread's own cell
+  __device__ __forceinline__ int o6(int i, int j) const {
+    return ((j + 12) % 6) * kStreamW + (i - gi0);
+  }
+  __device__ __forceinline__ int o5(int i, int j) const {
+    return ((j + 10) % 5) * kStreamW + (i - gi0);
+  }
+  __device__ __forceinline__ float M(int i, int j) const { return m1[o6(i, j)]; }
+  __device__ __forceinline__ float N(int i, int j) const { return n1[o6(i, j)]; }
+  __device__ __forceinline__ float H(int i, int j) const { return h[o6(i, j)]; }
+  __device__ __forceinline__ float Z2(int i, int j) const { return z2[o5(i, j)]; }
+  __device__ __forceinline__ float DZ(int i, int j) const { return dz[o5(i, j)]; }
+  // Z1 is only ever read at the thread's own cell (continuity, breaking)
+  __device__ __forceinline__ float Z1(int, int) const { return z1own; }
+};
+
+template <bool NL, bool SPH>
+__global__ void __launch_bounds__(kThreads)
+    k_stream(DevLayer L, int nstrip, int nseg, int seg, int per_xcd) {
+  __shared__ float sm[(6 * 3 + 5 * 2) * kStreamW];
+  const int t = tile_of_block(per_xcd);
+  if (t >= nstrip * nseg) return;
+  const int strip = t / nseg, sg = t % nseg;
+  const int nx = L.nx, ny = L.ny;
+  const int c0 = strip * kStreamI + 1;
+  const int j0 = sg * seg + 1;
+  const int j1 = min(j0 + seg - 1, ny);
+  const int lane = threadIdx.x;
+  const int gi = c0 - 2 + lane;
+  const bool col_in = gi >= 1 && gi <= nx;
+  Ring r{sm, sm + 6 * kStreamW, sm + 12 * kStreamW, sm + 18 * kStreamW, sm + 23 * kStreamW,
+         c0 - 2, 0.0f};
+
+  auto load_row = [&](int j, float &h, float &m, float &n, float &z) {
+    const bool in = col_in && j >= 1 && j <= ny;
+    const size_t g = in ? (size_t)(j - 1) * nx + (gi - 1) : 0;
+    h = in ? L.h[g] : 0.0f;
+    m = in ? L.m1[g] : 0.0f;
+    n = in ? L.n1[g] : 0.0f;
+    z = in ? L.z1[g] : 0.0f;
+  };
+  auto store_row = [&](int j, float h, float m, float n) {
+    const int o = r.o6(gi, j);
+    r.h[o] = h;
+    r.m1[o] = m;
+    r.n1[o] = n;
+  };
+  // lane 0 has no left neighbour in LDS; its Z2 is never read
+  auto mass_row = [&](int j, float z1) {
+    float z2 = 0.0f, dz2 = 0.0f;
+    if (col_in && lane >= 1 && j >= 1 && j <= ny) {
+      r.z1own = z1;
+      mass_cell<NL, SPH>(L, r, gi, j, z2, dz2);
+    }
+    const int o = r.o5(gi, j);
+    r.z2[o] = z2;
+    r.dz[o] = dz2;
+  };
+
+  float zp[4];
+  {
+    float h, m, n;
+    for (int k = 0; k < 4; ++k) {
+      load_row(j0 - 2 + k, h, m, n, zp[k]);
+      store_row(j0 - 2 + k, h, m, n);
+    }
+  }
+  __syncthreads();
+  for (int k = 1; k < 4; ++k) mass_row(j0 - 2 + k, zp[k]);
+  float zq0 = zp[2], zq1 = zp[3], zq2 = 0.0f;
+  float ph, pm, pn, pz;
+  load_row(j0 + 2, ph, pm, pn, pz);
+
+  int step = 0;
+  if (L.zmax) {
+    step = L.ctl[L.ctl_in] + 1;
+    if (L.advance && t == 0 && lane == 0) L.ctl[1 - L.ctl_in] = step;
+  }
+  const bool own = lane >= 2 && lane <= kStreamW - 3 && col_in;
+  for (int j = j0; j <= j1; ++j) {
+    store_row(j + 2, ph, pm, pn);
+    zq2 = pz;
+    __syncthreads();
+    mass_row(j + 2, zq2);
+    __syncthreads();
+    load_row(j + 3, ph, pm, pn, pz);
+    if (own) {
+      const int i = gi;
+      const size_t g = (size_t)(j - 1) * nx + (i - 1);
+      const bool rt = L.is_root;
+      float zo;
+      if (rt || (i >= 2 && j >= 2)) zo = r.Z2(i, j);
+      else if (i == 1 && j == 1) zo = r.Z2(2, 2);  // UPDATE ghosts, all_grids.f90:323-324
+      else if (i == 1) zo = r.Z2(2, j);
+      else zo = r.Z2(i, 2);
+      L.z2[g] = zo;
+      r.z1own = zq0;
+      if (NL) {
+        const bool mx = rt ? (i <= nx - 1 && j <= ny - 1) : (i >= 2 && i <= nx - 1 && j >= 2);
+        const bool my = rt ? (i <= nx - 1 && j <= ny - 1) : (i >= 2 && j >= 2 && j <= ny - 1);
+        L.m2[g] = mx ? nl_x<SPH>(L, r, i, j) : 0.0f;
+        L.n2[g] = my ? nl_y<SPH>(L, r, i, j) : 0.0f;
+      } else {
+        const int lo = rt ? 1 : 2;
+        const bool mx = i >= lo && i <= nx - 1 && j >= lo;
+        const bool my = j >= lo && j <= ny - 1 && i >= lo;
+        L.m2[g] = mx ? lin_x<SPH>(L, r, i, j) : 0.0f;
+        L.n2[g] = my ? lin_y<SPH>(L, r, i, j) : 0.0f;
+      }
+      if (L.zmax && (rt || (i >= 2 && j >= 2))) {
+        const float z2 = r.Z2(i, j);
+        if (r.H(i, j) + z2 > kGx) {
+          const float zm = L.zmax[g];
+          if (z2 >= L.arr_thr && !(zm >= L.arr_thr)) L.arrival[g] = step;
+          if (z2 > zm) L.zmax[g] = z2;
+        }
+      }
+    }
+    zq0 = zq1;
+    zq1 = zq2;
+  }
+}
+
+template <bool NL, bool SPH>
+hipError_t launch_stream(const DevLayer &L, int seg, hipStream_t st) {
+  const int ns = (L.nx + kStreamI - 1) / kStreamI, ng = (L.ny + seg - 1) / seg;
+  const int nt = ns * ng, per = (nt + 7) / 8;
+  hipLaunchKernelGGL((k_stream<NL, SPH>), dim3(8 * per), dim3(kThreads), 0, st, L, ns, ng, seg,
+                     per);
+  return hipGetLastError();
 }
 
 // Reference maximum recorder (GPUOutput.cu:46-53 semantics: no wet test)
@@ -823,24 +968,36 @@ hipError_t launch_shape(const DevLayer &L, int mode, bool nl, bool sph, hipStrea
   return hipErrorInvalidValue;
 }
 
-const int kVariants[][2] = {{64, 16}, {64, 8}, {128, 8}, {32, 32}, {64, 32}};
+// Fused-step launch shapes: strip kernel with rows-per-segment SEG
+// (variants 0-4), then LDS tiles TX x TY (variants 5-9).
+const int kVariants[][3] = {{0, 256, 32}, {0, 256, 16}, {0, 256, 24}, {0, 256, 48},
+                            {0, 256, 64}, {1, 64, 16},  {1, 64, 8},   {1, 128, 8},
+                            {1, 32, 32},  {1, 64, 32}};
 
 }  // namespace
 
 int num_tile_variants() { return (int)(sizeof(kVariants) / sizeof(kVariants[0])); }
 void tile_variant_shape(int v, int *tx, int *ty) {
-  *tx = kVariants[v][0];
-  *ty = kVariants[v][1];
+  *tx = kVariants[v][0] ? kVariants[v][1] : -kVariants[v][1];  // negative: strip kernel
+  *ty = kVariants[v][2];
 }
 
 hipError_t launch_step(const DevLayer &L, int mode, bool nl, bool sph, int variant,
                        hipStream_t st) {
-  switch (variant) {
-    case 0: return launch_shape<64, 16>(L, mode, nl, sph, st);
-    case 1: return launch_shape<64, 8>(L, mode, nl, sph, st);
-    case 2: return launch_shape<128, 8>(L, mode, nl, sph, st);
-    case 3: return launch_shape<32, 32>(L, mode, nl, sph, st);
-    case 4: return launch_shape<64, 32>(L, mode, nl, sph, st);
+  if (variant < 0 || variant >= num_tile_variants()) return hipErrorInvalidValue;
+  if (mode == kFused && kVariants[variant][0] == 0) {
+    const int seg = kVariants[variant][2];
+    if (nl) return sph ? launch_stream<true, true>(L, seg, st) : launch_stream<true, false>(L, seg, st);
+    return sph ? launch_stream<false, true>(L, seg, st) : launch_stream<false, false>(L, seg, st);
+  }
+  // split continuity / momentum launches (parents of nested layers) use a 64x8 tile
+  const int v = mode == kFused ? variant : 6;
+  switch (kVariants[v][1] * 1000 + kVariants[v][2]) {
+    case 64016: return launch_shape<64, 16>(L, mode, nl, sph, st);
+    case 64008: return launch_shape<64, 8>(L, mode, nl, sph, st);
+    case 128008: return launch_shape<128, 8>(L, mode, nl, sph, st);
+    case 32032: return launch_shape<32, 32>(L, mode, nl, sph, st);
+    case 64032: return launch_shape<64, 32>(L, mode, nl, sph, st);
     default: return hipErrorInvalidValue;
   }
 }
