@@ -6,24 +6,31 @@
 // breaking damping as EXP(-REF) (moment.f90:1323).  The Fortran calls libm
 // powf/expf on the host; a GPU math library gives different last bits.  To
 // make the device result reproducible on any host, both functions here are
-// computed in IEEE double using only +, -, *, / (each correctly rounded on
-// CPU and GPU alike, with FP contraction disabled) and bit manipulation, then
-// rounded once to float.  The double result is within a few 1e-16 of the
-// exact value, so the float result is the correctly rounded one except when
-// the exact value lies within ~1e-15 (relative) of a float rounding midpoint.
+// computed in IEEE double using only +, -, *, explicit fma (each correctly
+// rounded on CPU and GPU alike; FP contraction is disabled around them) and
+// bit manipulation, then rounded once to float.  The double result is
+// within ~1e-16 (relative) of the exact value, so the float result is the
+// correctly rounded one except when the exact value lies that close to a
+// float rounding midpoint.
 //
-// Method (textbook):
-//   ln x = e*ln2 + 2*atanh(s),  s = (m-1)/(m+1),  m in [sqrt(1/2), sqrt(2))
-//   exp t = 2^k * exp(r),  k = round(t/ln2),  r = t - k*ln2 (Cody-Waite split)
-// with Taylor series for atanh (12 odd terms) and exp (degree 14).
+// Method (table-driven, as in the usual libm designs):
+//   log2 x = e + log2(c_k) + log2(1 + r),  x = 2^e m, m in [1,2),
+//            c_k the centre of m's 1/128 interval, r = m/c_k - 1 computed
+//            exactly as fma(m, 1/c_k, -1) with 1/c_k rounded to 28 bits;
+//   2^t    = 2^(k/32) * 2^g,  k = round(32 t), |g| <= 1/64;
+// with degree-6 polynomials for log2(1+r) and 2^g.  The tables are built by
+// constexpr code from series (atanh for ln, Taylor for exp) at compile time,
+// so host and device hold identical values.
 #pragma once
 
 #include <stdint.h>
 
 #if defined(__HIPCC__) || defined(__HIP__)
 #define GC_HD __host__ __device__ __forceinline__
+#define GC_TABLE __device__ __constant__
 #else
 #define GC_HD static inline
+#define GC_TABLE static
 #endif
 
 namespace gcdet {
@@ -35,82 +42,130 @@ GC_HD uint64_t double_to_bits(double d) {
   union { uint64_t u; double d; } v; v.d = d; return v.u;
 }
 
-// natural log of a positive, finite, normal double
-GC_HD double ln_pos(double x) {
-  uint64_t b = double_to_bits(x);
-  int e = (int)((b >> 52) & 0x7ff) - 1023;
-  double m = bits_to_double((b & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL);
-  if (m > 1.4142135623730951) { m = m * 0.5; e = e + 1; }
-  double s = (m - 1.0) / (m + 1.0);
-  double s2 = s * s;
-  // atanh(s)/s = sum_{k>=0} s^(2k) / (2k+1)
-  double p = 1.0 / 25.0;
-  p = p * s2 + 1.0 / 23.0;
-  p = p * s2 + 1.0 / 21.0;
-  p = p * s2 + 1.0 / 19.0;
-  p = p * s2 + 1.0 / 17.0;
-  p = p * s2 + 1.0 / 15.0;
-  p = p * s2 + 1.0 / 13.0;
-  p = p * s2 + 1.0 / 11.0;
-  p = p * s2 + 1.0 / 9.0;
-  p = p * s2 + 1.0 / 7.0;
-  p = p * s2 + 1.0 / 5.0;
-  p = p * s2 + 1.0 / 3.0;
-  p = p * s2 + 1.0;
-  const double ln2_hi = 6.93147180369123816490e-01;  // 0x3FE62E42FEE00000
-  const double ln2_lo = 1.90821492927058770002e-10;  // 0x3DEA39EF35793C76
-  double de = (double)e;
-  return (de * ln2_hi + 2.0 * s * p) + de * ln2_lo;
+constexpr double kLn2 = 0.69314718055994530942;
+
+// ---- compile-time table construction (plain IEEE double arithmetic)
+constexpr double cx_ln(double x) {  // x in [0.5, 2]: 2*atanh((x-1)/(x+1))
+  double s = (x - 1.0) / (x + 1.0), s2 = s * s, p = 0.0;
+  for (int k = 40; k >= 0; --k) p = p * s2 + 1.0 / (2.0 * k + 1.0);
+  return 2.0 * s * p;
+}
+constexpr double cx_exp(double r) {  // |r| <= 1
+  double p = 0.0;
+  for (int k = 30; k >= 1; --k) p = 1.0 + p * r / k;
+  return p;
+}
+constexpr double cx_round28(double v) {  // v in (0.5, 1]: keep 28 significant bits
+  double s = v * 268435456.0;               // 2^28
+  long long q = (long long)(s + 0.5);
+  return (double)q / 268435456.0;
 }
 
-// e^t; only ever rounded to float by the callers, so beyond |t| = 200 the
+struct Tables {
+  double invc[128];
+  double logc[128];  // log2(1/invc), of the stored invc
+  double exp2j[32];  // 2^(j/32)
+};
+
+constexpr Tables make_tables() {
+  Tables t{};
+  for (int k = 0; k < 128; ++k) {
+    double c = 1.0 + (k + 0.5) / 128.0;
+    double ic = cx_round28(1.0 / c);
+    t.invc[k] = ic;
+    t.logc[k] = -cx_ln(ic) / kLn2;
+  }
+  for (int j = 0; j < 32; ++j) t.exp2j[j] = cx_exp(j * (kLn2 / 32.0));
+  return t;
+}
+
+GC_TABLE constexpr Tables kTab = make_tables();
+
+// log2(1+r)/r and 2^g coefficients
+constexpr double kL1 = 1.0 / kLn2, kL2 = -1.0 / (2.0 * kLn2), kL3 = 1.0 / (3.0 * kLn2),
+                 kL4 = -1.0 / (4.0 * kLn2), kL5 = 1.0 / (5.0 * kLn2),
+                 kL6 = -1.0 / (6.0 * kLn2);
+constexpr double kE1 = kLn2, kE2 = kLn2 * kLn2 / 2.0, kE3 = kLn2 * kLn2 * kLn2 / 6.0,
+                 kE4 = kLn2 * kLn2 * kLn2 * kLn2 / 24.0,
+                 kE5 = kLn2 * kLn2 * kLn2 * kLn2 * kLn2 / 120.0,
+                 kE6 = kLn2 * kLn2 * kLn2 * kLn2 * kLn2 * kLn2 / 720.0;
+
+GC_HD double dfma(double a, double b, double c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_fma(a, b, c);
+#else
+  return __builtin_fma(a, b, c);  // correctly rounded (libm fma if no FMA unit)
+#endif
+}
+
+GC_HD float bits_to_float(uint32_t u) {
+  union { uint32_t u; float f; } v; v.u = u; return v.f;
+}
+GC_HD uint32_t float_to_bits(float f) {
+  union { uint32_t u; float f; } v; v.f = f; return v.u;
+}
+
+// log2 of a positive, finite, normal float (32-bit decomposition)
+template <class TAB>
+GC_HD double log2_pos(float x, const TAB &T) {
+  const uint32_t b = float_to_bits(x);
+  const int e = (int)(b >> 23) - 127;
+  const int k = (int)((b >> 16) & 127);
+  const double m = (double)bits_to_float((b & 0x007fffffu) | 0x3f800000u);
+  const double r = dfma(m, T.invc[k], -1.0);
+  double p = dfma(r, kL6, kL5);
+  p = dfma(r, p, kL4);
+  p = dfma(r, p, kL3);
+  p = dfma(r, p, kL2);
+  p = dfma(r, p, kL1);
+  return (double)e + (T.logc[k] + r * p);
+}
+
+// 2^t; only ever rounded to float by the callers, so beyond |t| = 300 the
 // result is 0 or +inf there, as libm's expf/powf give (WAVE_BREAKING meets
 // t = -inf where the cell is dry: COEF = EXP(-(+inf)) = 0).  NaN passes.
-GC_HD double exp_core(double t) {
+template <class TAB>
+GC_HD double exp2_core(double t, const TAB &T) {
   if (t != t) return t;
-  if (t < -200.0) return 0.0;
-  if (t > 200.0) return bits_to_double(0x7ff0000000000000ULL);
-  const double inv_ln2 = 1.44269504088896338700e+00;
-  const double ln2_hi = 6.93147180369123816490e-01;
-  const double ln2_lo = 1.90821492927058770002e-10;
-  double kd = t * inv_ln2;
-  // round half away from zero, done with truncation so that both sides agree
-  long long k = (long long)(kd + (kd >= 0.0 ? 0.5 : -0.5));
-  kd = (double)k;
-  double r = (t - kd * ln2_hi) - kd * ln2_lo;
-  double p = 1.0 / 87178291200.0;          // 1/14!
-  p = p * r + 1.0 / 6227020800.0;          // 1/13!
-  p = p * r + 1.0 / 479001600.0;           // 1/12!
-  p = p * r + 1.0 / 39916800.0;            // 1/11!
-  p = p * r + 1.0 / 3628800.0;             // 1/10!
-  p = p * r + 1.0 / 362880.0;              // 1/9!
-  p = p * r + 1.0 / 40320.0;               // 1/8!
-  p = p * r + 1.0 / 5040.0;                // 1/7!
-  p = p * r + 1.0 / 720.0;                 // 1/6!
-  p = p * r + 1.0 / 120.0;                 // 1/5!
-  p = p * r + 1.0 / 24.0;                  // 1/4!
-  p = p * r + 1.0 / 6.0;                   // 1/3!
-  p = p * r + 0.5;                         // 1/2!
-  p = p * r + 1.0;
-  p = p * r + 1.0;
-  uint64_t scale = (uint64_t)(k + 1023) << 52;
-  return p * bits_to_double(scale);
+  if (t < -300.0) return 0.0;
+  if (t > 300.0) return bits_to_double(0x7ff0000000000000ULL);
+  const double s = t * 32.0;
+  const int k = (int)(s + (s >= 0.0 ? 0.5 : -0.5));
+  const double g = t - (double)k * (1.0 / 32.0);
+  const int j = k & 31;
+  const int e2 = k >> 5;  // floor(k / 32)
+  double q = dfma(g, kE6, kE5);
+  q = dfma(g, q, kE4);
+  q = dfma(g, q, kE3);
+  q = dfma(g, q, kE2);
+  q = dfma(g, q, kE1);
+  q = dfma(g, q, 1.0);
+  return (T.exp2j[j] * q) * bits_to_double((uint64_t)(uint32_t)(e2 + 1023) << 52);
 }
 
 // x^y for the depths the momentum routines raise to 7/3: x > 0 finite
 // (they bound it below by 1e-3 or 1e-5).  NaN, zero, negative and infinite
-// x follow C's powf for y > 0: NaN, 0, NaN, +inf.
-GC_HD float powf_det(float x, float y) {
+// x follow C's powf for y > 0: NaN, 0, NaN, +inf; subnormal x (whose power
+// underflows for y > 1) gives 0.
+template <class TAB>
+GC_HD float powf_det(float x, float y, const TAB &T) {
   if (x != x) return x;
   if (!(x > 0.0f)) return x == 0.0f ? 0.0f : (float)bits_to_double(0x7ff8000000000000ULL);
   if (x > 3.0e38f) return (float)bits_to_double(0x7ff0000000000000ULL);
-  double t = (double)y * ln_pos((double)x);
-  return (float)exp_core(t);
+  if (x < 1.17549435e-38f) return 0.0f;
+  return (float)exp2_core((double)y * log2_pos(x, T), T);
 }
 
-// e^x for |x| < 80
-GC_HD float expf_det(float x) {
-  return (float)exp_core((double)x);
+// e^x
+template <class TAB>
+GC_HD float expf_det(float x, const TAB &T) {
+  return (float)exp2_core((double)x * (1.0 / kLn2), T);
 }
+
+#if !defined(__HIP_DEVICE_COMPILE__)
+// host entry points on the compile-time tables
+static inline float powf_det(float x, float y) { return powf_det(x, y, kTab); }
+static inline float expf_det(float x) { return expf_det(x, kTab); }
+#endif
 
 }  // namespace gcdet

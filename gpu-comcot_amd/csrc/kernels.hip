@@ -33,6 +33,7 @@ template <int P>
 struct Lds {
   float *z1, *m1, *n1, *h, *z2, *dz;
   int gi0, gj0;  // global (1-based) index of LDS element 0
+  const gcdet::Tables *tab;  // pow/exp tables, LDS copy
   __device__ __forceinline__ int o(int i, int j) const { return (j - gj0) * P + (i - gi0); }
   __device__ __forceinline__ float Z1(int i, int j) const { return z1[o(i, j)]; }
   __device__ __forceinline__ float Z2(int i, int j) const { return z2[o(i, j)]; }
@@ -201,7 +202,7 @@ __device__ float lin_x(const DevLayer &L, const A &s, int i, int j) {
     if (L.fric) {
       float xqq = 0.25f * (s.N(i, j) + s.N(i + 1, j) + s.N(i, jm1) + s.N(i + 1, jm1));
       if (hm >= 1.0e-3f)
-        ff = L.kfric * sqrtf(s.M(i, j) * s.M(i, j) + xqq * xqq) / gcdet::powf_det(hm, kManExp);
+        ff = L.kfric * sqrtf(s.M(i, j) * s.M(i, j) + xqq * xqq) / gcdet::powf_det(hm, kManExp, *s.tab);
       else
         ff = 0.0f;
       xm = xm - ff * s.M(i, j);
@@ -241,7 +242,7 @@ __device__ float lin_y(const DevLayer &L, const A &s, int i, int j) {
     if (L.fric) {
       float xpp = 0.25f * (s.M(i, j) + s.M(i, j + 1) + s.M(im1, j) + s.M(im1, j + 1));
       if (hn >= 1.0e-5f)
-        ff = L.kfric * sqrtf(s.N(i, j) * s.N(i, j) + xpp * xpp) / gcdet::powf_det(hn, kManExp);
+        ff = L.kfric * sqrtf(s.N(i, j) * s.N(i, j) + xpp * xpp) / gcdet::powf_det(hn, kManExp, *s.tab);
       else
         ff = 0.0f;
       xn = xn - ff * s.N(i, j);
@@ -298,7 +299,7 @@ __device__ __forceinline__ void wave_breaking(const DevLayer &L, const A &s, int
   float dzdt = fabsf(s.Z2(i, j) - s.Z1(i, j)) / L.dt;
   if (dzdt > dzdt_f) {
     float ref = (dzdt - dzdt_f) / (dzdt_i - dzdt_f);
-    float coef = gcdet::expf_det(-(1.0f * ref));
+    float coef = gcdet::expf_det(-(1.0f * ref), *s.tab);
     advx = coef * advx;
     advy = coef * advy;
   }
@@ -334,7 +335,7 @@ __device__ float nl_x(const DevLayer &L, const A &s, int i, int j) {
   const float p = s.M(i, j);
   const float xqq = 0.25f * (s.N(i, j) + s.N(ip1, j) + s.N(i, jm1) + s.N(ip1, jm1));
   float FF = 0.0f;
-  if (L.fric) FF = L.kfric * sqrtf(p * p + xqq * xqq) / gcdet::powf_det(DF, kManExp);
+  if (L.fric) FF = L.kfric * sqrtf(p * p + xqq * xqq) / gcdet::powf_det(DF, kManExp, *s.tab);
   float XP;
   if (SPH) {
     const float r2 = cf(L, kF2, j);
@@ -363,43 +364,27 @@ __device__ float nl_x(const DevLayer &L, const A &s, int i, int j) {
     const float fx = SPH ? cf(L, kR21, j) : L.rx;
     const float fy = SPH ? cf(L, kR0, j) : L.ry;
     const float d = dp1_at(L, s, i, j);
-    float advx, advy;
-    if (p < 0.0f) {
-      if (dp1_at(L, s, ip1, j) < kGx || s.DZ(ip1, j) < kGx) {
-        advx = fx * (-(p * p / d));
-      } else {
-        float pp = s.M(ip1, j);
-        advx = fx * (pp * pp / dp1_at(L, s, ip1, j) - p * p / d);
-      }
-    } else {
-      if (dp1_at(L, s, im1, j) < kGx || s.DZ(i, j) < kGx) {
-        advx = fx * (p * p / d);
-      } else {
-        float pm = s.M(im1, j);
-        advx = fx * (p * p / d - pm * pm / dp1_at(L, s, im1, j));
-      }
-    }
-    if (xqq < 0.0f) {
-      if (s.DZ(i, jp1) < kGx || s.DZ(ip1, jp1) < kGx) {
-        advy = fy * (-(p * xqq / d));
-      } else if (dp1_at(L, s, i, jp1) < kGx) {
-        advy = fy * (-(p * xqq / d));
-      } else {
-        float xqe = 0.25f * (s.N(i, jp1) + s.N(ip1, jp1) + s.N(i, j) + s.N(ip1, j));
-        advy = fy * (s.M(i, jp1) * xqe / dp1_at(L, s, i, jp1) - p * xqq / d);
-      }
-    } else {
-      if (s.DZ(i, jm1) < kGx || s.DZ(ip1, jm1) < kGx) {
-        advy = fy * (p * xqq / d);
-      } else if (dp1_at(L, s, i, jm1) < kGx) {
-        advy = fy * (p * xqq / d);
-      } else {
-        float xqe = 0.25f * (s.N(i, jm1) + s.N(ip1, jm1) + s.N(i, j - 2) + s.N(ip1, j - 2));
-        advy = fy * (p * xqq / d - s.M(i, jm1) * xqe / dp1_at(L, s, i, jm1));
-      }
-    }
-    wave_breaking(L, s, i, j, advx, advy);
-    XP = XP - advx - advy;
+    // Upwind terms without divergent branches: each Fortran case is one of
+    // fx*(-A), fx*A, fx*(B-A), fx*(A-B) of the same two quotients.
+    const bool pn = p < 0.0f;
+    const int ia = pn ? ip1 : im1;
+    const float da = dp1_at(L, s, ia, j);
+    const bool degx = pn ? (da < kGx || s.DZ(ip1, j) < kGx) : (da < kGx || s.DZ(i, j) < kGx);
+    const float ax = p * p / d;
+    const float pa = s.M(ia, j);
+    const float bx = pa * pa / da;
+    const float advx = fx * (degx ? (pn ? -ax : ax) : (pn ? bx - ax : ax - bx));
+    const bool qn = xqq < 0.0f;
+    const int jn = qn ? jp1 : jm1, jf = qn ? j : j - 2;
+    const float dn = dp1_at(L, s, i, jn);
+    const bool degy = s.DZ(i, jn) < kGx || s.DZ(ip1, jn) < kGx || dn < kGx;
+    const float ay = p * xqq / d;
+    const float xqe = 0.25f * (s.N(i, jn) + s.N(ip1, jn) + s.N(i, jf) + s.N(ip1, jf));
+    const float by = s.M(i, jn) * xqe / dn;
+    float advy = fy * (degy ? (qn ? -ay : ay) : (qn ? by - ay : ay - by));
+    float advx_ = advx;
+    wave_breaking(L, s, i, j, advx_, advy);
+    XP = XP - advx_ - advy;
   }
   if (L.fric) XP = XP / (1.0f + FF);
   if (fabsf(XP) < kEps) XP = 0.0f;
@@ -441,7 +426,7 @@ __device__ float nl_y(const DevLayer &L, const A &s, int i, int j) {
   const float q = s.N(i, j);
   const float xpp = 0.25f * (s.M(i, j) + s.M(i, jp1) + s.M(im1, j) + s.M(im1, jp1));
   float FF = 0.0f;
-  if (L.fric) FF = L.kfric * sqrtf(q * q + xpp * xpp) / gcdet::powf_det(DF, kManExp);
+  if (L.fric) FF = L.kfric * sqrtf(q * q + xpp * xpp) / gcdet::powf_det(DF, kManExp, *s.tab);
   float XQ;
   if (SPH) {
     const float r4 = cf(L, kF4, j);
@@ -469,41 +454,23 @@ __device__ float nl_y(const DevLayer &L, const A &s, int i, int j) {
     const float fy = SPH ? cf(L, kR0, j) : L.ry;
     const float fx = SPH ? cf(L, kR22, j) : L.rx;
     const float d = dq1_at(L, s, i, j);
-    float advx, advy;
-    if (q < 0.0f) {
-      if (dq1_at(L, s, i, jp1) < kGx || s.DZ(i, jp1) < kGx) {
-        advy = fy * (-(q * q / d));
-      } else {
-        float qq = s.N(i, jp1);
-        advy = fy * (qq * qq / dq1_at(L, s, i, jp1) - q * q / d);
-      }
-    } else {
-      if (dq1_at(L, s, i, jm1) < kGx || s.DZ(i, j) < kGx) {
-        advy = fy * (q * q / d);
-      } else {
-        float qm = s.N(i, jm1);
-        advy = fy * (q * q / d - qm * qm / dq1_at(L, s, i, jm1));
-      }
-    }
-    if (xpp < 0.0f) {
-      if (s.DZ(ip1, j) < kGx || s.DZ(ip1, jp1) < kGx) {
-        advx = fx * (-(q * xpp / d));
-      } else if (dq1_at(L, s, ip1, j) < kGx) {
-        advx = fx * (-(q * xpp / d));
-      } else {
-        float xpe = 0.25f * (s.M(ip1, j) + s.M(ip1, jp1) + s.M(i, j) + s.M(i, jp1));
-        advx = fx * (s.N(ip1, j) * xpe / dq1_at(L, s, ip1, j) - q * xpp / d);
-      }
-    } else {
-      if (s.DZ(im1, j) < kGx || s.DZ(im1, jp1) < kGx) {
-        advx = fx * (q * xpp / d);
-      } else if (dq1_at(L, s, i - 1, j) < kGx) {
-        advx = fx * (q * xpp / d);
-      } else {
-        float xpe = 0.25f * (s.M(im1, j) + s.M(im1, jp1) + s.M(i - 2, j) + s.M(i - 2, jp1));
-        advx = fx * (q * xpp / d - s.N(im1, j) * xpe / dq1_at(L, s, im1, j));
-      }
-    }
+    const bool qn = q < 0.0f;
+    const int ja = qn ? jp1 : jm1;
+    const float da = dq1_at(L, s, i, ja);
+    const bool degy = qn ? (da < kGx || s.DZ(i, jp1) < kGx) : (da < kGx || s.DZ(i, j) < kGx);
+    const float ay = q * q / d;
+    const float qa = s.N(i, ja);
+    const float by = qa * qa / da;
+    float advy = fy * (degy ? (qn ? -ay : ay) : (qn ? by - ay : ay - by));
+    const bool pn = xpp < 0.0f;
+    const int in = pn ? ip1 : im1, ifar = pn ? i : i - 2;
+    const float dn = dq1_at(L, s, pn ? ip1 : i - 1, j);
+    const bool degx = pn ? (s.DZ(ip1, j) < kGx || s.DZ(ip1, jp1) < kGx || dn < kGx)
+                         : (s.DZ(im1, j) < kGx || s.DZ(im1, jp1) < kGx || dn < kGx);
+    const float ax = q * xpp / d;
+    const float xpe = 0.25f * (s.M(in, j) + s.M(in, jp1) + s.M(ifar, j) + s.M(ifar, jp1));
+    const float bx = s.N(in, j) * xpe / dq1_at(L, s, in, j);
+    float advx = fx * (degx ? (pn ? -ax : ax) : (pn ? bx - ax : ax - bx));
     wave_breaking(L, s, i, j, advx, advy);
     XQ = XQ - advx - advy;
   }
@@ -513,6 +480,14 @@ __device__ float nl_y(const DevLayer &L, const A &s, int i, int j) {
   if (XQ > lim) XQ = lim;
   if (XQ < -lim) XQ = -lim;
   return XQ;
+}
+
+// Copy the pow/exp tables into LDS (read before the first __syncthreads).
+__device__ __forceinline__ void stage_tables(gcdet::Tables &dst, int tid) {
+  const double *src = reinterpret_cast<const double *>(&gcdet::kTab);
+  double *d = reinterpret_cast<double *>(&dst);
+  constexpr int n = sizeof(gcdet::Tables) / sizeof(double);
+  for (int k = tid; k < n; k += kThreads) d[k] = src[k];
 }
 
 // Tile -> workgroup order: workgroup b runs on XCD b%8; XCD x gets the
@@ -531,8 +506,10 @@ __global__ void __launch_bounds__(kThreads)
   const int t = tile_of_block(per_xcd);
   if (t >= ntiles) return;
   const int ti0 = (t % tiles_x) * TX + 1, tj0 = (t / tiles_x) * TY + 1;
-  Lds<P> s{sm, sm + A, sm + 2 * A, sm + 3 * A, sm + 4 * A, sm + 5 * A, ti0 - 2, tj0 - 2};
+  __shared__ gcdet::Tables stab;
+  Lds<P> s{sm, sm + A, sm + 2 * A, sm + 3 * A, sm + 4 * A, sm + 5 * A, ti0 - 2, tj0 - 2, &stab};
   const int nx = L.nx, ny = L.ny, tid = threadIdx.x;
+  if (NL || !SPH) stage_tables(stab, tid);
 
   // ---- stage inputs with a two-cell halo
   for (int idx = tid; idx < A; idx += kThreads) {
@@ -590,6 +567,7 @@ __global__ void __launch_bounds__(kThreads)
     step = L.ctl[L.ctl_in] + 1;
     if (L.advance && t == 0 && tid == 0) L.ctl[1 - L.ctl_in] = step;
   }
+#pragma unroll 1
   for (int idx = tid; idx < TX * TY; idx += kThreads) {
     const int i = ti0 + idx % TX, j = tj0 + idx / TX;
     if (i > nx || j > ny) continue;
@@ -648,6 +626,7 @@ struct Ring {
   float *m1, *n1, *h, *z2, *dz;  // m1/n1/h: 6 rows, z2/dz: 5 rows of kStreamW
   int gi0;                       // global column of lane 0
   float z1own;                   // Z1 of the calling thread's own cell
+  const gcdet::Tables *tab;      // pow/exp tables, LDS copy
   __device__ __forceinline__ int o6(int i, int j) const {
     return ((j + 12) % 6) * kStreamW + (i - gi0);
   }
@@ -677,8 +656,10 @@ __global__ void __launch_bounds__(kThreads)
   const int lane = threadIdx.x;
   const int gi = c0 - 2 + lane;
   const bool col_in = gi >= 1 && gi <= nx;
+  __shared__ gcdet::Tables stab;
   Ring r{sm, sm + 6 * kStreamW, sm + 12 * kStreamW, sm + 18 * kStreamW, sm + 23 * kStreamW,
-         c0 - 2, 0.0f};
+         c0 - 2, 0.0f, &stab};
+  if (NL || !SPH) stage_tables(stab, lane);
 
   auto load_row = [&](int j, float &h, float &m, float &n, float &z) {
     const bool in = col_in && j >= 1 && j <= ny;
@@ -968,11 +949,11 @@ hipError_t launch_shape(const DevLayer &L, int mode, bool nl, bool sph, hipStrea
   return hipErrorInvalidValue;
 }
 
-// Fused-step launch shapes: strip kernel with rows-per-segment SEG
-// (variants 0-4), then LDS tiles TX x TY (variants 5-9).
-const int kVariants[][3] = {{0, 256, 32}, {0, 256, 16}, {0, 256, 24}, {0, 256, 48},
-                            {0, 256, 64}, {1, 64, 16},  {1, 64, 8},   {1, 128, 8},
-                            {1, 32, 32},  {1, 64, 32}};
+// Fused-step launch shapes: LDS tiles TX x TY (variants 0-5; 0 is the
+// default), then the strip kernel with SEG rows per segment (6-10).
+const int kVariants[][3] = {{1, 64, 8},   {1, 64, 16},  {1, 32, 32},  {1, 128, 8},
+                            {1, 64, 32},  {1, 32, 16},  {0, 256, 32}, {0, 256, 16},
+                            {0, 256, 24}, {0, 256, 48}, {0, 256, 64}};
 
 }  // namespace
 
@@ -991,13 +972,14 @@ hipError_t launch_step(const DevLayer &L, int mode, bool nl, bool sph, int varia
     return sph ? launch_stream<false, true>(L, seg, st) : launch_stream<false, false>(L, seg, st);
   }
   // split continuity / momentum launches (parents of nested layers) use a 64x8 tile
-  const int v = mode == kFused ? variant : 6;
+  const int v = mode == kFused ? variant : 0;
   switch (kVariants[v][1] * 1000 + kVariants[v][2]) {
     case 64016: return launch_shape<64, 16>(L, mode, nl, sph, st);
     case 64008: return launch_shape<64, 8>(L, mode, nl, sph, st);
     case 128008: return launch_shape<128, 8>(L, mode, nl, sph, st);
     case 32032: return launch_shape<32, 32>(L, mode, nl, sph, st);
     case 64032: return launch_shape<64, 32>(L, mode, nl, sph, st);
+    case 32016: return launch_shape<32, 16>(L, mode, nl, sph, st);
     default: return hipErrorInvalidValue;
   }
 }
