@@ -1,0 +1,647 @@
+// Scenario batch engine: many independent COMCOT nests advanced together.
+//
+// Each scenario is a layer tree (an outermost layer and at most one level
+// of nested layers) with its own step count.  All active scenarios move
+// in lockstep, one root step at a time, and every phase of the step is
+// ONE launch over all of them:
+//
+//   fused step of every single-layer root      } persistent step kernel,
+//   continuity half of every nested root       } one launch per layer kind
+//   for each child slot s, sub-step k = 1..max REL_TIME:
+//     JNQ/NEWQ of every child s with k <= its REL_TIME   (one launch)
+//     fused step of those children                       (one per kind)
+//     JNZ of the children at their half step             (one launch)
+//   momentum half of every nested root (+ max / arrival)
+//
+// which is the per-scenario order of the drop-in path (all_grids.f90:34-
+// 308, comcot.f90:359-439), so every scenario's planes are bit-identical
+// to a run of that scenario alone.  The launches of two root steps are
+// captured in a HIP graph and replayed; the plan is rebuilt when the set
+// of active scenarios changes (a scenario finishes, is added or removed).
+//
+// Step launches use the persistent kernel k_batch: (layer, tile) items in
+// 8 per-XCD queues with atomic heads (cleared by a fill kernel at the
+// start of each root step, inside the graph).  Tile shapes of the fused
+// launches come from the autotuner below: each candidate shape is timed
+// on the live batch with the max/arrival outputs switched off (such a
+// "dry" launch writes only level-2 planes, which the next real step
+// rewrites), and the fastest is kept per (layer kind, size class).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <vector>
+
+#include "comcot_consts.h"
+#include "gcomcot_hip_internal.h"
+#include "kernels.h"
+#include "tree.h"
+
+using namespace gc;
+
+namespace {
+
+struct Scenario {
+  int sid = 0;
+  Tree tree;
+  long long total = 0, done = 0;
+  bool finished() const { return done >= total; }
+};
+
+enum GroupType { kStepGroup = 0, kJnqGroup = 1, kJnzGroup = 2 };
+
+struct Group {
+  int type = kStepGroup;
+  int mode = kFused, nl = 0, sph = 1, shape = 0, t = 0;
+  std::vector<BatchDesc> descs;
+  std::vector<BatchItem> items;
+  int qstart[9] = {0};
+  std::vector<DevNest> nests;
+  std::vector<int> start;  // nnest + 1
+  size_t o_descs = 0, o_items = 0, o_qstart = 0, o_nests = 0, o_start = 0;
+  int head = 0;            // index of this launch's 8 heads
+  long long cells = 0;     // cell updates of a step group
+  std::vector<std::pair<const Tree *, const LayerRec *>> recs;  // layer of each desc
+};
+
+struct Plan {
+  std::vector<Group> step[2];
+  int nsteps = 0;          // 1 or 2 root steps
+  char *arena = nullptr;
+  int *heads = nullptr;
+  int heads_n[2] = {0, 0};
+  int heads_off[2] = {0, 0};
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  long long cells_per_step = 0;
+};
+
+struct Engine {
+  bool init = false;
+  hipStream_t st = nullptr;
+  std::map<int, std::unique_ptr<Scenario>> scen;
+  int next_sid = 1;
+  float arr_thr = 0.01f;
+  bool autotune = true;
+  int forced_shape = -1;
+  std::map<long long, int> tuned;  // (kind, size class) -> shape
+  Plan plan;
+  bool plan_valid = false;
+  long long launches = 0, graph_replays = 0, plans_built = 0;
+};
+
+Engine g_eng;
+
+bool trace_on();
+
+int eng_init() {
+  if (g_eng.init) return 0;
+  HIPCHK(hipSetDevice(0));
+  HIPCHK(hipStreamCreateWithFlags(&g_eng.st, hipStreamNonBlocking));
+  g_eng.init = true;
+  return 0;
+}
+
+void free_plan(Plan &P) {
+  if (P.exec) (void)hipGraphExecDestroy(P.exec);
+  if (P.graph) (void)hipGraphDestroy(P.graph);
+  if (P.arena) (void)hipFree(P.arena);
+  P = Plan();
+}
+
+void invalidate() {
+  if (g_eng.plan.arena || g_eng.plan.exec) {
+    (void)hipStreamSynchronize(g_eng.st);
+    free_plan(g_eng.plan);
+  }
+  g_eng.plan_valid = false;
+}
+
+Scenario *scenario(int sid) {
+  auto it = g_eng.scen.find(sid);
+  return it == g_eng.scen.end() ? nullptr : it->second.get();
+}
+
+std::vector<Scenario *> active() {
+  std::vector<Scenario *> a;
+  for (auto &kv : g_eng.scen) {
+    Scenario *s = kv.second.get();
+    if (!s->finished() && s->tree.layer(1)) a.push_back(s);
+  }
+  return a;
+}
+
+long long kind_key(int nl, int sph, long long cells) {
+  int cls = 0;
+  while ((1LL << (cls + 1)) <= cells) ++cls;
+  return ((long long)nl * 2 + sph) * 64 + cls;
+}
+
+int shape_for(int nl, int sph, long long cells) {
+  if (g_eng.forced_shape >= 0) return g_eng.forced_shape;
+  auto it = g_eng.tuned.find(kind_key(nl, sph, cells));
+  return it == g_eng.tuned.end() ? 0 : it->second;
+}
+
+// Step group for layers of one kind and mode; tiles of tx x ty.
+void add_layer_items(Group &g, const DevLayer &d, int tx, int ty) {
+  BatchDesc bd{};
+  bd.L = d;
+  bd.tiles_x = (d.nx + tx - 1) / tx;
+  bd.ntiles = bd.tiles_x * ((d.ny + ty - 1) / ty);
+  const int slot = (int)g.descs.size();
+  g.descs.push_back(bd);
+  for (int t = 0; t < bd.ntiles; ++t) g.items.push_back(BatchItem{slot, t});
+  g.cells += (long long)d.nx * d.ny;
+}
+
+void split_queues(Group &g) {
+  const long long n = (long long)g.items.size();
+  for (int q = 0; q <= 8; ++q) g.qstart[q] = (int)(n * q / 8);
+}
+
+struct Pending {  // layers of one launch before tiling
+  int mode, nl, sph;
+  std::vector<DevLayer> layers;
+  std::vector<std::pair<const Tree *, const LayerRec *>> recs;
+};
+
+void flush_steps(std::vector<Pending> &pend, std::vector<Group> &out) {
+  for (auto &p : pend) {
+    if (p.layers.empty()) continue;
+    Group g;
+    g.type = kStepGroup;
+    g.mode = p.mode;
+    g.nl = p.nl;
+    g.sph = p.sph;
+    long long cells = 0;
+    for (auto &d : p.layers) cells += (long long)d.nx * d.ny;
+    g.shape = p.mode == kFused ? shape_for(p.nl, p.sph, cells) : 0;
+    int tx = 64, ty = 8;
+    if (p.mode == kFused) batch_shape(g.shape, &tx, &ty);
+    for (auto &d : p.layers) add_layer_items(g, d, tx, ty);
+    g.recs = p.recs;
+    split_queues(g);
+    out.push_back(std::move(g));
+  }
+  pend.clear();
+}
+
+void pend_add(std::vector<Pending> &pend, int mode, const Tree &T, const LayerRec &L,
+              const DevLayer &d) {
+  for (auto &p : pend)
+    if (p.mode == mode && p.nl == L.nl && p.sph == L.sph) {
+      p.layers.push_back(d);
+      p.recs.emplace_back(&T, &L);
+      return;
+    }
+  pend.push_back(Pending{mode, L.nl, L.sph, {d}, {{&T, &L}}});
+}
+
+// Launch list of one root step of every scenario in `act`; rotates the
+// host-side time levels as the launches will.
+void emit_step(const std::vector<Scenario *> &act, std::vector<Group> &out) {
+  std::vector<Pending> pend;
+  size_t maxsib = 0;
+  for (Scenario *s : act) {
+    Tree &T = s->tree;
+    LayerRec &R = *T.layer(1);
+    if (R.children.empty()) pend_add(pend, kFused, T, R, dev_view(T, R, true, true));
+    else pend_add(pend, kMassOnly, T, R, dev_view(T, R, false, false));
+    maxsib = std::max(maxsib, R.children.size());
+  }
+  flush_steps(pend, out);
+  for (size_t sib = 0; sib < maxsib; ++sib) {
+    int maxrt = 0;
+    for (Scenario *s : act) {
+      LayerRec &R = *s->tree.layer(1);
+      if (R.children.size() > sib)
+        maxrt = std::max(maxrt, s->tree.layers[R.children[sib]].rel_time);
+    }
+    for (int k = 1; k <= maxrt; ++k) {
+      Group jq;
+      jq.type = kJnqGroup;
+      jq.t = k;
+      Group jz;
+      jz.type = kJnzGroup;
+      std::vector<LayerRec *> moved;
+      for (Scenario *s : act) {
+        Tree &T = s->tree;
+        LayerRec &R = *T.layer(1);
+        if (R.children.size() <= sib) continue;
+        LayerRec &C = T.layers[R.children[sib]];
+        if (k > C.rel_time) continue;
+        jq.nests.push_back(nest_view(R, C));
+        pend_add(pend, kFused, T, C, dev_view(T, C, true, false));
+        moved.push_back(&C);
+      }
+      out.push_back(std::move(jq));
+      flush_steps(pend, out);
+      for (Scenario *s : act) {
+        Tree &T = s->tree;
+        LayerRec &R = *T.layer(1);
+        if (R.children.size() <= sib) continue;
+        LayerRec &C = T.layers[R.children[sib]];
+        if (k > C.rel_time) continue;
+        const int nhalf = (int)std::floor(C.rel_time / 2.0f) + 1;
+        if (k == nhalf) jz.nests.push_back(nest_view(R, C));
+      }
+      if (!jz.nests.empty()) out.push_back(std::move(jz));
+      for (LayerRec *C : moved) C->cur = 1 - C->cur;  // UPDATE
+    }
+  }
+  for (Scenario *s : act) {
+    Tree &T = s->tree;
+    LayerRec &R = *T.layer(1);
+    if (!R.children.empty()) pend_add(pend, kMomentOnly, T, R, dev_view(T, R, true, true));
+  }
+  flush_steps(pend, out);
+  for (Scenario *s : act) {
+    LayerRec &R = *s->tree.layer(1);
+    R.cur = 1 - R.cur;
+    s->tree.ctl_in = 1 - s->tree.ctl_in;
+  }
+}
+
+size_t al(size_t x) { return (x + 255) & ~(size_t)255; }
+
+int upload_plan(Plan &P) {
+  size_t bytes = 0;
+  int heads = 0;
+  for (int r = 0; r < P.nsteps; ++r) {
+    P.heads_off[r] = heads;
+    for (auto &g : P.step[r]) {
+      if (g.type == kStepGroup) {
+        g.o_descs = bytes; bytes += al(g.descs.size() * sizeof(BatchDesc));
+        g.o_items = bytes; bytes += al(g.items.size() * sizeof(BatchItem));
+        g.o_qstart = bytes; bytes += al(sizeof(g.qstart));
+        g.head = heads; heads += 8;
+      } else {
+        g.start.assign(1, 0);
+        for (auto &n : g.nests)
+          g.start.push_back(g.start.back() +
+                            (g.type == kJnqGroup ? jnq_block_count(n) : jnz_block_count(n)));
+        g.o_nests = bytes; bytes += al(g.nests.size() * sizeof(DevNest));
+        g.o_start = bytes; bytes += al(g.start.size() * sizeof(int));
+      }
+    }
+    P.heads_n[r] = heads - P.heads_off[r];
+  }
+  const size_t o_heads = bytes;
+  bytes += al((size_t)std::max(heads, 1) * sizeof(int));
+  HIPCHK(hipMalloc(&P.arena, bytes));
+  std::vector<char> host(bytes, 0);
+  for (int r = 0; r < P.nsteps; ++r)
+    for (auto &g : P.step[r]) {
+      if (g.type == kStepGroup) {
+        std::memcpy(&host[g.o_descs], g.descs.data(), g.descs.size() * sizeof(BatchDesc));
+        std::memcpy(&host[g.o_items], g.items.data(), g.items.size() * sizeof(BatchItem));
+        std::memcpy(&host[g.o_qstart], g.qstart, sizeof(g.qstart));
+      } else {
+        std::memcpy(&host[g.o_nests], g.nests.data(), g.nests.size() * sizeof(DevNest));
+        std::memcpy(&host[g.o_start], g.start.data(), g.start.size() * sizeof(int));
+      }
+    }
+  HIPCHK(hipMemcpyAsync(P.arena, host.data(), bytes, hipMemcpyHostToDevice, g_eng.st));
+  HIPCHK(hipStreamSynchronize(g_eng.st));
+  P.heads = reinterpret_cast<int *>(P.arena + o_heads);
+  return 0;
+}
+
+BatchLaunch step_launch(const Plan &P, const Group &g) {
+  BatchLaunch B;
+  B.descs = reinterpret_cast<const BatchDesc *>(P.arena + g.o_descs);
+  B.items = reinterpret_cast<const BatchItem *>(P.arena + g.o_items);
+  B.qstart = reinterpret_cast<const int *>(P.arena + g.o_qstart);
+  B.heads = P.heads + g.head;
+  B.nitems = (int)g.items.size();
+  return B;
+}
+
+int enqueue_plan_step(const Plan &P, int r) {
+  hipStream_t st = g_eng.st;
+  hipError_t e = launch_fill_i32(P.heads + P.heads_off[r], P.heads_n[r], 0, st);
+  if (e != hipSuccess) return fail(210, "batch head reset: %s", hipGetErrorString(e));
+  for (const auto &g : P.step[r]) {
+    if (g.type == kStepGroup) {
+      e = launch_batch(step_launch(P, g), g.mode, g.nl != 0, g.sph != 0, g.shape, st);
+    } else {
+      const DevNest *n = reinterpret_cast<const DevNest *>(P.arena + g.o_nests);
+      const int *s = reinterpret_cast<const int *>(P.arena + g.o_start);
+      const int nn = (int)g.nests.size();
+      e = g.type == kJnqGroup ? launch_jnq_multi(n, s, nn, g.start.back(), g.t, st)
+                              : launch_jnz_multi(n, s, nn, g.start.back(), st);
+    }
+    if (e != hipSuccess) return fail(211, "batch launch: %s", hipGetErrorString(e));
+    ++g_eng.launches;
+  }
+  return 0;
+}
+
+// Time each candidate tile shape of every fused group of one root step
+// (dry launches: no max/arrival, no counter advance) and record the best.
+int tune(const std::vector<Scenario *> &act) {
+  std::vector<Group> probe;
+  {
+    // a throw-away emission to see which fused groups exist; time levels
+    // are restored afterwards
+    std::vector<std::vector<int>> cur;
+    std::vector<int> ctl;
+    for (Scenario *s : act) {
+      std::vector<int> c;
+      for (auto &L : s->tree.layers) c.push_back(L.cur);
+      cur.push_back(c);
+      ctl.push_back(s->tree.ctl_in);
+    }
+    emit_step(act, probe);
+    for (size_t i = 0; i < act.size(); ++i) {
+      for (int k = 0; k < kMaxLayers; ++k) act[i]->tree.layers[k].cur = cur[i][k];
+      act[i]->tree.ctl_in = ctl[i];
+    }
+  }
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  for (auto &g : probe) {
+    if (g.type != kStepGroup || g.mode != kFused) continue;
+    const long long key = kind_key(g.nl, g.sph, g.cells);
+    if (g_eng.tuned.count(key)) continue;
+    if (trace_on())
+      std::fprintf(stderr, "batch: tuning kind nl=%d sph=%d, %zu layers, %lld cells\n", g.nl,
+                   g.sph, g.descs.size(), g.cells);
+    int best = 0;
+    float best_ms = 1e30f;
+    for (int sh = 0; sh < batch_num_shapes(); ++sh) {
+      int tx, ty;
+      batch_shape(sh, &tx, &ty);
+      Plan T;
+      T.nsteps = 1;
+      Group d;
+      d.type = kStepGroup;
+      d.mode = kFused;
+      d.nl = g.nl;
+      d.sph = g.sph;
+      d.shape = sh;
+      // views at the layers' current time levels (the probe's later
+      // sub-steps saw rotated ones): read level 1, write level 2 only
+      for (auto &tr : g.recs) add_layer_items(d, dev_view(*tr.first, *tr.second, false, false), tx, ty);
+      split_queues(d);
+      T.step[0].push_back(std::move(d));
+      if (int r = upload_plan(T)) return r;
+      float tot = 0;
+      for (int rep = 0; rep < 4; ++rep) {
+        if (rep == 1) HIPCHK(hipEventRecord(e0, g_eng.st));
+        if (int r = enqueue_plan_step(T, 0)) return r;
+      }
+      HIPCHK(hipEventRecord(e1, g_eng.st));
+      HIPCHK(hipEventSynchronize(e1));
+      HIPCHK(hipEventElapsedTime(&tot, e0, e1));
+      free_plan(T);
+      if (tot < best_ms) { best_ms = tot; best = sh; }
+    }
+    g_eng.tuned[key] = best;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return 0;
+}
+
+int build_plan(const std::vector<Scenario *> &act) {
+  invalidate();
+  if (g_eng.autotune && g_eng.forced_shape < 0)
+    if (int r = tune(act)) return r;
+  Plan &P = g_eng.plan;
+  P.nsteps = 2;
+  emit_step(act, P.step[0]);
+  emit_step(act, P.step[1]);  // time levels are back where they started
+  if (trace_on())
+    std::fprintf(stderr, "batch: plan of %zu + %zu launches\n", P.step[0].size(),
+                 P.step[1].size());
+  for (auto &g : P.step[0])
+    if (g.type == kStepGroup) P.cells_per_step += g.cells;
+  if (int r = upload_plan(P)) return r;
+  if (P.step[0].size() + 1 > (size_t)kGraphMaxLaunches) {
+    // a very deep sub-stepping (huge REL_TIME) is enqueued step by step:
+    // graphs of that many nodes are not worth building
+    g_eng.plan_valid = true;
+    ++g_eng.plans_built;
+    return 0;
+  }
+  for (int r = 0; r < 2; ++r)
+    for (const auto &g : P.step[r])
+      if (g.type == kStepGroup)
+        HIPCHK(launch_batch(step_launch(P, g), g.mode, g.nl != 0, g.sph != 0, g.shape,
+                            g_eng.st, true));
+  HIPCHK(hipStreamBeginCapture(g_eng.st, hipStreamCaptureModeThreadLocal));
+  int r = enqueue_plan_step(P, 0);
+  if (!r) r = enqueue_plan_step(P, 1);
+  hipError_t ce = hipStreamEndCapture(g_eng.st, &P.graph);
+  if (r) return r;
+  HIPCHK(ce);
+  HIPCHK(hipGraphInstantiate(&P.exec, P.graph, nullptr, nullptr, 0));
+  g_eng.plan_valid = true;
+  ++g_eng.plans_built;
+  return 0;
+}
+
+bool trace_on() {
+  static const int on = std::getenv("GCOMCOT_BATCH_TRACE") ? 1 : 0;
+  return on != 0;
+}
+
+// Advance every active scenario by up to n root steps.
+int run(long long n) {
+  while (n > 0) {
+    std::vector<Scenario *> act = active();
+    if (act.empty()) break;
+    long long m = n;
+    for (Scenario *s : act) m = std::min(m, s->total - s->done);
+    if (trace_on())
+      std::fprintf(stderr, "batch: %zu active, epoch of %lld steps, plan %s\n", act.size(), m,
+                   g_eng.plan_valid ? "kept" : "rebuilt");
+    if (!g_eng.plan_valid)
+      if (int r = build_plan(act)) return r;
+    for (long long k = 0; k + 1 < m; k += 2) {
+      if (g_eng.plan.exec) {
+        HIPCHK(hipGraphLaunch(g_eng.plan.exec, g_eng.st));
+        ++g_eng.graph_replays;
+      } else {
+        if (int r = enqueue_plan_step(g_eng.plan, 0)) return r;
+        if (int r = enqueue_plan_step(g_eng.plan, 1)) return r;
+      }
+    }
+    if (m % 2) {
+      // one more step from the plan's starting time levels, then the
+      // levels differ from the captured ones: rebuild next time
+      if (int r = enqueue_plan_step(g_eng.plan, 0)) return r;
+      std::vector<Group> scratch;
+      emit_step(act, scratch);  // rotate host-side levels as that step did
+      HIPCHK(hipStreamSynchronize(g_eng.st));
+      invalidate();
+    }
+    for (Scenario *s : act) s->done += m;
+    for (Scenario *s : act)
+      if (s->finished()) { invalidate(); break; }
+    n -= m;
+  }
+  HIPCHK(hipStreamSynchronize(g_eng.st));
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gcomcot_batch_reset_(void) {
+  if (int e = eng_init()) return e;
+  invalidate();
+  (void)hipStreamSynchronize(g_eng.st);
+  for (auto &kv : g_eng.scen) tree_free(kv.second->tree);
+  g_eng.scen.clear();
+  g_eng.launches = g_eng.graph_replays = g_eng.plans_built = 0;
+  return 0;
+}
+
+int gcomcot_batch_new_scenario_(long long *steps, int *sid) {
+  if (int e = eng_init()) return e;
+  if (*steps < 0) return fail(30, "batch: negative step count");
+  auto s = std::make_unique<Scenario>();
+  s->sid = g_eng.next_sid++;
+  s->total = *steps;
+  s->tree.arr_thr = g_eng.arr_thr;
+  *sid = s->sid;
+  g_eng.scen[s->sid] = std::move(s);
+  return 0;
+}
+
+int gcomcot_batch_add_layer_(int *sid, int *layerid, int *parent, int *laycord, int *laygov,
+                             int *rel_size, int *rel_time, int *corners, int *fric_switch,
+                             float *fric_coef, float *dt, float *dx, float *dy, float *coef,
+                             float *H, float *Z, int *nx, int *ny) {
+  if (int e = eng_init()) return e;
+  Scenario *s = scenario(*sid);
+  if (!s) return fail(31, "batch: no scenario %d", *sid);
+  if (*layerid != 1 && *parent != 1)
+    return fail(32, "batch: layer %d: only one level of nesting below the root", *layerid);
+  if (*laycord != 0 && *laycord != 1) return fail(10, "LAYCORD %d not 0/1", *laycord);
+  if (!(*laygov == 0 || *laygov == 1 || *laygov == 9 || (*laycord == 1 && *laygov == 5)))
+    return fail(11, "LAYGOV %d not supported", *laygov);
+  if (*fric_switch != 0 && *fric_switch != 1)
+    return fail(12, "FRIC_SWITCH %d not supported", *fric_switch);
+  if (*dt <= 0.0f || *dx <= 0.0f || *dy <= 0.0f) return fail(13, "DT/DX/DY must be positive");
+  if (*laycord == 0 && !coef) return fail(14, "spherical layer needs the PARA row table");
+  invalidate();
+  const int level = *layerid == 1 ? 1 : 2;
+  if (int e = tree_create_layer(s->tree, g_eng.st, *layerid, *layerid == 1 ? -1 : *parent,
+                                level, *laycord, *laygov, *rel_size, *rel_time, corners,
+                                *fric_switch, *fric_coef, *dt, *dx, *dy, *nx, *ny, H, Z))
+    return e;
+  LayerRec &L = s->tree.layers[*layerid];
+  std::vector<float> c((size_t)kNumCoef * *ny, 0.0f);
+  if (*laycord == 0) std::memcpy(c.data(), coef, c.size() * sizeof(float));
+  return tree_upload_coef(L, c);
+}
+
+int gcomcot_batch_run_(long long *nsteps, int *nfinished) {
+  if (int e = eng_init()) return e;
+  if (int r = run(*nsteps)) return r;
+  int f = 0;
+  for (auto &kv : g_eng.scen) f += kv.second->finished() ? 1 : 0;
+  if (nfinished) *nfinished = f;
+  return 0;
+}
+
+int gcomcot_batch_finished_(int *sids, int *cap) {
+  int n = 0;
+  for (auto &kv : g_eng.scen)
+    if (kv.second->finished() && kv.second->tree.layer(1)) {
+      if (sids && n < *cap) sids[n] = kv.first;
+      ++n;
+    }
+  return n;
+}
+
+int gcomcot_batch_progress_(int *sid, long long *done, long long *total) {
+  Scenario *s = scenario(*sid);
+  if (!s) return fail(31, "batch: no scenario %d", *sid);
+  *done = s->done;
+  *total = s->total;
+  return 0;
+}
+
+int gcomcot_batch_get_state_(int *sid, int *lid, float *z, float *m, float *n) {
+  Scenario *s = scenario(*sid);
+  if (!s) return fail(31, "batch: no scenario %d", *sid);
+  LayerRec *L = s->tree.layer(*lid);
+  if (!L) return fail(33, "batch: scenario %d has no layer %d", *sid, *lid);
+  HIPCHK(hipStreamSynchronize(g_eng.st));
+  const size_t b = L->nxy() * sizeof(float);
+  if (z) HIPCHK(hipMemcpy(z, L->z[L->cur], b, hipMemcpyDeviceToHost));
+  if (m) HIPCHK(hipMemcpy(m, L->m[L->cur], b, hipMemcpyDeviceToHost));
+  if (n) HIPCHK(hipMemcpy(n, L->n[L->cur], b, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int gcomcot_batch_get_max_(int *sid, int *lid, float *zmax_wet, int *arrival) {
+  Scenario *s = scenario(*sid);
+  if (!s) return fail(31, "batch: no scenario %d", *sid);
+  LayerRec *L = s->tree.layer(*lid);
+  if (!L) return fail(33, "batch: scenario %d has no layer %d", *sid, *lid);
+  HIPCHK(hipStreamSynchronize(g_eng.st));
+  if (zmax_wet)
+    HIPCHK(hipMemcpy(zmax_wet, L->zmax_wet, L->nxy() * sizeof(float), hipMemcpyDeviceToHost));
+  if (arrival)
+    HIPCHK(hipMemcpy(arrival, L->arrival, L->nxy() * sizeof(int), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int gcomcot_batch_remove_(int *sid) {
+  Scenario *s = scenario(*sid);
+  if (!s) return fail(31, "batch: no scenario %d", *sid);
+  invalidate();
+  (void)hipStreamSynchronize(g_eng.st);
+  tree_free(s->tree);
+  g_eng.scen.erase(*sid);
+  return 0;
+}
+
+long long gcomcot_batch_device_bytes_(int *sid) {
+  if (*sid == 0) {
+    long long b = 0;
+    for (auto &kv : g_eng.scen) b += (long long)kv.second->tree.device_bytes();
+    return b;
+  }
+  Scenario *s = scenario(*sid);
+  return s ? (long long)s->tree.device_bytes() : -1;
+}
+
+long long gcomcot_layer_footprint_(int *nx, int *ny, int *nonlinear) {
+  return (long long)layer_bytes(*nx, *ny, *nonlinear != 0, false);
+}
+
+void gcomcot_batch_set_arrival_threshold_(float *thr) { g_eng.arr_thr = *thr; }
+
+int gcomcot_batch_set_tuning_(int *autotune, int *shape) {
+  if (*shape >= batch_num_shapes()) return fail(34, "batch: tile shape %d out of range", *shape);
+  g_eng.autotune = *autotune != 0;
+  g_eng.forced_shape = *shape;
+  invalidate();
+  return 0;
+}
+
+int gcomcot_batch_info_(long long *launches, long long *graph_replays, long long *plans,
+                        long long *cells_per_step, int *tuned_count) {
+  if (launches) *launches = g_eng.launches;
+  if (graph_replays) *graph_replays = g_eng.graph_replays;
+  if (plans) *plans = g_eng.plans_built;
+  if (cells_per_step) *cells_per_step = g_eng.plan_valid ? g_eng.plan.cells_per_step : 0;
+  if (tuned_count) *tuned_count = (int)g_eng.tuned.size();
+  return 0;
+}
+
+}  // extern "C"

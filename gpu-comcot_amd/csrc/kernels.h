@@ -54,8 +54,41 @@ struct DevNest {
   int is, ie, js, je;    // footprint in the parent, 1-based
   const float *ch, *cz1, *cz2;
   float *cm1, *cn1;      // child level-1 fluxes, boundary written
+  float *pz2_out;        // parent level-2 Z, footprint written by JNZ
 };
 hipError_t launch_jnq(const DevNest &N, int t, hipStream_t st);
-hipError_t launch_jnz(const DevNest &N, float *pz2_out, hipStream_t st);
+hipError_t launch_jnz(const DevNest &N, hipStream_t st);
+
+// ---- batch engine: one persistent launch over the tiles of many layers
+struct BatchDesc {
+  DevLayer L;
+  int tiles_x, ntiles;
+};
+struct alignas(8) BatchItem {
+  int slot, tile;  // descriptor index, tile index within that layer
+};
+struct BatchLaunch {
+  const BatchDesc *descs;
+  const BatchItem *items;
+  const int *qstart;  // 9 entries: items of XCD queue q are [qstart[q], qstart[q+1])
+  int *heads;         // 8 queue heads, zero before the launch
+  int nitems;
+};
+// Fused launches take tile shape `shape` (batch_shape); continuity and
+// momentum halves of nested parents use 64 x 8.  prepare: only look up
+// the kernel's resident-workgroup count (do it before a graph capture).
+int batch_num_shapes();
+void batch_shape(int k, int *tx, int *ty);
+hipError_t launch_batch(const BatchLaunch &B, int mode, bool nl, bool sph, int shape,
+                        hipStream_t st, bool prepare = false);
+// JNQ/NEWQ and JNZ of several nests in one launch: nest n owns blocks
+// [start[n], start[n+1]) of 128 threads; *_block_count gives each one's share.
+int jnq_block_count(const DevNest &N);
+int jnz_block_count(const DevNest &N);
+hipError_t launch_jnq_multi(const DevNest *nests, const int *start, int nnest, int nblocks,
+                            int t, hipStream_t st);
+hipError_t launch_jnz_multi(const DevNest *nests, const int *start, int nnest, int nblocks,
+                            hipStream_t st);
+hipError_t launch_fill_i32(int *p, int n, int v, hipStream_t st);
 
 }  // namespace gc

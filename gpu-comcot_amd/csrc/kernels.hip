@@ -498,18 +498,16 @@ __device__ __forceinline__ int tile_of_block(int per_xcd) {
   return (b & 7) * per_xcd + (b >> 3);
 }
 
+// One TX x TY tile t of layer L: stage, continuity, momentum, outputs.
+// sm holds 6 (TX+4)(TY+4) planes; stab the pow/exp tables, staged by the
+// caller before the call (the first barrier below publishes them).
 template <int MODE, bool NL, bool SPH, int TX, int TY>
-__global__ void __launch_bounds__(kThreads)
-    k_step(DevLayer L, int tiles_x, int ntiles, int per_xcd) {
+__device__ __forceinline__ void tile_step(const DevLayer &L, int t, int tiles_x, float *sm,
+                                          const gcdet::Tables *stab) {
   constexpr int P = TX + 4, R = TY + 4, A = P * R;
-  __shared__ float sm[6 * A];
-  const int t = tile_of_block(per_xcd);
-  if (t >= ntiles) return;
   const int ti0 = (t % tiles_x) * TX + 1, tj0 = (t / tiles_x) * TY + 1;
-  __shared__ gcdet::Tables stab;
-  Lds<P> s{sm, sm + A, sm + 2 * A, sm + 3 * A, sm + 4 * A, sm + 5 * A, ti0 - 2, tj0 - 2, &stab};
+  Lds<P> s{sm, sm + A, sm + 2 * A, sm + 3 * A, sm + 4 * A, sm + 5 * A, ti0 - 2, tj0 - 2, stab};
   const int nx = L.nx, ny = L.ny, tid = threadIdx.x;
-  if (NL || !SPH) stage_tables(stab, tid);
 
   // ---- stage inputs with a two-cell halo
   for (int idx = tid; idx < A; idx += kThreads) {
@@ -610,6 +608,17 @@ __global__ void __launch_bounds__(kThreads)
       }
     }
   }
+}
+
+template <int MODE, bool NL, bool SPH, int TX, int TY>
+__global__ void __launch_bounds__(kThreads)
+    k_step(DevLayer L, int tiles_x, int ntiles, int per_xcd) {
+  __shared__ float sm[6 * (TX + 4) * (TY + 4)];
+  __shared__ gcdet::Tables stab;
+  const int t = tile_of_block(per_xcd);
+  if (t >= ntiles) return;
+  if (NL || !SPH) stage_tables(stab, threadIdx.x);
+  tile_step<MODE, NL, SPH, TX, TY>(L, t, tiles_x, sm, &stab);
 }
 
 // ---------------------------------------------------------- strip kernel
@@ -863,9 +872,12 @@ __device__ float iface_flux(const DevNest &N, int side, int k, int t) {
 // k (t == 2, k in the line's full range JS-2..JE+2 / IS-2..IE+2) and, for
 // k in the footprint, the R child boundary fluxes of EDGEINTERP_VERT/HORI
 // (all_grids.f90:502-604), from the interface flux at k-1, k, k+1.
-__global__ void k_jnq(DevNest N, int t) {
+__device__ __forceinline__ int jnq_threads(const DevNest &N) {
+  return 2 * (N.je - N.js + 5) + 2 * (N.ie - N.is + 5);
+}
+
+__device__ void jnq_thread(const DevNest &N, int g, int t) {
   const int lv = N.je - N.js + 5, lh = N.ie - N.is + 5;  // k ranges incl. +-2
-  int g = blockIdx.x * blockDim.x + threadIdx.x;
   int side;
   if (g < 2 * lv) { side = g / lv; g -= side * lv; }
   else { g -= 2 * lv; if (g >= 2 * lh) return; side = 2 + g / lh; g -= (side - 2) * lh; }
@@ -900,11 +912,14 @@ __global__ void k_jnq(DevNest N, int t) {
   }
 }
 
+__global__ void k_jnq(DevNest N, int t) {
+  jnq_thread(N, blockIdx.x * blockDim.x + threadIdx.x, t);
+}
+
 // JNZ (all_grids.f90:342-421): parent Z2 over the footprint = mean of the
 // wet child cells, summed in the Fortran order (KI outer, KJ inner).
-__global__ void k_jnz(DevNest N, float *pz2_out) {
+__device__ void jnz_thread(const DevNest &N, int g) {
   const int w = N.ie - N.is + 1, hgt = N.je - N.js + 1;
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= w * hgt) return;
   const int i = N.is + g % w, j = N.js + g / w;
   const int ir = N.rel_size;
@@ -923,7 +938,75 @@ __global__ void k_jnz(DevNest N, float *pz2_out) {
       }
     }
   }
-  pz2_out[(size_t)(j - 1) * N.pnx + (i - 1)] = count > half ? sum / count : 0.0f;
+  N.pz2_out[(size_t)(j - 1) * N.pnx + (i - 1)] = count > half ? sum / count : 0.0f;
+}
+
+__global__ void k_jnz(DevNest N) { jnz_thread(N, blockIdx.x * blockDim.x + threadIdx.x); }
+
+// ------------------------------------------------------------ batch forms
+// Several nests in one launch: nest n owns blocks [start[n], start[n+1]).
+__device__ __forceinline__ int nest_of_block(const int *start, int nnest) {
+  int lo = 0, hi = nnest - 1;
+  const int b = blockIdx.x;
+  while (lo < hi) {  // last n with start[n] <= b
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ void k_jnq_multi(const DevNest *__restrict__ nests, const int *__restrict__ start,
+                            int nnest, int t) {
+  const int n = __builtin_amdgcn_readfirstlane(nest_of_block(start, nnest));
+  const DevNest N = nests[n];
+  const int g = (blockIdx.x - start[n]) * blockDim.x + threadIdx.x;
+  if (g < jnq_threads(N)) jnq_thread(N, g, t);
+}
+
+__global__ void k_jnz_multi(const DevNest *__restrict__ nests, const int *__restrict__ start,
+                            int nnest) {
+  const int n = __builtin_amdgcn_readfirstlane(nest_of_block(start, nnest));
+  const DevNest N = nests[n];
+  jnz_thread(N, (blockIdx.x - start[n]) * blockDim.x + threadIdx.x);
+}
+
+__global__ void k_fill_i32(int *p, int n, int v) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < n) p[g] = v;
+}
+
+// Persistent step kernel of the batch engine: a grid sized to the chip
+// pulls (layer, tile) work items.  Items are split into 8 contiguous
+// queues, one per XCD (workgroup b runs on XCD b % 8), each drained
+// through an atomic head; a workgroup whose own queue is empty takes items
+// from the others, so uneven scenario sizes still balance.  Every
+// workgroup leaves once all 8 heads have passed their ends.
+template <int MODE, bool NL, bool SPH, int TX, int TY>
+__global__ void __launch_bounds__(kThreads)
+    k_batch(const BatchDesc *__restrict__ descs, const BatchItem *__restrict__ items,
+            const int *__restrict__ qstart, int *heads) {
+  __shared__ float sm[6 * (TX + 4) * (TY + 4)];
+  __shared__ gcdet::Tables stab;
+  __shared__ int s_item;
+  if (NL || !SPH) stage_tables(stab, threadIdx.x);
+  const int x = blockIdx.x & 7;
+  for (int c = 0; c < 8; ++c) {
+    const int q = (x + c) & 7;
+    const int lo = qstart[q], hi = qstart[q + 1];
+    for (;;) {
+      __syncthreads();  // previous tile done with LDS and s_item
+      if (threadIdx.x == 0) s_item = lo + atomicAdd(&heads[q], 1);
+      __syncthreads();
+      const int it = __builtin_amdgcn_readfirstlane(s_item);
+      if (it >= hi) break;
+      const BatchItem w = items[it];
+      const int slot = __builtin_amdgcn_readfirstlane(w.slot);
+      const int tile = __builtin_amdgcn_readfirstlane(w.tile);
+      const DevLayer L = descs[slot].L;
+      tile_step<MODE, NL, SPH, TX, TY>(L, tile, descs[slot].tiles_x, sm, &stab);
+    }
+  }
 }
 
 template <int MODE, bool NL, bool SPH, int TX, int TY>
@@ -996,9 +1079,113 @@ hipError_t launch_jnq(const DevNest &N, int t, hipStream_t st) {
   return hipGetLastError();
 }
 
-hipError_t launch_jnz(const DevNest &N, float *pz2_out, hipStream_t st) {
+hipError_t launch_jnz(const DevNest &N, hipStream_t st) {
   const int total = (N.ie - N.is + 1) * (N.je - N.js + 1);
-  hipLaunchKernelGGL(k_jnz, dim3((total + 127) / 128), dim3(128), 0, st, N, pz2_out);
+  hipLaunchKernelGGL(k_jnz, dim3((total + 127) / 128), dim3(128), 0, st, N);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------ batch launchers
+namespace {
+
+int device_cus() {
+  static int cus = 0;
+  if (!cus) {
+    int dev = 0;
+    hipDeviceProp_t p;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&p, dev) == hipSuccess)
+      cus = p.multiProcessorCount;
+    if (cus <= 0) cus = 256;
+  }
+  return cus;
+}
+
+template <int MODE, bool NL, bool SPH, int TX, int TY>
+hipError_t launch_batch_t(const BatchLaunch &B, hipStream_t st, bool prepare) {
+  static int resident = 0;  // workgroups that fit on the chip at once
+  if (!resident) {
+    int per_cu = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, reinterpret_cast<const void *>(&k_batch<MODE, NL, SPH, TX, TY>), kThreads, 0);
+    if (e != hipSuccess) return e;
+    resident = (per_cu > 0 ? per_cu : 1) * device_cus();
+  }
+  if (prepare) return hipSuccess;
+  int grid = resident;
+  const int want = (B.nitems + 7) / 8 * 8;
+  if (want < grid) grid = want;
+  if (grid < 8) grid = 8;
+  hipLaunchKernelGGL((k_batch<MODE, NL, SPH, TX, TY>), dim3(grid), dim3(kThreads), 0, st,
+                     B.descs, B.items, B.qstart, B.heads);
+  return hipGetLastError();
+}
+
+template <int TX, int TY>
+hipError_t batch_shape_t(const BatchLaunch &B, int mode, bool nl, bool sph, hipStream_t st,
+                         bool prepare) {
+#define GC_CASE(MD, A, C) \
+  if (mode == MD && nl == A && sph == C) return launch_batch_t<MD, A, C, TX, TY>(B, st, prepare);
+  GC_CASE(kFused, false, true) GC_CASE(kFused, false, false)
+  GC_CASE(kFused, true, true) GC_CASE(kFused, true, false)
+#undef GC_CASE
+  return hipErrorInvalidValue;
+}
+
+const int kBatchShapes[][2] = {{64, 8}, {32, 16}, {64, 16}};
+
+}  // namespace
+
+int batch_num_shapes() { return (int)(sizeof(kBatchShapes) / sizeof(kBatchShapes[0])); }
+void batch_shape(int k, int *tx, int *ty) {
+  *tx = kBatchShapes[k][0];
+  *ty = kBatchShapes[k][1];
+}
+
+hipError_t launch_batch(const BatchLaunch &B, int mode, bool nl, bool sph, int shape,
+                        hipStream_t st, bool prepare) {
+  if (B.nitems <= 0 && !prepare) return hipSuccess;
+  if (mode != kFused) {  // continuity / momentum halves of nested parents: 64 x 8
+#define GC_CASE(MD, A, C) \
+    if (mode == MD && nl == A && sph == C) return launch_batch_t<MD, A, C, 64, 8>(B, st, prepare);
+    GC_CASE(kMassOnly, false, true) GC_CASE(kMassOnly, false, false)
+    GC_CASE(kMassOnly, true, true) GC_CASE(kMassOnly, true, false)
+    GC_CASE(kMomentOnly, false, true) GC_CASE(kMomentOnly, false, false)
+    GC_CASE(kMomentOnly, true, true) GC_CASE(kMomentOnly, true, false)
+#undef GC_CASE
+    return hipErrorInvalidValue;
+  }
+  switch (shape) {
+    case 0: return batch_shape_t<64, 8>(B, mode, nl, sph, st, prepare);
+    case 1: return batch_shape_t<32, 16>(B, mode, nl, sph, st, prepare);
+    case 2: return batch_shape_t<64, 16>(B, mode, nl, sph, st, prepare);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+int jnq_block_count(const DevNest &N) {
+  return (2 * (N.je - N.js + 5) + 2 * (N.ie - N.is + 5) + 127) / 128;
+}
+int jnz_block_count(const DevNest &N) {
+  return ((N.ie - N.is + 1) * (N.je - N.js + 1) + 127) / 128;
+}
+
+hipError_t launch_jnq_multi(const DevNest *nests, const int *start, int nnest, int nblocks,
+                            int t, hipStream_t st) {
+  if (nnest <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_jnq_multi, dim3(nblocks), dim3(128), 0, st, nests, start, nnest, t);
+  return hipGetLastError();
+}
+
+hipError_t launch_jnz_multi(const DevNest *nests, const int *start, int nnest, int nblocks,
+                            hipStream_t st) {
+  if (nnest <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_jnz_multi, dim3(nblocks), dim3(128), 0, st, nests, start, nnest);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_i32(int *p, int n, int v, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_i32, dim3((n + 255) / 256), dim3(256), 0, st, p, n, v);
   return hipGetLastError();
 }
 

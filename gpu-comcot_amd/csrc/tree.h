@@ -1,0 +1,100 @@
+// Layer trees on the device: one COMCOT nest (an outermost layer and its
+// nested layers), its device planes and the enqueue order of one step.
+//
+// The drop-in C ABI keeps one tree (the reference's global layer table,
+// GPUcomcot.cu:5-22); the batch engine keeps one tree per scenario.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <vector>
+
+#include "kernels.h"
+
+namespace gc {
+
+constexpr int kMaxLayers = 20;  // layer ids 1..19 (GPUHeader.h:51)
+// Launch sequences longer than this per root step are not captured into
+// HIP graphs (deep sub-stepping, e.g. a REL_TIME in the thousands).
+constexpr int kGraphMaxLaunches = 4096;
+
+struct LayerRec {
+  bool used = false;
+  int id = 0, parent = -1, level = 1;
+  int laycord = 0, laygov = 0, nl = 0, sph = 1;
+  int rel_size = 1, rel_time = 1, corners[4] = {0, 0, 0, 0};
+  int fric = 0, modscm = 0, linchk = 1;
+  float fric_coef = 0, dt = 0, dx = 0, dy = 0, rx = 0, ry = 0, grx = 0, gry = 0, kfric = 0;
+  int nx = 0, ny = 0;
+  void *block = nullptr;  // one allocation holding every plane below
+  size_t block_bytes = 0;
+  float *h = nullptr, *z[2] = {nullptr, nullptr}, *m[2] = {nullptr, nullptr},
+        *n[2] = {nullptr, nullptr};
+  float *zmax_raw = nullptr, *zmax_wet = nullptr, *coef = nullptr, *r24 = nullptr;
+  float *dzp = nullptr, *xflux = nullptr, *yflux = nullptr;
+  int *arrival = nullptr;
+  int cur = 0;  // buffer index holding time level 1
+  std::vector<int> children;
+  size_t nxy() const { return (size_t)nx * ny; }
+};
+
+struct Tree {
+  LayerRec layers[kMaxLayers];
+  int *ctl = nullptr;  // device int[2]: root-step counters for the arrival field
+  int ctl_in = 0;      // which counter holds the completed step count
+  float arr_thr = 0.01f;
+  int variant = 0;     // tile variant of the step kernels
+  LayerRec *layer(int id) {
+    return (id >= 1 && id < kMaxLayers && layers[id].used) ? &layers[id] : nullptr;
+  }
+  size_t device_bytes() const;
+};
+
+// ---- error state shared by the whole library (reference: print + exit)
+extern int g_fatal;
+int fail(int code, const char *fmt, ...);
+const char *last_message();
+int take_last_code();
+
+#define HIPCHK(call)                                                             \
+  do {                                                                           \
+    hipError_t e_ = (call);                                                      \
+    if (e_ != hipSuccess)                                                        \
+      return ::gc::fail(100 + (int)e_, "%s at %s:%d: %s", #call, __FILE__,       \
+                        __LINE__, hipGetErrorString(e_));                        \
+  } while (0)
+
+// Bytes of device memory a layer of nx x ny cells takes (planes + tables).
+size_t layer_bytes(int nx, int ny, bool nl, bool r24);
+
+// Register layer `id` of tree T: validates the nest geometry as the
+// reference's SUBGRID_MATCHING does (initialization.f90:882-1131), derives
+// the ALLOC scalars (comcot.f90:537-552, 602), allocates and clears the
+// device planes, uploads H and Z.  A new root frees the old tree.
+int tree_create_layer(Tree &T, hipStream_t st, int id, int parent, int level, int laycord,
+                      int laygov, int rel_size, int rel_time, const int *corners,
+                      int fric_switch, float fric_coef, float dt, float dx, float dy, int nx,
+                      int ny, const float *H, const float *Z);
+int tree_upload_coef(LayerRec &L, const std::vector<float> &coef);
+int tree_alloc_r24(LayerRec &L, const float *R2, const float *R4);
+void tree_free(Tree &T);
+
+DevLayer dev_view(const Tree &T, const LayerRec &L, bool with_max, bool advance);
+DevNest nest_view(const LayerRec &P, const LayerRec &C);
+
+// One layer pass (kFused / kMassOnly / kMomentOnly) on stream st.
+int tree_launch(const Tree &T, const LayerRec &L, int mode, bool with_max, bool advance,
+                hipStream_t st);
+// Sub-steps of every child of P for one step of P (ALL_GRID order).
+int tree_enqueue_children(Tree &T, LayerRec &P, bool with_max, hipStream_t st);
+// One complete root step: mass+open, nests, momentum + max, rotation.
+int tree_enqueue_root_step(Tree &T, hipStream_t st);
+// Kernel launches one root step of the tree enqueues.
+long long tree_launches_per_step(Tree &T);
+
+// R2/R4 of the reference ABI as a row factor of max(HP,0) (PARA,
+// comcot.f90:733-744); false when no factor reproduces the row exactly.
+bool row_factor(const float *R, const float *H, int nx, int ny, int j, bool is_x, float *F);
+
+}  // namespace gc

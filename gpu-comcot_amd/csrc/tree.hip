@@ -1,0 +1,392 @@
+// Layer trees: registration, device planes and the step's enqueue order.
+//
+// Children are kept in registration order like the reference's sibling
+// list (GPUcomcot.cu:157-190), which is also the order ALL_GRID visits
+// them (all_grids.f90:34-35).  Each layer owns two time levels of Z, M, N
+// on the device; a level rotation swaps them.
+//
+// Device memory per layer, one allocation: H, Z x2, M x2, N x2, raw max,
+// wet max, arrival (11 planes of 4 B = 44 B/cell; +4 for the DZ plane of
+// nonlinear layers), the per-row coefficient table and the interface
+// flux cache.  Planes start on 256-byte boundaries.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+
+#include "comcot_consts.h"
+#include "tree.h"
+
+namespace gc {
+
+// ------------------------------------------------------------ errors
+int g_fatal = 1;
+static int g_last_code = 0;
+static char g_last_msg[512] = {0};
+
+int fail(int code, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_last_msg, sizeof(g_last_msg), fmt, ap);
+  va_end(ap);
+  g_last_code = code;
+  if (g_fatal) {
+    fprintf(stderr, "gcomcot_hip: %s\n", g_last_msg);
+    exit(EXIT_FAILURE);
+  }
+  return code;
+}
+
+const char *last_message() { return g_last_msg; }
+
+int take_last_code() {
+  int c = g_last_code;
+  g_last_code = 0;
+  g_last_msg[0] = 0;
+  return c;
+}
+
+// ------------------------------------------------------------ memory
+namespace {
+
+size_t pad(size_t bytes) { return (bytes + 255) & ~(size_t)255; }
+
+struct Carve {
+  char *base;
+  size_t off = 0;
+  template <class T>
+  T *take(size_t count) {
+    T *p = reinterpret_cast<T *>(base + off);
+    off += pad(count * sizeof(T));
+    return p;
+  }
+};
+
+}  // namespace
+
+size_t layer_bytes(int nx, int ny, bool nl, bool r24) {
+  const size_t n = (size_t)nx * ny;
+  size_t b = 9 * pad(n * 4) + pad(n * 4);  // h, z x2, m x2, n x2, zmax x2; arrival
+  if (nl) b += pad(n * 4);
+  b += pad(2 * (size_t)nx * 4) + pad(2 * (size_t)ny * 4) + pad((size_t)kNumCoef * ny * 4);
+  if (r24) b += pad(2 * n * 4);
+  return b;
+}
+
+size_t Tree::device_bytes() const {
+  size_t b = 0;
+  for (const auto &L : layers)
+    if (L.used) b += L.block_bytes + (L.r24 ? 2 * L.nxy() * 4 : 0);
+  return b;
+}
+
+static void free_layer(LayerRec &L) {
+  if (L.block) (void)hipFree(L.block);
+  if (L.r24) (void)hipFree(L.r24);
+  L = LayerRec();
+}
+
+void tree_free(Tree &T) {
+  for (auto &L : T.layers)
+    if (L.used) free_layer(L);
+  if (T.ctl) (void)hipFree(T.ctl);
+  T.ctl = nullptr;
+  T.ctl_in = 0;
+}
+
+int tree_create_layer(Tree &T, hipStream_t st, int id, int parent, int level, int laycord,
+                      int laygov, int rel_size, int rel_time, const int *corners,
+                      int fric_switch, float fric_coef, float dt, float dx, float dy, int nx,
+                      int ny, const float *H, const float *Z) {
+  if (id < 1 || id >= kMaxLayers) return fail(1, "invalid layer id %d (1..%d)", id, kMaxLayers - 1);
+  if (nx < 3 || ny < 3) return fail(2, "layer %d: grid %dx%d too small", id, nx, ny);
+  if (id == 1 && parent != -1) return fail(3, "layer 1 must be the root (parent -1)");
+  if (id != 1) {
+    LayerRec *P = T.layer(parent);
+    if (!P) return fail(4, "layer %d: parent %d not initialised", id, parent);
+    const int is = corners[0], ie = corners[1], js = corners[2], je = corners[3];
+    if (is < 4 || js < 4 || ie > P->nx - 2 || je > P->ny - 2 || ie < is || je < js)
+      return fail(5, "layer %d: footprint I %d..%d J %d..%d must lie in 4..NX-2, 4..NY-2 of its parent",
+                  id, is, ie, js, je);
+    if (rel_size < 1 || nx != (ie - is + 1) * rel_size + 1 || ny != (je - js + 1) * rel_size + 1)
+      return fail(6, "layer %d: %dx%d does not match footprint x REL_SIZE %d + 1", id, nx, ny,
+                  rel_size);
+    if (rel_time < 1) return fail(7, "layer %d: REL_TIME %d < 1", id, rel_time);
+  }
+  if (!T.ctl) {
+    HIPCHK(hipMalloc(&T.ctl, 2 * sizeof(int)));
+    HIPCHK(hipMemsetAsync(T.ctl, 0, 2 * sizeof(int), st));
+  }
+  if (T.layers[id].used) {
+    if (id == 1) {  // a new root starts a new layer tree
+      for (int k = 1; k < kMaxLayers; ++k)
+        if (T.layers[k].used) free_layer(T.layers[k]);
+    } else {
+      if (LayerRec *op = T.layer(T.layers[id].parent)) {
+        auto &ch = op->children;
+        for (size_t q = 0; q < ch.size(); ++q)
+          if (ch[q] == id) { ch.erase(ch.begin() + q); break; }
+      }
+      free_layer(T.layers[id]);
+    }
+  }
+  if (id == 1) {
+    T.ctl_in = 0;
+    HIPCHK(hipMemsetAsync(T.ctl, 0, 2 * sizeof(int), st));
+  }
+  LayerRec &L = T.layers[id];
+  L.used = true;
+  L.id = id;
+  L.parent = id == 1 ? -1 : parent;
+  L.level = level;
+  L.laycord = laycord;
+  L.laygov = laygov;
+  L.sph = laycord == 0;
+  L.nl = L.sph ? (laygov == 1 || laygov == 9) : (laygov == 1 || laygov == 5 || laygov == 9);
+  L.rel_size = id == 1 ? 1 : rel_size;
+  L.rel_time = id == 1 ? 1 : rel_time;
+  if (id == 1) {
+    L.corners[0] = 1; L.corners[1] = nx; L.corners[2] = 1; L.corners[3] = ny;
+  } else {
+    for (int k = 0; k < 4; ++k) L.corners[k] = corners[k];
+  }
+  L.fric = fric_switch != 1;
+  L.fric_coef = fric_coef;
+  L.dt = dt;
+  L.dx = dx;
+  L.dy = dy;
+  // ALLOC (comcot.f90:537-552, 602); CONMOME drops convection for LAYGOV 9
+  L.rx = dt / dx;
+  L.ry = dt / dy;
+  L.grx = kGrav * L.rx;
+  L.gry = kGrav * L.ry;
+  L.modscm = id == 1 ? 0 : 1;
+  if (laygov % 2 != 0) L.modscm = 1;
+  L.linchk = laygov == 9 ? 0 : 1;
+  const float cst = 0.5f * dt * kGrav;
+  L.kfric = cst * fric_coef * fric_coef;
+  L.nx = nx;
+  L.ny = ny;
+  const size_t n = L.nxy();
+  L.block_bytes = layer_bytes(nx, ny, L.nl != 0, false);
+  {
+    hipError_t e = hipMalloc(&L.block, L.block_bytes);
+    if (e != hipSuccess) {
+      const size_t want = L.block_bytes;
+      L = LayerRec();
+      return fail(100 + (int)e, "layer %d: %zu bytes of device memory: %s", id, want,
+                  hipGetErrorString(e));
+    }
+  }
+  Carve c{static_cast<char *>(L.block)};
+  L.h = c.take<float>(n);
+  for (int k = 0; k < 2; ++k) {
+    L.z[k] = c.take<float>(n);
+    L.m[k] = c.take<float>(n);
+    L.n[k] = c.take<float>(n);
+  }
+  L.zmax_raw = c.take<float>(n);
+  L.zmax_wet = c.take<float>(n);
+  L.arrival = c.take<int>(n);
+  if (L.nl) L.dzp = c.take<float>(n);
+  L.xflux = c.take<float>(2 * (size_t)nx);
+  L.yflux = c.take<float>(2 * (size_t)ny);
+  L.coef = c.take<float>((size_t)kNumCoef * ny);
+  HIPCHK(hipMemsetAsync(L.block, 0, L.block_bytes, st));
+  HIPCHK(hipMemsetAsync(L.arrival, 0xff, n * sizeof(int), st));  // -1 = not arrived
+  HIPCHK(hipMemcpyAsync(L.h, H, n * sizeof(float), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(L.z[0], Z, n * sizeof(float), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(L.zmax_raw, Z, n * sizeof(float), hipMemcpyHostToDevice, st));
+  L.cur = 0;
+  if (id != 1) T.layers[parent].children.push_back(id);
+  HIPCHK(hipStreamSynchronize(st));
+  return 0;
+}
+
+int tree_upload_coef(LayerRec &L, const std::vector<float> &coef) {
+  if (coef.size() != (size_t)kNumCoef * L.ny) return fail(9, "coefficient table size");
+  HIPCHK(hipMemcpy(L.coef, coef.data(), coef.size() * sizeof(float), hipMemcpyHostToDevice));
+  return 0;
+}
+
+int tree_alloc_r24(LayerRec &L, const float *R2, const float *R4) {
+  const size_t n = L.nxy();
+  HIPCHK(hipMalloc(&L.r24, 2 * n * sizeof(float)));
+  HIPCHK(hipMemcpy(L.r24, R2, n * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(L.r24 + n, R4, n * sizeof(float), hipMemcpyHostToDevice));
+  return 0;
+}
+
+// ---------------------------------------------------------- device views
+DevLayer dev_view(const Tree &T, const LayerRec &L, bool with_max, bool advance) {
+  DevLayer d{};
+  d.nx = L.nx;
+  d.ny = L.ny;
+  d.is_root = L.id == 1;
+  d.modscm = L.modscm;
+  d.linchk = L.linchk;
+  d.fric = L.fric;
+  d.rx = L.rx;
+  d.ry = L.ry;
+  d.grx = L.grx;
+  d.gry = L.gry;
+  d.dt = L.dt;
+  d.kfric = L.kfric;
+  d.h = L.h;
+  d.coef = L.coef;
+  d.r24 = L.r24;
+  d.z1 = L.z[L.cur];
+  d.m1 = L.m[L.cur];
+  d.n1 = L.n[L.cur];
+  d.z2 = L.z[1 - L.cur];
+  d.m2 = L.m[1 - L.cur];
+  d.n2 = L.n[1 - L.cur];
+  d.dzp = L.dzp;
+  d.zmax = with_max ? L.zmax_wet : nullptr;
+  d.arrival = L.arrival;
+  d.ctl = T.ctl;
+  d.ctl_in = T.ctl_in;
+  d.advance = advance ? 1 : 0;
+  d.arr_thr = T.arr_thr;
+  return d;
+}
+
+DevNest nest_view(const LayerRec &P, const LayerRec &C) {
+  DevNest q{};
+  q.pnx = P.nx;
+  q.pny = P.ny;
+  q.p_sph = P.sph;
+  q.p_nl = P.nl;
+  q.p_r24 = P.r24 != nullptr;
+  q.ph = P.h;
+  q.pz2 = P.z[1 - P.cur];
+  q.pm1 = P.m[P.cur];
+  q.pn1 = P.n[P.cur];
+  q.pcoef = P.coef;
+  q.pr24 = P.r24;
+  q.pgrx = P.grx;
+  q.pgry = P.gry;
+  q.xflux = P.xflux;
+  q.yflux = P.yflux;
+  q.cnx = C.nx;
+  q.cny = C.ny;
+  q.rel_size = C.rel_size;
+  q.rel_time = C.rel_time;
+  q.is = C.corners[0];
+  q.ie = C.corners[1];
+  q.js = C.corners[2];
+  q.je = C.corners[3];
+  q.ch = C.h;
+  q.cz1 = C.z[C.cur];
+  q.cz2 = C.z[1 - C.cur];
+  q.cm1 = C.m[C.cur];
+  q.cn1 = C.n[C.cur];
+  q.pz2_out = P.z[1 - P.cur];
+  return q;
+}
+
+int tree_launch(const Tree &T, const LayerRec &L, int mode, bool with_max, bool advance,
+                hipStream_t st) {
+  DevLayer d = dev_view(T, L, with_max, advance);
+  hipError_t e = launch_step(d, mode, L.nl != 0, L.sph != 0, T.variant, st);
+  if (e != hipSuccess) return fail(200 + (int)e, "step kernel launch failed: %s", hipGetErrorString(e));
+  return 0;
+}
+
+// ALL_GRID (all_grids.f90:34-308) / subgrid_solver_rcsv (GPUall_grids.cu:655-699)
+int tree_enqueue_children(Tree &T, LayerRec &P, bool with_max, hipStream_t st) {
+  for (int cid : P.children) {
+    LayerRec &C = T.layers[cid];
+    const int nhalf = (int)std::floor(C.rel_time / 2.0f) + 1;
+    for (int k = 1; k <= C.rel_time; ++k) {
+      hipError_t e = launch_jnq(nest_view(P, C), k, st);
+      if (e != hipSuccess) return fail(201, "jnq launch failed: %s", hipGetErrorString(e));
+      if (C.children.empty()) {
+        if (int r = tree_launch(T, C, kFused, with_max, false, st)) return r;
+      } else {
+        if (int r = tree_launch(T, C, kMassOnly, false, false, st)) return r;
+        if (int r = tree_enqueue_children(T, C, with_max, st)) return r;
+        if (int r = tree_launch(T, C, kMomentOnly, with_max, false, st)) return r;
+      }
+      if (k == nhalf) {
+        e = launch_jnz(nest_view(P, C), st);
+        if (e != hipSuccess) return fail(202, "jnz launch failed: %s", hipGetErrorString(e));
+      }
+      C.cur = 1 - C.cur;  // UPDATE
+    }
+  }
+  return 0;
+}
+
+// comcot.f90:359-439: continuity + open boundary, nested layers, momentum
+// with the wet max/arrival, level rotation.
+int tree_enqueue_root_step(Tree &T, hipStream_t st) {
+  LayerRec *R = T.layer(1);
+  if (!R) return fail(8, "no root layer");
+  if (R->children.empty()) {
+    if (int r = tree_launch(T, *R, kFused, true, true, st)) return r;
+  } else {
+    if (int r = tree_launch(T, *R, kMassOnly, false, false, st)) return r;
+    if (int r = tree_enqueue_children(T, *R, true, st)) return r;
+    if (int r = tree_launch(T, *R, kMomentOnly, true, true, st)) return r;
+  }
+  R->cur = 1 - R->cur;
+  T.ctl_in = 1 - T.ctl_in;
+  return 0;
+}
+
+static long long launches_below(Tree &T, const LayerRec &P) {
+  long long n = 0;
+  for (int cid : P.children) {
+    const LayerRec &C = T.layers[cid];
+    const long long inner = C.children.empty() ? 1 : 2 + launches_below(T, C);
+    n += (long long)C.rel_time * (1 + inner) + 1;  // JNQ + step(s) per sub-step, one JNZ
+  }
+  return n;
+}
+
+long long tree_launches_per_step(Tree &T) {
+  LayerRec *R = T.layer(1);
+  if (!R) return 0;
+  return R->children.empty() ? 1 : 2 + launches_below(T, *R);
+}
+
+bool row_factor(const float *R, const float *H, int nx, int ny, int j, bool is_x, float *F) {
+  auto hpq = [&](int i) {  // PQ_DEPTH, 0-based i
+    float h = H[(size_t)j * nx + i];
+    if (!(h > kGx)) return 0.0f;
+    if (is_x) {
+      int ip1 = i + 1 >= nx - 1 ? nx - 1 : i + 1;
+      return 0.5f * (h + H[(size_t)j * nx + ip1]);
+    }
+    int jp1 = j + 1 >= ny - 1 ? ny - 1 : j + 1;
+    return 0.5f * (h + H[(size_t)jp1 * nx + i]);
+  };
+  int best = -1;
+  float bh = 0.0f;
+  for (int i = 0; i < nx; ++i) {
+    float v = fmaxf(hpq(i), 0.0f);
+    if (v > bh) { bh = v; best = i; }
+  }
+  auto check = [&](float f) {
+    for (int i = 0; i < nx; ++i)
+      if (!(f * fmaxf(hpq(i), 0.0f) == R[(size_t)j * nx + i])) return false;
+    return true;
+  };
+  if (best < 0) {  // row entirely dry: R must be all zero
+    *F = 0.0f;
+    return check(0.0f);
+  }
+  float f0 = R[(size_t)j * nx + best] / bh;
+  float c = f0;
+  for (int k = 0; k < 4; ++k) c = nextafterf(c, -INFINITY);
+  for (int k = 0; k < 9; ++k, c = nextafterf(c, INFINITY)) {
+    if (check(c)) { *F = c; return true; }
+  }
+  return false;
+}
+
+}  // namespace gc

@@ -255,6 +255,92 @@ def config2(pnx=2048, pny=1536, dx=2.0, footprint=450, rel_size=4, rel_time=4, c
     return Scenario(f"config2-{pnx}x{pny}+{cnx}x{cny}", [P, C], steps, {"seed": seed})
 
 
+def child_dt(laycord, laygov, h, dx, dy, lat_first, lat_last, dt_parent):
+    """CR_CHECK for a nested layer (initialization.f90:1206-1247): (REL_TIME, DT)."""
+    ha, hp = arr(h)
+    rt = ctypes.c_int(0)
+    dt = ctypes.c_float(0)
+    lib().gcomcot_child_dt_(ip(laycord), ip(laygov), ip(h.shape[1]), ip(h.shape[0]), hp,
+                            fp(dx), fp(dy), fp(lat_first), fp(lat_last), fp(dt_parent),
+                            ctypes.byref(rt), ctypes.byref(dt))
+    return rt.value, float(dt.value)
+
+
+def _cart_child(world, src, parent, dx, lon0, lat0, IS, JS, footprint_i, footprint_j,
+                rel_size, laygov=1, rel_time=None):
+    """Cartesian child over parent cells IS.., JS.. (ghost row/column first)."""
+    dlon = dx / 60.0
+    IE, JE = IS + footprint_i - 1, JS + footprint_j - 1
+    cnx = footprint_i * rel_size + 1
+    cny = footprint_j * rel_size + 1
+    west = lon0 + (IS - 1) * dlon - 0.5 * dlon
+    south = lat0 + (JS - 1) * dlon - 0.5 * dlon
+    clon = west + (np.arange(1, cnx + 1) - 1.5) * dlon / rel_size
+    clat = south + (np.arange(1, cny + 1) - 1.5) * dlon / rel_size
+    mid_lat = 0.5 * (clat[0] + clat[-1])
+    CLON, CLAT = np.meshgrid(clon, clat)
+    ch = depth(world, CLON, CLAT)
+    cz = eta(src, CLON, CLAT, ch)
+    cdx = float(np.float32(R_EARTH * math.cos(math.radians(mid_lat)) * dx * RAD_MIN / rel_size))
+    cdy = float(np.float32(R_EARTH * dx * RAD_MIN / rel_size))
+    if rel_time is None:
+        rel_time, cdt = child_dt(1, laygov, ch, cdx, cdy, 0.0, 0.0, parent.dt)
+    else:
+        cdt = float(np.float32(np.float32(parent.dt) / np.float32(rel_time)))
+    return LayerSpec(id=2, parent=1, level=2, laycord=1, laygov=laygov, nx=cnx, ny=cny,
+                     dx=cdx, dy=cdy, dt=cdt, h=ch, z0=cz, fric_switch=0 if laygov else 1,
+                     fric_coef=0.013 if laygov else 0.0, rel_size=rel_size,
+                     rel_time=rel_time, corners=(IS, IE, JS, JE))
+
+
+def config3(n=64, case=0, min_cells=256 * 256, max_cells=2048 * 1536, nest_share=0.25,
+            steps=(1000, 4000), child_side=(400, 1200)):
+    """Batch of independent scenarios.
+
+    Parent cell counts log-uniform in [min_cells, max_cells], aspect 4:3,
+    spacing 1-4 arc minutes, spherical linear; a share `nest_share` of the
+    cases carries one Cartesian nonlinear child (REL_SIZE 3-5, child side
+    `child_side` cells, REL_TIME from CR_CHECK) over the shelf ramp;
+    elliptical sources 0.5-3 m, sigma 20-80 km, random position and
+    strike; root step counts uniform in `steps`.  Returns a list of
+    Scenario, each with its own `steps`."""
+    rng = np.random.default_rng(SEED0 + 3000 + case)
+    out = []
+    for k in range(n):
+        cells = math.exp(rng.uniform(math.log(min_cells), math.log(max_cells)))
+        ny = max(24, int(round(math.sqrt(cells * 3.0 / 4.0))))
+        nx = max(32, int(round(ny * 4.0 / 3.0)))
+        # spacing 1-4 arc minutes, coarsest first dropped until the domain
+        # spans at most 50 degrees of latitude (and stays within +-65)
+        fits = [d for d in (1.0, 2.0, 3.0, 4.0) if (ny - 1) * d / 60.0 <= 50.0]
+        dx = float(rng.choice(fits))
+        nested = (k % round(1.0 / nest_share) == 0) if nest_share > 0 else False
+        span = (ny - 1) * dx / 60.0
+        lon0, lat0 = 140.0, float(rng.uniform(-10.0, 65.0 - span))
+        lon1 = lon0 + (nx - 1) * dx / 60.0
+        lat1 = lat0 + (ny - 1) * dx / 60.0
+        seed = SEED0 + 3000 + 100 * case + k
+        w = make_world(seed, lon0, lat0, lon1, lat1, 8, 0)
+        sa = float(rng.uniform(20.0, 80.0))
+        sb = float(rng.uniform(20.0, sa))
+        src = source(w, 0, float(rng.uniform(0.5, 3.0)), sa, sb, 0.0,
+                     float(rng.uniform(0.35, 0.8)), float(rng.uniform(0.2, 0.8)),
+                     float(rng.uniform(0.0, math.pi)))
+        P = _sph_root(w, nx, ny, dx, 0, src, 1, 0.0, lon0, lat0)
+        layers = [P]
+        if nested:
+            R = int(rng.choice([3, 4, 5]))
+            side = int(rng.uniform(child_side[0], child_side[1]))
+            fp_ = max(4, min(side // R, ny - 12, nx // 2))
+            IS = max(4, int(0.02 * nx))
+            JS = max(4, (ny - fp_) // 2 + 1)
+            layers.append(_cart_child(w, src, P, dx, lon0, lat0, IS, JS, fp_, fp_, R))
+        st = int(rng.integers(steps[0], steps[1] + 1))
+        name = f"config3-{k}-{nx}x{ny}" + (f"+{layers[1].nx}x{layers[1].ny}" if nested else "")
+        out.append(Scenario(name, layers, st, {"seed": seed}))
+    return out
+
+
 def custom(nx=64, ny=56, dx=2.0, laycord=0, laygov=0, fric_coef=0.013, child=None, case=0,
            steps=100, embayments=1, peak=1.5, cx=0.45, cy=0.5, strike=0.4, src_kind=0,
            sig=40.0, beach_slope=None, bay_floor=None, coast_frac=None):

@@ -163,6 +163,53 @@ int gcomcot_last_error_(char *buf, int *len);
 int gcomcot_set_tile_variant_(int *variant);
 int gcomcot_tile_variants_(int *count, int *tx, int *ty);
 
+/* ------------------------------------------------------------------ */
+/* Part 3: scenario batch engine (no reference counterpart)            */
+/* ------------------------------------------------------------------ */
+/* Many independent scenarios, each an outermost layer with at most one
+ * level of nested layers and its own number of root steps, advanced
+ * together: every phase of a root step is one launch over all active
+ * scenarios (a persistent kernel draining per-XCD tile queues), two root
+ * steps are replayed from a HIP graph, and fused tile shapes are picked
+ * by an on-line autotuner.  Each scenario's planes are bit-identical to a
+ * run of the same layers alone through Part 1/2.  Scenarios can be added
+ * and removed between calls (continuous batching).  All calls return 0 or
+ * an error code (see gcomcot_set_error_mode_ / gcomcot_last_error_). */
+
+/* drop every scenario and its device memory */
+int gcomcot_batch_reset_(void);
+/* new, empty scenario that will run *steps root steps; id in *sid */
+int gcomcot_batch_new_scenario_(long long *steps, int *sid);
+/* layer of scenario *sid; arguments as gcomcot_init_layer_ex_ (layer 1
+ * first, then children with *parent == 1) */
+int gcomcot_batch_add_layer_(int *sid, int *layerid, int *parent, int *laycord, int *laygov,
+                             int *rel_size, int *rel_time, int *corners, int *fric_switch,
+                             float *fric_coef, float *dt, float *dx, float *dy, float *coef,
+                             float *H, float *Z, int *nx, int *ny);
+/* advance every unfinished scenario by up to *nsteps root steps (each
+ * stops at its own count); *nfinished = scenarios finished and not yet
+ * removed.  Returns after the device work is complete. */
+int gcomcot_batch_run_(long long *nsteps, int *nfinished);
+/* ids of finished scenarios still held (up to *cap written); returns how
+ * many there are */
+int gcomcot_batch_finished_(int *sids, int *cap);
+int gcomcot_batch_progress_(int *sid, long long *done, long long *total);
+/* level-1 (current) Z, M, N of layer *lid of scenario *sid (NULL skips) */
+int gcomcot_batch_get_state_(int *sid, int *lid, float *z, float *m, float *n);
+/* wet-only maximum and first-arrival root step of layer *lid */
+int gcomcot_batch_get_max_(int *sid, int *lid, float *zmax_wet, int *arrival);
+int gcomcot_batch_remove_(int *sid);
+/* device bytes held by scenario *sid (0: all scenarios) */
+long long gcomcot_batch_device_bytes_(int *sid);
+/* device bytes one layer of nx x ny cells takes (admission control) */
+long long gcomcot_layer_footprint_(int *nx, int *ny, int *nonlinear);
+void gcomcot_batch_set_arrival_threshold_(float *thr);
+/* *autotune != 0: time the tile shapes on the live batch; *shape >= 0
+ * forces that fused tile shape (0: 64x8, 1: 32x16, 2: 64x16) */
+int gcomcot_batch_set_tuning_(int *autotune, int *shape);
+int gcomcot_batch_info_(long long *launches, long long *graph_replays, long long *plans,
+                        long long *cells_per_step, int *tuned_count);
+
 #ifdef __cplusplus
 }
 #endif

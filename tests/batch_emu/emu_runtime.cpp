@@ -1,0 +1,119 @@
+// CPU stand-in for the HIP runtime calls of tree.hip / batch.hip (see
+// hip/hip_runtime.h here).  Work submitted to a stream runs at once unless
+// the stream is capturing, in which case it is recorded into the graph.
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <vector>
+
+#include "hip/hip_runtime.h"
+
+struct Work {
+  void (*fn)(void *);
+  std::vector<char> arg;
+};
+struct EmuGraph {
+  std::vector<Work> work;
+};
+struct EmuStream {
+  bool capturing = false;
+  EmuGraph *graph = nullptr;
+};
+struct EmuEvent {
+  int dummy = 0;
+};
+
+static std::map<void *, size_t> g_alloc;
+static long long g_live = 0;
+
+long long emu_live_bytes() { return g_live; }
+
+void emu_submit(hipStream_t st, void (*fn)(void *), void *arg, size_t n) {
+  if (st && st->capturing) {
+    Work w{fn, std::vector<char>((char *)arg, (char *)arg + n)};
+    st->graph->work.push_back(std::move(w));
+    return;
+  }
+  fn(arg);
+}
+
+hipError_t hipSetDevice(int) { return hipSuccess; }
+hipError_t hipGetDevice(int *d) { *d = 0; return hipSuccess; }
+hipError_t hipGetDeviceProperties(hipDeviceProp_t *p, int) {
+  std::memset(p, 0, sizeof(*p));
+  std::strcpy(p->name, "emulator");
+  p->multiProcessorCount = 4;
+  p->warpSize = 64;
+  return hipSuccess;
+}
+const char *hipGetErrorString(hipError_t e) { return e == hipSuccess ? "success" : "error"; }
+hipError_t hipMalloc(void **p, size_t n) {
+  *p = std::calloc(n ? n : 1, 1);
+  if (!*p) return hipErrorOutOfMemory;
+  // fill with a pattern so that reads of never-written memory show up
+  std::memset(*p, 0x7f, n);
+  g_alloc[*p] = n;
+  g_live += (long long)n;
+  return hipSuccess;
+}
+hipError_t hipFree(void *p) {
+  if (!p) return hipSuccess;
+  auto it = g_alloc.find(p);
+  if (it == g_alloc.end()) { std::fprintf(stderr, "emu: free of unknown pointer\n"); std::abort(); }
+  g_live -= (long long)it->second;
+  g_alloc.erase(it);
+  std::free(p);
+  return hipSuccess;
+}
+struct SetArg { void *p; int v; size_t n; };
+static void do_set(void *a) { SetArg *s = (SetArg *)a; std::memset(s->p, s->v, s->n); }
+hipError_t hipMemset(void *p, int v, size_t n) { std::memset(p, v, n); return hipSuccess; }
+hipError_t hipMemsetAsync(void *p, int v, size_t n, hipStream_t st) {
+  if (st && st->capturing) { std::fprintf(stderr, "emu: memset captured into a graph\n"); std::abort(); }
+  SetArg a{p, v, n};
+  emu_submit(st, do_set, &a, sizeof(a));
+  return hipSuccess;
+}
+hipError_t hipMemcpy(void *d, const void *s, size_t n, hipMemcpyKind) { std::memcpy(d, s, n); return hipSuccess; }
+hipError_t hipMemcpyAsync(void *d, const void *s, size_t n, hipMemcpyKind k, hipStream_t st) {
+  if (st && st->capturing) { std::fprintf(stderr, "emu: memcpy captured\n"); std::abort(); }
+  return hipMemcpy(d, s, n, k);
+}
+hipError_t hipStreamCreateWithFlags(hipStream_t *s, unsigned) { *s = new EmuStream(); return hipSuccess; }
+hipError_t hipStreamDestroy(hipStream_t s) { delete s; return hipSuccess; }
+hipError_t hipStreamSynchronize(hipStream_t s) {
+  if (s && s->capturing) { std::fprintf(stderr, "emu: synchronize while capturing\n"); std::abort(); }
+  return hipSuccess;
+}
+hipError_t hipStreamBeginCapture(hipStream_t s, hipStreamCaptureMode) {
+  s->capturing = true;
+  s->graph = new EmuGraph();
+  return hipSuccess;
+}
+hipError_t hipStreamEndCapture(hipStream_t s, hipGraph_t *g) {
+  s->capturing = false;
+  *g = s->graph;
+  s->graph = nullptr;
+  return hipSuccess;
+}
+hipError_t hipGraphInstantiate(hipGraphExec_t *e, hipGraph_t g, hipGraphNode_t *, char *, size_t) {
+  *e = new EmuGraph(*g);
+  return hipSuccess;
+}
+hipError_t hipGraphLaunch(hipGraphExec_t e, hipStream_t st) {
+  for (auto &w : e->work) emu_submit(st, w.fn, w.arg.data(), w.arg.size());
+  return hipSuccess;
+}
+hipError_t hipGraphExecDestroy(hipGraphExec_t e) { delete e; return hipSuccess; }
+hipError_t hipGraphDestroy(hipGraph_t g) { delete g; return hipSuccess; }
+hipError_t hipEventCreate(hipEvent_t *e) { *e = new EmuEvent(); return hipSuccess; }
+hipError_t hipEventDestroy(hipEvent_t e) { delete e; return hipSuccess; }
+hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
+hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
+hipError_t hipEventElapsedTime(float *ms, hipEvent_t, hipEvent_t) {
+  static float t = 1.0f;
+  t = t * 0.5f + 0.25f;  // varying, so the tuner's choice is exercised
+  *ms = t;
+  return hipSuccess;
+}
