@@ -1,22 +1,35 @@
 """Benchmark of the COMCOT shallow-water step on one MI355X.
 
-Workload (default): the single-layer nonlinear spherical configuration,
+Headline (value): the single-layer nonlinear spherical configuration,
 4096 x 3072 cells at 0.5 arc-min, Manning friction n = 0.013, moving
 shoreline, wet-only maximum and first-arrival fields accumulated on the
-device, 5000 steps, FP32 — one kernel launch per time step.
+device, 5000 steps, FP32 -- one launch of the fused step kernel per time
+step.  value = cell updates / wall time of the timed steps (GCUPS).
 
-Timed region: K steps of the device loop with every input resident in
-HBM, bracketed by device synchronisation.  HIP events recorded on the
-library's stream between steps give each step's duration; for a single
-layer each step is one launch of the fused step kernel, so their mean is
-that kernel's average duration, from which the achieved algorithmic
-bandwidth is computed (36 B per cell update: Z, M, N, H read, Z, M, N
-written, the maximum read and written).
+Timed region: K steps with every input resident in HBM, bracketed by
+device synchronisation.  HIP events recorded on the library's stream
+between steps give each step's duration; for a single layer each step is
+one launch of the fused step kernel, so their mean is that kernel's
+average duration, from which the achieved algorithmic bandwidth is
+computed (36 B per cell update: Z, M, N, H read, Z, M, N written, the
+maximum read and written).  roofline.traffic is the HBM bytes per launch
+measured with rocprofv3 (FETCH_SIZE + WRITE_SIZE, each scaled by the
+same counter's reading for a copy of known size), read from
+profiles/traffic.json when it holds an entry for this workload.
 
 CPU baseline: the reference's own Fortran (flang-built from its sources
 into oracle/_ref) timed on a bounded sample (the first steps of the same
 workload, one core); the C++ restatement stands in when that build is
 absent.
+
+Batch (batch.*): the 64-scenario mix (parents 256x256..2048x1536 at 1-4
+arc-min, a quarter with a nonlinear Cartesian child, 1000-4000 steps each)
+run through the batch engine, against the same scenarios run one at a
+time through the whole-run device loop, and against the reference's
+per-step call sequence (drop-in entry points, synchronous per call)
+timed on a sample of steps.  Serving (serve.*): Poisson request streams
+drawn from the same generator plus 10 % 4096x3072 nonlinear cases, through
+the continuous-batching loop; p50/p99 time-to-forecast and scenarios/s.
 
 Prints one JSON line.
 """
@@ -34,6 +47,7 @@ import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL = {"linear": 28, "nonlinear_max": 36}
+METRIC = "GCUPS + %HBM roofline; batched scenarios/s with p50/p99 time-to-forecast"
 
 
 def parse():
@@ -45,8 +59,33 @@ def parse():
     ap.add_argument("--ny", type=int, default=3072)
     ap.add_argument("--cpu-steps", type=int, default=3,
                     help="steps of the bounded CPU-baseline sample (0 = skip)")
-    ap.add_argument("--tile-variant", type=int, default=-1)
+    ap.add_argument("--tile-variant", type=int, default=-1,
+                    help="tile variant of the step kernel (-1: autotune)")
+    ap.add_argument("--batch", type=int, default=1, help="run the 64-scenario batch (0 = skip)")
+    ap.add_argument("--batch-cases", type=int, default=64)
+    ap.add_argument("--dropin-sample", type=int, default=100,
+                    help="steps per scenario of the drop-in call-sequence sample")
+    ap.add_argument("--serve", type=int, default=1, help="run the serving streams (0 = skip)")
+    ap.add_argument("--serve-requests", type=int, default=24)
+    ap.add_argument("--serve-rates", default="2,5,10,20")
+    ap.add_argument("--accuracy", type=int, default=1,
+                    help="FP32 vs FP64-oracle error on a reduced grid (0 = skip)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run the streaming-copy probe on MIB MiB (counter calibration)")
     return ap.parse_args()
+
+
+def copy_probe(mib, reps=20):
+    import ctypes
+    from gcomcot._lib import check, fp, ip, lib
+    ms = ctypes.c_float(0)
+    check(lib().gcomcot_hbm_copy_(ip(mib), ip(reps), ctypes.byref(ms)), "copy probe")
+    gbs = 2 * (mib << 20) / (ms.value * 1e-3) / 1e9
+    return {"mib": mib, "ms_per_copy": round(ms.value, 5), "read_plus_write_GBs": round(gbs, 1)}
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def cpu_baseline(sc, nsteps):
@@ -69,28 +108,57 @@ def cpu_baseline(sc, nsteps):
                       f"{dt:.1f} s"}
 
 
-def main():
-    a = parse()
-    if a.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (got --gpus {a.gpus}); scenario-level "
-              "sharding across GPUs is described in DESIGN.md", file=sys.stderr)
-        sys.exit(2)
+def autotune_variant(run, steps=6):
+    """Time every tile variant of the fused step on the live layer (the
+    device loop, a few steps each) and keep the fastest."""
+    import ctypes
+    from gcomcot._lib import ip, lib
+    n = ctypes.c_int(0)
+    lib().gcomcot_tile_variants_(ctypes.byref(n), None, None)
+    best, best_ms = 0, float("inf")
+    for v in range(n.value):
+        lib().gcomcot_set_tile_variant_(ip(v))
+        run.run(2, graph=False)
+        _, each = run.timed(steps)
+        ms = float(np.median(each))
+        if ms < best_ms:
+            best, best_ms = v, ms
+    lib().gcomcot_set_tile_variant_(ip(best))
+    return best
+
+
+def traffic_entry(workload_key):
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f).get(workload_key)
+    except (OSError, ValueError):
+        return None
+
+
+def bench_c1(a):
     from gcomcot import DeviceRun, config1
     from gcomcot._lib import ip, lib
 
+    log(f"config1 {a.nx}x{a.ny}: generating")
     sc = config1(nx=a.nx, ny=a.ny, steps=a.steps)
     cells = sc.cell_updates_per_step()
+    run = DeviceRun(sc)
     if a.tile_variant >= 0:
         lib().gcomcot_set_tile_variant_(ip(a.tile_variant))
-    run = DeviceRun(sc)
+        variant = a.tile_variant
+    else:
+        variant = autotune_variant(run)
+    run.close()
+    run = DeviceRun(sc)  # fresh state after tuning
+    lib().gcomcot_set_tile_variant_(ip(variant))
     run.run(a.warmup, graph=False)
     lib().gcomcot_run_steps_(ip(0), ip(0))  # sync
+    log(f"config1: timing {a.steps} steps (tile variant {variant})")
     t0 = time.perf_counter()
     ms_total, each = run.timed(a.steps)
-    t1 = time.perf_counter()
-    wall = t1 - t0
-    # the first step's interval also holds the queue start; use the rest
-    per = each[1:] if len(each) > 1 else each
+    wall = time.perf_counter() - t0
+    per = each[1:] if len(each) > 1 else each  # the first interval holds the queue start
     kern_ms = float(np.mean(per))
     bpc = BYTES_PER_CELL["nonlinear_max"]
     achieved = bpc * cells / (kern_ms * 1e-3) / 1e9
@@ -98,14 +166,153 @@ def main():
     mx = run.maxima(1)
     arrived = int(np.count_nonzero(mx["arrival"] >= 0))
     run.close()
+    import ctypes
+    tx = (ctypes.c_int * 64)()
+    ty = (ctypes.c_int * 64)()
+    n = ctypes.c_int(0)
+    lib().gcomcot_tile_variants_(ctypes.byref(n), tx, ty)
+    key = f"config1-{a.nx}x{a.ny}"
+    tr = traffic_entry(key)
+    traffic = None
+    if tr and tr.get("tile") == [tx[variant], ty[variant]]:
+        traffic = tr.get("bytes_per_launch")
+    return sc, cells, gcups, wall, kern_ms, achieved, traffic, arrived, ms_total, \
+        (tx[variant], ty[variant])
 
+
+def bench_batch(a):
+    from gcomcot import DeviceRun
+    from gcomcot import scenarios as S
+    from gcomcot.batch import Batch
+
+    log(f"batch: generating {a.batch_cases} scenarios")
+    cs = S.config3(n=a.batch_cases)
+    upd = [sum(L.nx * L.ny * (L.rel_time if L.id > 1 else 1) for L in c.layers) * c.steps
+           for c in cs]
+    total = float(sum(upd))
+    b = Batch()
+    for c in cs:
+        b.add(c)
+    log("batch: running the batch engine")
+    t0 = time.perf_counter()
+    b.run(10 ** 9)
+    t_batch = time.perf_counter() - t0
+    info = b.info()
+    b.close()
+    log("batch: one scenario at a time, whole-run device loop")
+    t_alone = 0.0
+    for c in cs:
+        d = DeviceRun(c)
+        t0 = time.perf_counter()
+        d.run(c.steps)
+        t_alone += time.perf_counter() - t0
+        d.close()
+    log("batch: one scenario at a time, reference call sequence (sample)")
+    t_dropin = 0.0
+    k = max(1, a.dropin_sample)
+    for c in cs:
+        d = DeviceRun(c)
+        d.dropin_step()
+        t0 = time.perf_counter()
+        for _ in range(k):
+            d.dropin_step()
+        t_dropin += (time.perf_counter() - t0) / k * c.steps
+        d.close()
+    n = len(cs)
+    return {
+        "scenarios": n, "nested": sum(len(c.layers) > 1 for c in cs),
+        "cell_updates": total,
+        "batched_s": round(t_batch, 3), "batched_scenarios_per_s": round(n / t_batch, 3),
+        "batched_gcups": round(total / t_batch / 1e9, 3),
+        "alone_loop_s": round(t_alone, 3), "alone_loop_scenarios_per_s": round(n / t_alone, 3),
+        "alone_dropin_s_est": round(t_dropin, 3),
+        "alone_dropin_scenarios_per_s": round(n / t_dropin, 3),
+        "speedup_vs_alone_loop": round(t_alone / t_batch, 3),
+        "speedup_vs_dropin": round(t_dropin / t_batch, 3),
+        "dropin_sample": f"{k} steps per scenario, scaled to its step count",
+        "launches": info["launches"], "graph_replays": info["graph_replays"],
+        "plans": info["plans"],
+    }
+
+
+def bench_serve(a):
+    from gcomcot import scenarios as S
+    from gcomcot.serve import Request, Server, poisson_arrivals, summarize
+
+    rates = [float(r) for r in a.serve_rates.split(",") if r]
+    nreq = a.serve_requests
+    nbig = max(1, round(0.1 * nreq))
+    log(f"serve: generating {nbig} large nonlinear cases")
+    rng = np.random.default_rng(20261202 + 4000)
+    bigs = [S.config1(case=10 + i, steps=int(rng.integers(1000, 4001))) for i in range(nbig)]
+    out = {}
+    for ri, rate in enumerate(rates):
+        log(f"serve: {nreq} requests at {rate:g} req/s")
+        small = S.config3(n=nreq - nbig, case=10 + ri)
+        pool = small + bigs
+        order = np.random.default_rng(ri).permutation(len(pool))
+        t = poisson_arrivals(rate, len(pool), seed=100 + ri)
+        reqs = [Request(i, t[i], pool[j]) for i, j in enumerate(order)]
+        srv = Server(chunk=128)
+        done = srv.serve(reqs)
+        s = summarize(done)
+        srv.close()
+        s["rate_req_per_s"] = rate
+        s["peak_running"] = srv.peak_running
+        out[f"{rate:g}"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in s.items()}
+    return out
+
+
+def bench_accuracy():
+    """Max |FP32 - FP64 oracle| of the GPU and of the reference Fortran on a
+    reduced nonlinear grid (elevation m, flux m^2/s)."""
+    from gcomcot import DeviceRun
+    from gcomcot import scenarios as S
+    from oracle import bind
+    sc = S.config1(nx=256, ny=192, steps=400)
+    o64 = bind.Oracle(sc, 64)
+    o64.step(sc.steps)
+    r64 = o64.state(1)
+    d = DeviceRun(sc)
+    d.run(sc.steps)
+    g = d.state(1)
+    d.close()
+    err = lambda x, k: float(np.max(np.abs(np.asarray(x[k], np.float64) - r64[k])))
+    res = {"grid": "256x192 nonlinear spherical, 400 steps",
+           "gpu_fp32_vs_fp64": {k: err(g, k) for k in ("z1", "m1", "n1")}}
+    if bind.Reference.available():
+        ref = bind.Reference(sc)
+        ref.step(sc.steps)
+        rs = ref.state(1)
+        res["reference_fp32_vs_fp64"] = {k: err(rs, k) for k in ("z1", "m1", "n1")}
+        res["gpu_vs_reference_max_abs"] = {k: float(np.max(np.abs(
+            np.asarray(g[k], np.float64) - rs[k]))) for k in ("z1", "m1", "n1")}
+    return res
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (got --gpus {a.gpus}); scenario-level "
+              "sharding across GPUs is described in DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+    if a.copy_probe:
+        print(json.dumps({"copy_probe": copy_probe(a.copy_probe)}))
+        return
+    from gcomcot import config1
+
+    sc, cells, gcups, wall, kern_ms, achieved, traffic, arrived, ms_total, tile = bench_c1(a)
     cpu = None
     if a.cpu_steps > 0:
-        csc = config1(nx=a.nx, ny=a.ny, steps=a.cpu_steps)
-        cpu = cpu_baseline(csc, a.cpu_steps)
+        log("cpu baseline")
+        cpu = cpu_baseline(config1(nx=a.nx, ny=a.ny, steps=a.cpu_steps), a.cpu_steps)
+    batch = bench_batch(a) if a.batch else None
+    serve = bench_serve(a) if a.serve else None
+    acc = bench_accuracy() if a.accuracy else None
+    copy = copy_probe(1024)
 
     line = {
-        "metric": "GCUPS + %HBM roofline; batched scenarios/s with p50/p99 time-to-forecast",
+        "metric": METRIC,
         "value": round(gcups, 3),
         "unit": "GCUPS",
         "n_gpus": 1,
@@ -119,14 +326,18 @@ def main():
         "data": "synthetic (seeded bathymetry, dipole source)",
         "config": {"workload": f"single-layer nonlinear spherical {a.nx}x{a.ny} at 0.5 arc-min, "
                                "Manning 0.013, wet max + arrival on device",
-                   "cells": cells, "parallelism": "single GPU"},
+                   "cells": cells, "tile": list(tile), "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic},
         "cpu_baseline": cpu,
         "kernel_ms_mean": round(kern_ms, 5),
-        "kernel_ms_p50": round(float(np.median(per)), 5),
         "event_ms_total": round(ms_total, 3),
         "cells_arrived": arrived,
+        "hbm_copy": copy,
+        "batch": batch,
+        "serve": serve,
+        "accuracy": acc,
     }
     print(json.dumps(line))
 

@@ -318,6 +318,32 @@ int gcomcot_last_error_(char *buf, int *len) {
   return take_last_code();
 }
 
+int gcomcot_hbm_copy_(int *mib, int *reps, float *ms_per_copy) {
+  if (int e = ensure_init()) return e;
+  if (*mib < 1 || *reps < 1) return fail(27, "gcomcot_hbm_copy_: sizes must be positive");
+  const size_t bytes = (size_t)*mib << 20;
+  void *a = nullptr, *b = nullptr;
+  HIPCHK(hipMalloc(&a, bytes));
+  HIPCHK(hipMalloc(&b, bytes));
+  HIPCHK(hipMemsetAsync(a, 0, bytes, g_stream));
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(launch_copy(a, b, bytes, g_stream));  // warm-up
+  HIPCHK(hipEventRecord(e0, g_stream));
+  for (int k = 0; k < *reps; ++k) HIPCHK(launch_copy(a, b, bytes, g_stream));
+  HIPCHK(hipEventRecord(e1, g_stream));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  *ms_per_copy = ms / *reps;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  return 0;
+}
+
 int gcomcot_set_tile_variant_(int *variant) {
   if (*variant < 0 || *variant >= num_tile_variants()) return 1;
   g_tree.variant = *variant;

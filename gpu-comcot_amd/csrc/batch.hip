@@ -79,6 +79,7 @@ struct Plan {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   long long cells_per_step = 0;
+  bool owns_arena = true;  // false: the engine's reusable arena
 };
 
 struct Engine {
@@ -92,6 +93,8 @@ struct Engine {
   std::map<long long, int> tuned;  // (kind, size class) -> shape
   Plan plan;
   bool plan_valid = false;
+  char *arena = nullptr;   // descriptor/item memory reused by successive plans
+  size_t arena_bytes = 0;
   long long launches = 0, graph_replays = 0, plans_built = 0;
 };
 
@@ -110,7 +113,7 @@ int eng_init() {
 void free_plan(Plan &P) {
   if (P.exec) (void)hipGraphExecDestroy(P.exec);
   if (P.graph) (void)hipGraphDestroy(P.graph);
-  if (P.arena) (void)hipFree(P.arena);
+  if (P.arena && P.owns_arena) (void)hipFree(P.arena);
   P = Plan();
 }
 
@@ -136,10 +139,10 @@ std::vector<Scenario *> active() {
   return a;
 }
 
+// Tuning classes: layer kind and whether the launch alone fills the chip
+// (below ~4M cells a launch leaves CUs idle and favours smaller tiles).
 long long kind_key(int nl, int sph, long long cells) {
-  int cls = 0;
-  while ((1LL << (cls + 1)) <= cells) ++cls;
-  return ((long long)nl * 2 + sph) * 64 + cls;
+  return ((long long)nl * 2 + sph) * 2 + (cells >= (4LL << 20) ? 1 : 0);
 }
 
 int shape_for(int nl, int sph, long long cells) {
@@ -270,7 +273,7 @@ void emit_step(const std::vector<Scenario *> &act, std::vector<Group> &out) {
 
 size_t al(size_t x) { return (x + 255) & ~(size_t)255; }
 
-int upload_plan(Plan &P) {
+int upload_plan(Plan &P, bool engine_arena = false) {
   size_t bytes = 0;
   int heads = 0;
   for (int r = 0; r < P.nsteps; ++r) {
@@ -294,7 +297,19 @@ int upload_plan(Plan &P) {
   }
   const size_t o_heads = bytes;
   bytes += al((size_t)std::max(heads, 1) * sizeof(int));
-  HIPCHK(hipMalloc(&P.arena, bytes));
+  if (engine_arena) {
+    if (g_eng.arena_bytes < bytes) {
+      if (g_eng.arena) (void)hipFree(g_eng.arena);
+      g_eng.arena = nullptr;
+      g_eng.arena_bytes = 0;
+      HIPCHK(hipMalloc(&g_eng.arena, bytes + bytes / 2));
+      g_eng.arena_bytes = bytes + bytes / 2;
+    }
+    P.arena = g_eng.arena;
+    P.owns_arena = false;
+  } else {
+    HIPCHK(hipMalloc(&P.arena, bytes));
+  }
   std::vector<char> host(bytes, 0);
   for (int r = 0; r < P.nsteps; ++r)
     for (auto &g : P.step[r]) {
@@ -424,7 +439,7 @@ int build_plan(const std::vector<Scenario *> &act) {
                  P.step[1].size());
   for (auto &g : P.step[0])
     if (g.type == kStepGroup) P.cells_per_step += g.cells;
-  if (int r = upload_plan(P)) return r;
+  if (int r = upload_plan(P, true)) return r;
   if (P.step[0].size() + 1 > (size_t)kGraphMaxLaunches) {
     // a very deep sub-stepping (huge REL_TIME) is enqueued step by step:
     // graphs of that many nodes are not worth building
@@ -503,6 +518,9 @@ int gcomcot_batch_reset_(void) {
   (void)hipStreamSynchronize(g_eng.st);
   for (auto &kv : g_eng.scen) tree_free(kv.second->tree);
   g_eng.scen.clear();
+  if (g_eng.arena) (void)hipFree(g_eng.arena);
+  g_eng.arena = nullptr;
+  g_eng.arena_bytes = 0;
   g_eng.launches = g_eng.graph_replays = g_eng.plans_built = 0;
   return 0;
 }

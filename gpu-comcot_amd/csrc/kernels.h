@@ -90,5 +90,7 @@ hipError_t launch_jnq_multi(const DevNest *nests, const int *start, int nnest, i
 hipError_t launch_jnz_multi(const DevNest *nests, const int *start, int nnest, int nblocks,
                             hipStream_t st);
 hipError_t launch_fill_i32(int *p, int n, int v, hipStream_t st);
+// Streaming copy of bytes (a multiple of 16) from src to dst.
+hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st);
 
 }  // namespace gc

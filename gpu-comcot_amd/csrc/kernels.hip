@@ -971,6 +971,15 @@ __global__ void k_jnz_multi(const DevNest *__restrict__ nests, const int *__rest
   jnz_thread(N, (blockIdx.x - start[n]) * blockDim.x + threadIdx.x);
 }
 
+// Plain streaming copy (16 B per lane per access): the reference for
+// HBM bandwidth and for scaling the memory counters of rocprofv3.
+__global__ void __launch_bounds__(256) k_copy(const float4 *__restrict__ src,
+                                              float4 *__restrict__ dst, size_t n) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    dst[i] = src[i];
+}
+
 __global__ void k_fill_i32(int *p, int n, int v) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g < n) p[g] = v;
@@ -994,12 +1003,14 @@ __global__ void __launch_bounds__(kThreads)
   for (int c = 0; c < 8; ++c) {
     const int q = (x + c) & 7;
     const int lo = qstart[q], hi = qstart[q + 1];
+    if (threadIdx.x == 0) s_item = lo + atomicAdd(&heads[q], 1);
     for (;;) {
-      __syncthreads();  // previous tile done with LDS and s_item
-      if (threadIdx.x == 0) s_item = lo + atomicAdd(&heads[q], 1);
-      __syncthreads();
+      __syncthreads();  // s_item written; previous tile done with LDS
       const int it = __builtin_amdgcn_readfirstlane(s_item);
+      __syncthreads();  // every lane has read s_item
       if (it >= hi) break;
+      // claim the next item now: the atomic's round trip overlaps this tile
+      if (threadIdx.x == 0) s_item = lo + atomicAdd(&heads[q], 1);
       const BatchItem w = items[it];
       const int slot = __builtin_amdgcn_readfirstlane(w.slot);
       const int tile = __builtin_amdgcn_readfirstlane(w.tile);
@@ -1180,6 +1191,13 @@ hipError_t launch_jnz_multi(const DevNest *nests, const int *start, int nnest, i
                             hipStream_t st) {
   if (nnest <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_jnz_multi, dim3(nblocks), dim3(128), 0, st, nests, start, nnest);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st) {
+  const size_t n = bytes / 16;
+  hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, st, static_cast<const float4 *>(src),
+                     static_cast<float4 *>(dst), n);
   return hipGetLastError();
 }
 

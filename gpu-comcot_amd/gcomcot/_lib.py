@@ -38,6 +38,7 @@ def lib():
         L.gcomcot_cr_dt_.restype = ctypes.c_float
         L.gcomcot_set_tile_variant_.restype = ctypes.c_int
         L.gcomcot_tile_variants_.restype = ctypes.c_int
+        L.gcomcot_hbm_copy_.restype = ctypes.c_int
         _lib = L
     return _lib
 

@@ -163,6 +163,12 @@ int gcomcot_last_error_(char *buf, int *len);
 int gcomcot_set_tile_variant_(int *variant);
 int gcomcot_tile_variants_(int *count, int *tx, int *ty);
 
+/* Measurement aid: *reps copies of a *mib MiB buffer with a plain
+ * streaming kernel; *ms_per_copy = mean duration (HIP events).  Gives the
+ * chip's practical HBM bandwidth and the scale of rocprofv3's memory
+ * counters for a transfer of known size. */
+int gcomcot_hbm_copy_(int *mib, int *reps, float *ms_per_copy);
+
 /* ------------------------------------------------------------------ */
 /* Part 3: scenario batch engine (no reference counterpart)            */
 /* ------------------------------------------------------------------ */
