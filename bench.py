@@ -70,6 +70,8 @@ def parse():
     ap.add_argument("--serve-rates", default="2,5,10,20")
     ap.add_argument("--accuracy", type=int, default=1,
                     help="FP32 vs FP64-oracle error on a reduced grid (0 = skip)")
+    ap.add_argument("--graphs", type=int, default=1,
+                    help="batch engine replays HIP graphs (0: enqueue each step)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run the streaming-copy probe on MIB MiB (counter calibration)")
     return ap.parse_args()
@@ -190,7 +192,7 @@ def bench_batch(a):
     upd = [sum(L.nx * L.ny * (L.rel_time if L.id > 1 else 1) for L in c.layers) * c.steps
            for c in cs]
     total = float(sum(upd))
-    b = Batch()
+    b = Batch(graphs=bool(a.graphs))
     for c in cs:
         b.add(c)
     log("batch: running the batch engine")
@@ -204,7 +206,7 @@ def bench_batch(a):
     for c in cs:
         d = DeviceRun(c)
         t0 = time.perf_counter()
-        d.run(c.steps)
+        d.run(c.steps, graph=bool(a.graphs))
         t_alone += time.perf_counter() - t0
         d.close()
     log("batch: one scenario at a time, reference call sequence (sample)")

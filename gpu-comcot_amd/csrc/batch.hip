@@ -89,6 +89,7 @@ struct Engine {
   int next_sid = 1;
   float arr_thr = 0.01f;
   bool autotune = true;
+  bool graphs = true;      // replay plans from HIP graphs (else enqueue each step)
   int forced_shape = -1;
   std::map<long long, int> tuned;  // (kind, size class) -> shape
   Plan plan;
@@ -440,7 +441,7 @@ int build_plan(const std::vector<Scenario *> &act) {
   for (auto &g : P.step[0])
     if (g.type == kStepGroup) P.cells_per_step += g.cells;
   if (int r = upload_plan(P, true)) return r;
-  if (P.step[0].size() + 1 > (size_t)kGraphMaxLaunches) {
+  if (!g_eng.graphs || P.step[0].size() + 1 > (size_t)kGraphMaxLaunches) {
     // a very deep sub-stepping (huge REL_TIME) is enqueued step by step:
     // graphs of that many nodes are not worth building
     g_eng.plan_valid = true;
@@ -648,6 +649,12 @@ int gcomcot_batch_set_tuning_(int *autotune, int *shape) {
   if (*shape >= batch_num_shapes()) return fail(34, "batch: tile shape %d out of range", *shape);
   g_eng.autotune = *autotune != 0;
   g_eng.forced_shape = *shape;
+  invalidate();
+  return 0;
+}
+
+int gcomcot_batch_set_graphs_(int *on) {
+  g_eng.graphs = *on != 0;
   invalidate();
   return 0;
 }

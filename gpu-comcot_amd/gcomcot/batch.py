@@ -49,13 +49,14 @@ def _is_nl(s):
 class Batch:
     """The library's batch engine (one per process)."""
 
-    def __init__(self, arrival_threshold=0.01, autotune=True, shape=-1):
+    def __init__(self, arrival_threshold=0.01, autotune=True, shape=-1, graphs=True):
         L = lib()
         _setup(L)
         L.gcomcot_set_error_mode_(ip(0))
         check(L.gcomcot_batch_reset_(), "batch reset")
         L.gcomcot_batch_set_arrival_threshold_(fp(arrival_threshold))
         check(L.gcomcot_batch_set_tuning_(ip(1 if autotune else 0), ip(shape)), "batch tuning")
+        check(L.gcomcot_batch_set_graphs_(ip(1 if graphs else 0)), "batch graphs")
         self.specs = {}
 
     def add(self, sc, steps=None):

@@ -53,8 +53,9 @@ class DeviceRun:
         L = lib()
         L.gcomcot_set_error_mode_(ip(0))   # errors raise here instead of exiting
         L.cuda_shutdown_()
-        L.gcomcot_init_gpu_()
-        _ok("gcomcot_init_gpu_")
+        if reference_init:  # the reference's start-up call (prints the device)
+            L.gcomcot_init_gpu_()
+            _ok("gcomcot_init_gpu_")
         L.gcomcot_set_arrival_threshold_(fp(arrival_threshold))
         self.specs = {}
         for s in scenario.layers:

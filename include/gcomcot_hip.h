@@ -213,6 +213,9 @@ void gcomcot_batch_set_arrival_threshold_(float *thr);
 /* *autotune != 0: time the tile shapes on the live batch; *shape >= 0
  * forces that fused tile shape (0: 64x8, 1: 32x16, 2: 64x16) */
 int gcomcot_batch_set_tuning_(int *autotune, int *shape);
+/* *on == 0: enqueue every root step instead of replaying HIP graphs (for
+ * tools that cannot follow graph launches) */
+int gcomcot_batch_set_graphs_(int *on);
 int gcomcot_batch_info_(long long *launches, long long *graph_replays, long long *plans,
                         long long *cells_per_step, int *tuned_count);
 
