@@ -989,7 +989,11 @@ __global__ void k_fill_i32(int *p, int n, int v) {
 // pulls (layer, tile) work items.  Items are split into 8 contiguous
 // queues, one per XCD (workgroup b runs on XCD b % 8), each drained
 // through an atomic head; a workgroup whose own queue is empty takes items
-// from the others, so uneven scenario sizes still balance.  Every
+// from the others, so uneven scenario sizes still balance.  A claim takes
+// a run of consecutive items sized for about three claims per workgroup
+// and queue: device-scope atomics on one address are serialised across
+// the XCDs, so one claim per tile would cost more than the tile.  The
+// next claim is issued before the current run is processed.  Every
 // workgroup leaves once all 8 heads have passed their ends.
 template <int MODE, bool NL, bool SPH, int TX, int TY>
 __global__ void __launch_bounds__(kThreads)
@@ -1000,22 +1004,28 @@ __global__ void __launch_bounds__(kThreads)
   __shared__ int s_item;
   if (NL || !SPH) stage_tables(stab, threadIdx.x);
   const int x = blockIdx.x & 7;
+  const int nbq = ((int)gridDim.x + 7 - x) / 8;  // workgroups on this XCD
   for (int c = 0; c < 8; ++c) {
     const int q = (x + c) & 7;
     const int lo = qstart[q], hi = qstart[q + 1];
-    if (threadIdx.x == 0) s_item = lo + atomicAdd(&heads[q], 1);
+    if (hi <= lo) continue;
+    const int per = max(1, (hi - lo + 3 * nbq - 1) / (3 * nbq));
+    if (threadIdx.x == 0) s_item = lo + atomicAdd(&heads[q], per);
     for (;;) {
       __syncthreads();  // s_item written; previous tile done with LDS
-      const int it = __builtin_amdgcn_readfirstlane(s_item);
+      const int i0 = __builtin_amdgcn_readfirstlane(s_item);
       __syncthreads();  // every lane has read s_item
-      if (it >= hi) break;
-      // claim the next item now: the atomic's round trip overlaps this tile
-      if (threadIdx.x == 0) s_item = lo + atomicAdd(&heads[q], 1);
-      const BatchItem w = items[it];
-      const int slot = __builtin_amdgcn_readfirstlane(w.slot);
-      const int tile = __builtin_amdgcn_readfirstlane(w.tile);
-      const DevLayer L = descs[slot].L;
-      tile_step<MODE, NL, SPH, TX, TY>(L, tile, descs[slot].tiles_x, sm, &stab);
+      if (i0 >= hi) break;
+      if (threadIdx.x == 0) s_item = lo + atomicAdd(&heads[q], per);
+      const int i1 = min(i0 + per, hi);
+      for (int it = i0; it < i1; ++it) {
+        if (it > i0) __syncthreads();  // previous tile done with LDS
+        const BatchItem w = items[it];
+        const int slot = __builtin_amdgcn_readfirstlane(w.slot);
+        const int tile = __builtin_amdgcn_readfirstlane(w.tile);
+        const DevLayer L = descs[slot].L;
+        tile_step<MODE, NL, SPH, TX, TY>(L, tile, descs[slot].tiles_x, sm, &stab);
+      }
     }
   }
 }
