@@ -24,7 +24,8 @@ momt_launch_ openbd_launch_""".split()
 
 def header_functions():
     text = open(HEADER).read()
-    return sorted(set(re.findall(r"^\s*(?:void|int|float)\s+(\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:void|int|float|long long)\s+(\w+)\s*\(", text,
+                                 re.M)))
 
 
 def test_library_exports_every_declared_symbol():
@@ -40,6 +41,42 @@ def test_reference_boundary_is_complete():
     names = set(header_functions())
     missing = [s for s in REFERENCE_SYMBOLS if s not in names]
     assert not missing, missing
+
+
+FLANG = "/opt/rocm/lib/llvm/bin/flang"
+
+
+@pytest.mark.skipif(not os.path.exists(FLANG), reason="flang not installed")
+def test_fortran_interface_module_compiles(tmp_path):
+    import subprocess
+    src = os.path.join(ROOT, "include", "gcomcot_hip.f90")
+    subprocess.run([FLANG, "-c", src, "-o", str(tmp_path / "m.o"), "-module-dir",
+                    str(tmp_path)], check=True)
+    # every bind(C) name of the module is a symbol the library exports
+    names = re.findall(r'bind\(C, name="(\w+)"\)', open(src).read())
+    from gcomcot import lib
+    L = lib()
+    assert len(names) >= 30
+    for n in names:
+        assert hasattr(L, n), n
+
+
+@pytest.mark.skipif(not (os.path.exists(FLANG) and os.path.isdir("/root/reference")),
+                    reason="needs flang and the reference sources")
+def test_unmodified_reference_links_against_the_library():
+    """The reference's own Fortran, main program included, compiled as is
+    and linked against libgcomcot_hip.so: every GPU entry it calls must
+    resolve to this library (the link of the reference Makefile:38)."""
+    import subprocess
+    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "dropin",
+                    "REF=/root/reference"], check=True, capture_output=True)
+    exe = os.path.join(ROOT, "oracle", "_ref", "dropin", "comcot_hip")
+    out = subprocess.run(["nm", "-D", "--undefined-only", exe], check=True,
+                         capture_output=True, text=True).stdout
+    undefined = {ln.split()[-1] for ln in out.splitlines() if ln.strip()}
+    assert set(REFERENCE_SYMBOLS) <= undefined  # taken from the shared library
+    ldd = subprocess.run(["ldd", exe], check=True, capture_output=True, text=True).stdout
+    assert "libgcomcot_hip.so" in ldd and "not found" not in ldd.split("libgcomcot_hip")[1][:80]
 
 
 def _libm():
