@@ -69,7 +69,7 @@ def parse():
     ap.add_argument("--serve-requests", type=int, default=24)
     ap.add_argument("--serve-rates", default="2,5,10,20")
     ap.add_argument("--accuracy", type=int, default=1,
-                    help="FP32 vs FP64-oracle error on a reduced grid (0 = skip)")
+                    help="with the CPU baseline: FP32 vs FP64-oracle error on a reduced grid")
     ap.add_argument("--graphs", type=int, default=1,
                     help="batch engine replays HIP graphs (0: enqueue each step)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -265,9 +265,10 @@ def bench_serve(a):
     return out
 
 
-def bench_accuracy():
-    """Max |FP32 - FP64 oracle| of the GPU and of the reference Fortran on a
-    reduced nonlinear grid (elevation m, flux m^2/s)."""
+def cpu_accuracy():
+    """Part of the CPU-baseline leg: max |FP32 - FP64 oracle| of the GPU and
+    of the reference Fortran on a reduced nonlinear grid (elevation m, flux
+    m^2/s); the oracle and the reference are the checkers here."""
     from gcomcot import DeviceRun
     from gcomcot import scenarios as S
     from oracle import bind
@@ -308,9 +309,10 @@ def main():
     if a.cpu_steps > 0:
         log("cpu baseline")
         cpu = cpu_baseline(config1(nx=a.nx, ny=a.ny, steps=a.cpu_steps), a.cpu_steps)
+        if a.accuracy:
+            cpu["accuracy"] = cpu_accuracy()
     batch = bench_batch(a) if a.batch else None
     serve = bench_serve(a) if a.serve else None
-    acc = bench_accuracy() if a.accuracy else None
     copy = copy_probe(1024)
 
     line = {
@@ -339,7 +341,6 @@ def main():
         "hbm_copy": copy,
         "batch": batch,
         "serve": serve,
-        "accuracy": acc,
     }
     print(json.dumps(line))
 

@@ -19,13 +19,14 @@
 // captured in a HIP graph and replayed; the plan is rebuilt when the set
 // of active scenarios changes (a scenario finishes, is added or removed).
 //
-// Step launches use the persistent kernel k_batch: (layer, tile) items in
-// 8 per-XCD queues with atomic heads (cleared by a fill kernel at the
-// start of each root step, inside the graph).  Tile shapes of the fused
-// launches come from the autotuner below: each candidate shape is timed
-// on the live batch with the max/arrival outputs switched off (such a
-// "dry" launch writes only level-2 planes, which the next real step
-// rewrites), and the fastest is kept per (layer kind, size class).
+// Step launches run (layer, tile) items either in the persistent kernel
+// k_batch (8 per-XCD queues with atomic heads, cleared by a fill kernel at
+// the start of each root step, inside the graph) or one workgroup per item
+// (k_batch_grid).  Form and tile shape of the fused launches come from the
+// autotuner below: each candidate is timed on the live batch with the
+// max/arrival outputs switched off (such a "dry" launch writes only
+// level-2 planes, which the next real step rewrites), and the fastest is
+// kept per (layer kind, size class).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -149,7 +150,7 @@ long long kind_key(int nl, int sph, long long cells) {
 int shape_for(int nl, int sph, long long cells) {
   if (g_eng.forced_shape >= 0) return g_eng.forced_shape;
   auto it = g_eng.tuned.find(kind_key(nl, sph, cells));
-  return it == g_eng.tuned.end() ? 0 : it->second;
+  return it == g_eng.tuned.end() ? batch_num_shapes() / 2 : it->second;
 }
 
 // Step group for layers of one kind and mode; tiles of tx x ty.
@@ -185,9 +186,11 @@ void flush_steps(std::vector<Pending> &pend, std::vector<Group> &out) {
     g.sph = p.sph;
     long long cells = 0;
     for (auto &d : p.layers) cells += (long long)d.nx * d.ny;
-    g.shape = p.mode == kFused ? shape_for(p.nl, p.sph, cells) : 0;
+    // fused launches: the tuned form and tile; continuity/momentum halves
+    // of nested roots: one workgroup per 64 x 8 tile
+    g.shape = p.mode == kFused ? shape_for(p.nl, p.sph, cells) : batch_num_shapes() / 2;
     int tx = 64, ty = 8;
-    if (p.mode == kFused) batch_shape(g.shape, &tx, &ty);
+    batch_shape(g.shape, &tx, &ty);
     for (auto &d : p.layers) add_layer_items(g, d, tx, ty);
     g.recs = p.recs;
     split_queues(g);

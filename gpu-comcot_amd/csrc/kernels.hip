@@ -47,21 +47,22 @@ __device__ __forceinline__ float cf(const DevLayer &L, int k, int j) {
   return L.coef[k * L.ny + (j - 1)];
 }
 
-// PQ_DEPTH (initialization.f90:2512-2523)
-template <class A>
+// PQ_DEPTH (initialization.f90:2512-2523).  IN: the cell is known to lie
+// away from the grid edges, so no index clamp can apply.
+template <bool IN, class A>
 __device__ __forceinline__ float hp_at(const DevLayer &L, const A &s, int i, int j) {
   const float h = s.H(i, j);
   if (h > kGx) {
-    const int ip1 = i + 1 >= L.nx ? L.nx : i + 1;
+    const int ip1 = (IN || i + 1 < L.nx) ? i + 1 : L.nx;
     return 0.5f * (h + s.H(ip1, j));
   }
   return 0.0f;
 }
-template <class A>
+template <bool IN, class A>
 __device__ __forceinline__ float hq_at(const DevLayer &L, const A &s, int i, int j) {
   const float h = s.H(i, j);
   if (h > kGx) {
-    const int jp1 = j + 1 >= L.ny ? L.ny : j + 1;
+    const int jp1 = (IN || j + 1 < L.ny) ? j + 1 : L.ny;
     return 0.5f * (h + s.H(i, jp1));
   }
   return 0.0f;
@@ -134,15 +135,15 @@ __device__ float open_cell(const DevLayer &L, const A &s, int i, int j) {
 // Continuity for one cell: MASS_S (mass.f90:61-100) / MASS_C (:103-157),
 // CONMASS_S (:227-277) / CONMASS (:162-224) when NL.  dz2 is the total
 // depth DZ(:,:,2) of the nonlinear forms (0 where not computed).
-template <bool NL, bool SPH, class A>
+template <bool NL, bool SPH, bool IN, class A>
 __device__ __forceinline__ void mass_cell(const DevLayer &L, const A &s, int i, int j,
                                           float &z2, float &dz2) {
-  if (L.is_root && (i == 1 || j == 1 || i == L.nx || j == L.ny)) {
+  if (!IN && L.is_root && (i == 1 || j == 1 || i == L.nx || j == L.ny)) {
     z2 = open_cell(L, s, i, j);
     dz2 = 0.0f;
     return;
   }
-  if (i < 2 || j < 2) {  // ghost row/column of an inner layer
+  if (!IN && (i < 2 || j < 2)) {  // ghost row/column of an inner layer
     z2 = 0.0f;
     dz2 = 0.0f;
     return;
@@ -181,22 +182,22 @@ __device__ __forceinline__ void mass_cell(const DevLayer &L, const A &s, int i, 
 
 // ---------------------------------------------------------------- linear
 // MOMT_S x (moment.f90:86-115) / MOMT_C x (moment.f90:190-230)
-template <bool SPH, class A>
+template <bool SPH, bool IN, class A>
 __device__ float lin_x(const DevLayer &L, const A &s, int i, int j) {
   if (!(s.H(i, j) > kGx && s.H(i + 1, j) > kGx)) return 0.0f;
   if (SPH) {
     int jm1 = j - 1;
-    if (jm1 <= 1) jm1 = 1;
+    if (!IN && jm1 <= 1) jm1 = 1;
     float tot_n = s.N(i, j) + s.N(i + 1, j) + s.N(i, jm1) + s.N(i + 1, jm1);
-    float r2 = L.r24 ? L.r24[gidx(L, i, j)] : cf(L, kF2, j) * fmaxf(hp_at(L, s, i, j), 0.0f);
+    float r2 = L.r24 ? L.r24[gidx(L, i, j)] : cf(L, kF2, j) * fmaxf(hp_at<IN>(L, s, i, j), 0.0f);
     float xm = s.M(i, j) - r2 * (s.Z2(i + 1, j) - s.Z2(i, j)) + cf(L, kR3, j) * tot_n;
     if (fabsf(xm) < kEps) xm = 0.0f;
     return xm;
   } else {
     int jm1 = j - 1, jp1 = j + 1;
-    if (jm1 < 1) jm1 = 1;
-    if (jp1 > L.ny) jp1 = L.ny;
-    float hm = hp_at(L, s, i, j) + 0.5f * (s.Z2(i, j) + s.Z2(i + 1, j));
+    if (!IN && jm1 < 1) jm1 = 1;
+    if (!IN && jp1 > L.ny) jp1 = L.ny;
+    float hm = hp_at<IN>(L, s, i, j) + 0.5f * (s.Z2(i, j) + s.Z2(i + 1, j));
     float xm = s.M(i, j) - L.grx * hm * (s.Z2(i + 1, j) - s.Z2(i, j));
     float ff = 0.0f;
     if (L.fric) {
@@ -220,23 +221,23 @@ __device__ float lin_x(const DevLayer &L, const A &s, int i, int j) {
 }
 
 // MOMT_S y (moment.f90:117-146) / MOMT_C y (moment.f90:233-273)
-template <bool SPH, class A>
+template <bool SPH, bool IN, class A>
 __device__ float lin_y(const DevLayer &L, const A &s, int i, int j) {
   if (!(s.H(i, j) > kGx && s.H(i, j + 1) > kGx)) return 0.0f;
   if (SPH) {
     int im1 = i - 1;
-    if (im1 <= 1) im1 = 1;
+    if (!IN && im1 <= 1) im1 = 1;
     float tot_m = s.M(im1, j) + s.M(im1, j + 1) + s.M(i, j) + s.M(i, j + 1);
     float r4 = L.r24 ? L.r24[(size_t)L.nx * L.ny + gidx(L, i, j)]
-                     : cf(L, kF4, j) * fmaxf(hq_at(L, s, i, j), 0.0f);
+                     : cf(L, kF4, j) * fmaxf(hq_at<IN>(L, s, i, j), 0.0f);
     float xn = s.N(i, j) - r4 * (s.Z2(i, j + 1) - s.Z2(i, j)) - cf(L, kR5, j) * tot_m;
     if (fabsf(xn) < kEps) xn = 0.0f;
     return xn;
   } else {
     int im1 = i - 1, ip1 = i + 1;
-    if (im1 < 1) im1 = 1;
-    if (ip1 >= L.nx) ip1 = L.nx;
-    float hn = hq_at(L, s, i, j) + 0.5f * (s.Z2(i, j) + s.Z2(i, j + 1));
+    if (!IN && im1 < 1) im1 = 1;
+    if (!IN && ip1 >= L.nx) ip1 = L.nx;
+    float hn = hq_at<IN>(L, s, i, j) + 0.5f * (s.Z2(i, j) + s.Z2(i, j + 1));
     float xn = s.N(i, j) - L.gry * hn * (s.Z2(i, j + 1) - s.Z2(i, j));
     float ff = 0.0f;
     if (L.fric) {
@@ -263,69 +264,69 @@ __device__ float lin_y(const DevLayer &L, const A &s, int i, int j) {
 // Total depths at the flux points (moment.f90:345-366, 849-870); DZ at
 // level 1 is identically zero in the reference (never refreshed after
 // ALLOC), so DP(I,J,1) = 0.25*(DZ2(I)+DZ2(I+1)).
-template <class A>
+template <bool IN, class A>
 __device__ __forceinline__ float dp1_at(const DevLayer &L, const A &s, int i, int j) {
-  const int ip1 = i + 1 >= L.nx ? L.nx : i + 1;
+  const int ip1 = (IN || i + 1 < L.nx) ? i + 1 : L.nx;
   float v = 0.25f * (s.DZ(i, j) + s.DZ(ip1, j));
   return v < kGx ? 0.0f : v;
 }
-template <class A>
+template <bool IN, class A>
 __device__ __forceinline__ float dp2_at(const DevLayer &L, const A &s, int i, int j) {
-  const int ip1 = i + 1 >= L.nx ? L.nx : i + 1;
+  const int ip1 = (IN || i + 1 < L.nx) ? i + 1 : L.nx;
   float v = 0.5f * (s.DZ(i, j) + s.DZ(ip1, j));
   return v < kGx ? 0.0f : v;
 }
-template <class A>
+template <bool IN, class A>
 __device__ __forceinline__ float dq1_at(const DevLayer &L, const A &s, int i, int j) {
-  const int jp1 = j + 1 >= L.ny ? L.ny : j + 1;
+  const int jp1 = (IN || j + 1 < L.ny) ? j + 1 : L.ny;
   float v = 0.25f * (s.DZ(i, j) + s.DZ(i, jp1));
   return v < kGx ? 0.0f : v;
 }
-template <class A>
+template <bool IN, class A>
 __device__ __forceinline__ float dq2_at(const DevLayer &L, const A &s, int i, int j) {
-  const int jp1 = j + 1 >= L.ny ? L.ny : j + 1;
+  const int jp1 = (IN || j + 1 < L.ny) ? j + 1 : L.ny;
   float v = 0.5f * (s.DZ(i, j) + s.DZ(i, jp1));
   return v < kGx ? 0.0f : v;
 }
 
-// WAVE_BREAKING (moment.f90:1272-1384): damp the convection terms where
-// the surface rises faster than 0.08*sqrt(g*D).
+// WAVE_BREAKING (moment.f90:1272-1384): the factor applied to both
+// convection terms where the surface rises faster than 0.08*sqrt(g*D),
+// else 1 (multiplying by 1.0f leaves every value, NaN and infinities
+// included, unchanged, so one factor per cell serves both directions).
 template <class A>
-__device__ __forceinline__ void wave_breaking(const DevLayer &L, const A &s, int i, int j,
-                                              float &advx, float &advy) {
+__device__ __forceinline__ float breaking_coef(const DevLayer &L, const A &s, int i, int j) {
   float c = sqrtf(kGrav * s.DZ(i, j));
   float dzdt_f = 0.08f * c;
   float dzdt_i = 0.65f * c;
   float dzdt = fabsf(s.Z2(i, j) - s.Z1(i, j)) / L.dt;
   if (dzdt > dzdt_f) {
     float ref = (dzdt - dzdt_f) / (dzdt_i - dzdt_f);
-    float coef = gcdet::expf_det(-(1.0f * ref), *s.tab);
-    advx = coef * advx;
-    advy = coef * advy;
+    return gcdet::expf_det(-(1.0f * ref), *s.tab);
   }
+  return 1.0f;
 }
 
 // X flux of CONMOME_S (moment.f90:889-1066, SPH) / CONMOME (:393-577)
-template <bool SPH, class A>
-__device__ float nl_x(const DevLayer &L, const A &s, int i, int j) {
+template <bool SPH, bool IN, class A>
+__device__ float nl_x(const DevLayer &L, const A &s, int i, int j, float brk) {
   const int IX = L.nx, JY = L.ny;
   int ip1 = i + 1, im1 = i - 1;
-  if (im1 <= 1) im1 = 1;
-  if (ip1 >= IX) ip1 = IX;
+  if (!IN && im1 <= 1) im1 = 1;
+  if (!IN && ip1 >= IX) ip1 = IX;
   int jp1 = j + 1, jm1 = j - 1;
-  if (jm1 <= 1) jm1 = 1;
-  if (jp1 >= JY) jp1 = JY;
-  if (s.H(i, j) <= kElmax || hp_at(L, s, i, j) <= kElmax) return 0.0f;
+  if (!IN && jm1 <= 1) jm1 = 1;
+  if (!IN && jp1 >= JY) jp1 = JY;
+  if (s.H(i, j) <= kElmax || hp_at<IN>(L, s, i, j) <= kElmax) return 0.0f;
   float DD, DF;
   const float dzi = s.DZ(i, j), dzp = s.DZ(ip1, j);
   if (dzi > kGx && dzp > kGx) {
-    DD = dp2_at(L, s, i, j);
-    DF = dp1_at(L, s, i, j);
+    DD = dp2_at<IN>(L, s, i, j);
+    DF = dp1_at<IN>(L, s, i, j);
   } else if (dzi > kGx && dzp <= kGx && s.H(ip1, j) + s.Z2(i, j) > kGx) {
-    DD = SPH ? hp_at(L, s, i, j) + 0.5f * (s.Z2(i, j) + s.Z2(ip1, j)) : 0.5f * dzi;
+    DD = SPH ? hp_at<IN>(L, s, i, j) + 0.5f * (s.Z2(i, j) + s.Z2(ip1, j)) : 0.5f * dzi;
     DF = DD;
   } else if (dzi <= kGx && dzp > kGx && s.H(i, j) + s.Z2(ip1, j) > kGx) {
-    DD = SPH ? hp_at(L, s, i, j) + 0.5f * (s.Z2(i, j) + s.Z2(ip1, j)) : 0.5f * dzp;
+    DD = SPH ? hp_at<IN>(L, s, i, j) + 0.5f * (s.Z2(i, j) + s.Z2(ip1, j)) : 0.5f * dzp;
     DF = DD;
   } else {
     return 0.0f;
@@ -339,7 +340,7 @@ __device__ float nl_x(const DevLayer &L, const A &s, int i, int j) {
   float XP;
   if (SPH) {
     const float r2 = cf(L, kF2, j);
-    if (L.modscm == 0 && j < JY) {
+    if (L.modscm == 0 && (IN || j < JY)) {
       XP = p - r2 * DD * (s.Z2(ip1, j) - s.Z2(i, j)) -
            r2 * DD / 12.0f *
                ((s.Z2(ip1, jp1) - 2.0f * s.Z2(ip1, j) + s.Z2(ip1, jm1)) -
@@ -351,7 +352,7 @@ __device__ float nl_x(const DevLayer &L, const A &s, int i, int j) {
     XP = XP + 4.0f * cf(L, kR3, j) * xqq;
   } else {
     XP = p - L.grx * DD * (s.Z2(ip1, j) - s.Z2(i, j));
-    if (L.modscm == 0 && j < JY) {
+    if (L.modscm == 0 && (IN || j < JY)) {
       float scm = L.grx * DD / 12.0f *
                   ((s.Z2(ip1, jp1) - 2.0f * s.Z2(ip1, j) + s.Z2(ip1, jm1)) -
                    (s.Z2(i, jp1) - 2.0f * s.Z2(i, j) + s.Z2(i, jm1)));
@@ -359,16 +360,16 @@ __device__ float nl_x(const DevLayer &L, const A &s, int i, int j) {
     }
     if (L.fric) XP = XP - FF * p;
   }
-  if (L.linchk == 1 && dp1_at(L, s, i, j) >= kGx &&
-      (i > kSpan && i < IX - kSpan && j > kSpan && j < JY - kSpan)) {
+  if (L.linchk == 1 && dp1_at<IN>(L, s, i, j) >= kGx &&
+      (IN || (i > kSpan && i < IX - kSpan && j > kSpan && j < JY - kSpan))) {
     const float fx = SPH ? cf(L, kR21, j) : L.rx;
     const float fy = SPH ? cf(L, kR0, j) : L.ry;
-    const float d = dp1_at(L, s, i, j);
+    const float d = dp1_at<IN>(L, s, i, j);
     // Upwind terms without divergent branches: each Fortran case is one of
     // fx*(-A), fx*A, fx*(B-A), fx*(A-B) of the same two quotients.
     const bool pn = p < 0.0f;
     const int ia = pn ? ip1 : im1;
-    const float da = dp1_at(L, s, ia, j);
+    const float da = dp1_at<IN>(L, s, ia, j);
     const bool degx = pn ? (da < kGx || s.DZ(ip1, j) < kGx) : (da < kGx || s.DZ(i, j) < kGx);
     const float ax = p * p / d;
     const float pa = s.M(ia, j);
@@ -376,15 +377,13 @@ __device__ float nl_x(const DevLayer &L, const A &s, int i, int j) {
     const float advx = fx * (degx ? (pn ? -ax : ax) : (pn ? bx - ax : ax - bx));
     const bool qn = xqq < 0.0f;
     const int jn = qn ? jp1 : jm1, jf = qn ? j : j - 2;
-    const float dn = dp1_at(L, s, i, jn);
+    const float dn = dp1_at<IN>(L, s, i, jn);
     const bool degy = s.DZ(i, jn) < kGx || s.DZ(ip1, jn) < kGx || dn < kGx;
     const float ay = p * xqq / d;
     const float xqe = 0.25f * (s.N(i, jn) + s.N(ip1, jn) + s.N(i, jf) + s.N(ip1, jf));
     const float by = s.M(i, jn) * xqe / dn;
     float advy = fy * (degy ? (qn ? -ay : ay) : (qn ? by - ay : ay - by));
-    float advx_ = advx;
-    wave_breaking(L, s, i, j, advx_, advy);
-    XP = XP - advx_ - advy;
+    XP = XP - brk * advx - brk * advy;
   }
   if (L.fric) XP = XP / (1.0f + FF);
   if (fabsf(XP) < kEps) XP = 0.0f;
@@ -397,26 +396,26 @@ __device__ float nl_x(const DevLayer &L, const A &s, int i, int j) {
 // Y flux of CONMOME_S (moment.f90:1071-1258, SPH) / CONMOME (:582-770).
 // CONMOME's Y equation subtracts its friction term from the X variable
 // (moment.f90:672), so only the 1/(1+FF) factor reaches XQ there.
-template <bool SPH, class A>
-__device__ float nl_y(const DevLayer &L, const A &s, int i, int j) {
+template <bool SPH, bool IN, class A>
+__device__ float nl_y(const DevLayer &L, const A &s, int i, int j, float brk) {
   const int IX = L.nx, JY = L.ny;
   int ip1 = i + 1, im1 = i - 1;
-  if (im1 <= 1) im1 = 1;
-  if (ip1 >= IX) ip1 = IX;
+  if (!IN && im1 <= 1) im1 = 1;
+  if (!IN && ip1 >= IX) ip1 = IX;
   int jp1 = j + 1, jm1 = j - 1;
-  if (jm1 <= 1) jm1 = 1;
-  if (jp1 >= JY) jp1 = JY;
-  if (s.H(i, j) <= kElmax || hq_at(L, s, i, j) <= kElmax) return 0.0f;
+  if (!IN && jm1 <= 1) jm1 = 1;
+  if (!IN && jp1 >= JY) jp1 = JY;
+  if (s.H(i, j) <= kElmax || hq_at<IN>(L, s, i, j) <= kElmax) return 0.0f;
   float DD, DF;
   const float dzi = s.DZ(i, j), dzp = s.DZ(i, jp1);
   if (dzi > kGx && dzp > kGx) {
-    DD = dq2_at(L, s, i, j);
-    DF = dq1_at(L, s, i, j);
+    DD = dq2_at<IN>(L, s, i, j);
+    DF = dq1_at<IN>(L, s, i, j);
   } else if (dzi > kGx && dzp <= kGx && s.H(i, jp1) + s.Z2(i, j) > kGx) {
-    DD = SPH ? hq_at(L, s, i, j) + 0.5f * (s.Z2(i, j) + s.Z2(i, jp1)) : 0.5f * dzi;
+    DD = SPH ? hq_at<IN>(L, s, i, j) + 0.5f * (s.Z2(i, j) + s.Z2(i, jp1)) : 0.5f * dzi;
     DF = DD;
   } else if (dzi <= kGx && dzp > kGx && s.H(i, j) + s.Z2(i, jp1) > kGx) {
-    DD = SPH ? hq_at(L, s, i, j) + 0.5f * (s.Z2(i, j) + s.Z2(i, jp1)) : 0.5f * dzp;
+    DD = SPH ? hq_at<IN>(L, s, i, j) + 0.5f * (s.Z2(i, j) + s.Z2(i, jp1)) : 0.5f * dzp;
     DF = DD;
   } else {
     return 0.0f;
@@ -430,7 +429,7 @@ __device__ float nl_y(const DevLayer &L, const A &s, int i, int j) {
   float XQ;
   if (SPH) {
     const float r4 = cf(L, kF4, j);
-    if (L.modscm == 0 && i < IX) {
+    if (L.modscm == 0 && (IN || i < IX)) {
       XQ = q - r4 * DD * (s.Z2(i, jp1) - s.Z2(i, j)) -
            r4 * DD / 12.0f *
                ((s.Z2(ip1, jp1) - 2.0f * s.Z2(i, jp1) + s.Z2(im1, jp1)) -
@@ -442,21 +441,21 @@ __device__ float nl_y(const DevLayer &L, const A &s, int i, int j) {
     XQ = XQ - 4.0f * cf(L, kR5, j) * xpp;
   } else {
     XQ = q - L.gry * DD * (s.Z2(i, jp1) - s.Z2(i, j));
-    if (L.modscm == 0 && i < IX) {
+    if (L.modscm == 0 && (IN || i < IX)) {
       float scm = L.gry * DD / 12.0f *
                   ((s.Z2(ip1, jp1) - 2.0f * s.Z2(i, jp1) + s.Z2(im1, jp1)) -
                    (s.Z2(ip1, j) - 2.0f * s.Z2(i, j) + s.Z2(im1, j)));
       XQ = XQ - scm;
     }
   }
-  if (L.linchk == 1 && dq1_at(L, s, i, j) >= kGx &&
-      (i > kSpan && i < IX - kSpan && j > kSpan && j < JY - kSpan)) {
+  if (L.linchk == 1 && dq1_at<IN>(L, s, i, j) >= kGx &&
+      (IN || (i > kSpan && i < IX - kSpan && j > kSpan && j < JY - kSpan))) {
     const float fy = SPH ? cf(L, kR0, j) : L.ry;
     const float fx = SPH ? cf(L, kR22, j) : L.rx;
-    const float d = dq1_at(L, s, i, j);
+    const float d = dq1_at<IN>(L, s, i, j);
     const bool qn = q < 0.0f;
     const int ja = qn ? jp1 : jm1;
-    const float da = dq1_at(L, s, i, ja);
+    const float da = dq1_at<IN>(L, s, i, ja);
     const bool degy = qn ? (da < kGx || s.DZ(i, jp1) < kGx) : (da < kGx || s.DZ(i, j) < kGx);
     const float ay = q * q / d;
     const float qa = s.N(i, ja);
@@ -464,15 +463,14 @@ __device__ float nl_y(const DevLayer &L, const A &s, int i, int j) {
     float advy = fy * (degy ? (qn ? -ay : ay) : (qn ? by - ay : ay - by));
     const bool pn = xpp < 0.0f;
     const int in = pn ? ip1 : im1, ifar = pn ? i : i - 2;
-    const float dn = dq1_at(L, s, pn ? ip1 : i - 1, j);
+    const float dn = dq1_at<IN>(L, s, pn ? ip1 : i - 1, j);
     const bool degx = pn ? (s.DZ(ip1, j) < kGx || s.DZ(ip1, jp1) < kGx || dn < kGx)
                          : (s.DZ(im1, j) < kGx || s.DZ(im1, jp1) < kGx || dn < kGx);
     const float ax = q * xpp / d;
     const float xpe = 0.25f * (s.M(in, j) + s.M(in, jp1) + s.M(ifar, j) + s.M(ifar, jp1));
-    const float bx = s.N(in, j) * xpe / dq1_at(L, s, in, j);
+    const float bx = s.N(in, j) * xpe / dq1_at<IN>(L, s, in, j);
     float advx = fx * (degx ? (pn ? -ax : ax) : (pn ? bx - ax : ax - bx));
-    wave_breaking(L, s, i, j, advx, advy);
-    XQ = XQ - advx - advy;
+    XQ = XQ - brk * advx - brk * advy;
   }
   if (L.fric) XQ = XQ / (1.0f + FF);
   if (fabsf(XQ) < kEps) XQ = 0.0f;
@@ -498,21 +496,23 @@ __device__ __forceinline__ int tile_of_block(int per_xcd) {
   return (b & 7) * per_xcd + (b >> 3);
 }
 
-// One TX x TY tile t of layer L: stage, continuity, momentum, outputs.
-// sm holds 6 (TX+4)(TY+4) planes; stab the pow/exp tables, staged by the
-// caller before the call (the first barrier below publishes them).
-template <int MODE, bool NL, bool SPH, int TX, int TY>
-__device__ __forceinline__ void tile_step(const DevLayer &L, int t, int tiles_x, float *sm,
+// One TX x TY tile of layer L (first cell ti0, tj0): stage, continuity,
+// momentum, outputs.  sm holds 6 (TX+4)(TY+4) planes; stab the pow/exp
+// tables, staged by the caller before the call (the first barrier below
+// publishes them).  IN: the tile and its halo lie more than SPAN+2 cells
+// inside the grid, so no index clamp, boundary ring, ghost row or range
+// limit of the Fortran loops can apply and all of them compile away.
+template <int MODE, bool NL, bool SPH, int TX, int TY, bool IN>
+__device__ __forceinline__ void tile_body(const DevLayer &L, int t, int ti0, int tj0, float *sm,
                                           const gcdet::Tables *stab) {
   constexpr int P = TX + 4, R = TY + 4, A = P * R;
-  const int ti0 = (t % tiles_x) * TX + 1, tj0 = (t / tiles_x) * TY + 1;
   Lds<P> s{sm, sm + A, sm + 2 * A, sm + 3 * A, sm + 4 * A, sm + 5 * A, ti0 - 2, tj0 - 2, stab};
   const int nx = L.nx, ny = L.ny, tid = threadIdx.x;
 
   // ---- stage inputs with a two-cell halo
   for (int idx = tid; idx < A; idx += kThreads) {
     const int gi = ti0 - 2 + idx % P, gj = tj0 - 2 + idx / P;
-    const bool in = gi >= 1 && gi <= nx && gj >= 1 && gj <= ny;
+    const bool in = IN || (gi >= 1 && gi <= nx && gj >= 1 && gj <= ny);
     const size_t g = in ? (size_t)(gj - 1) * nx + (gi - 1) : 0;
     const float h = in ? L.h[g] : 0.0f;
     s.h[idx] = h;
@@ -528,8 +528,9 @@ __device__ __forceinline__ void tile_step(const DevLayer &L, int t, int tiles_x,
           if (L.dzp) {
             dz = L.dzp[g];
           } else {
-            const bool massed = L.is_root ? (gi >= 2 && gi <= nx - 1 && gj >= 2 && gj <= ny - 1)
-                                          : (gi >= 2 && gj >= 2);
+            const bool massed =
+                IN || (L.is_root ? (gi >= 2 && gi <= nx - 1 && gj >= 2 && gj <= ny - 1)
+                                 : (gi >= 2 && gj >= 2));
             dz = massed ? z2 + h : 0.0f;
           }
         }
@@ -550,9 +551,9 @@ __device__ __forceinline__ void tile_step(const DevLayer &L, int t, int tiles_x,
     const int cj0 = MODE == kFused ? tj0 - 1 : tj0;
     for (int idx = tid; idx < CX * CY; idx += kThreads) {
       const int i = ci0 + idx % CX, j = cj0 + idx / CX;
-      if (i < 1 || i > nx || j < 1 || j > ny) continue;
+      if (!IN && (i < 1 || i > nx || j < 1 || j > ny)) continue;
       float z2, dz2;
-      mass_cell<NL, SPH>(L, s, i, j, z2, dz2);
+      mass_cell<NL, SPH, IN>(L, s, i, j, z2, dz2);
       s.z2[s.o(i, j)] = z2;
       if (NL) s.dz[s.o(i, j)] = dz2;
     }
@@ -568,11 +569,11 @@ __device__ __forceinline__ void tile_step(const DevLayer &L, int t, int tiles_x,
 #pragma unroll 1
   for (int idx = tid; idx < TX * TY; idx += kThreads) {
     const int i = ti0 + idx % TX, j = tj0 + idx / TX;
-    if (i > nx || j > ny) continue;
+    if (!IN && (i > nx || j > ny)) continue;
     const size_t g = (size_t)(j - 1) * nx + (i - 1);
     if (MODE != kMomentOnly) {
       float zo;
-      if (L.is_root || (i >= 2 && j >= 2)) zo = s.Z2(i, j);
+      if (IN || L.is_root || (i >= 2 && j >= 2)) zo = s.Z2(i, j);
       else if (i == 1 && j == 1) zo = s.Z2(2, 2);  // UPDATE ghosts, all_grids.f90:323-324
       else if (i == 1) zo = s.Z2(2, j);
       else zo = s.Z2(i, 2);
@@ -586,19 +587,22 @@ __device__ __forceinline__ void tile_step(const DevLayer &L, int t, int tiles_x,
       // 0 into level 1; writing 0 keeps the swapped buffers identical to it.
       if (NL) {
         // CONMOME(_S) ranges (moment.f90:377-387, 583-593, 873-883, 1072-1082)
-        const bool mx = r ? (i <= nx - 1 && j <= ny - 1) : (i >= 2 && i <= nx - 1 && j >= 2);
-        const bool my = r ? (i <= nx - 1 && j <= ny - 1) : (i >= 2 && j >= 2 && j <= ny - 1);
-        L.m2[g] = mx ? nl_x<SPH>(L, s, i, j) : 0.0f;
-        L.n2[g] = my ? nl_y<SPH>(L, s, i, j) : 0.0f;
+        const bool mx =
+            IN || (r ? (i <= nx - 1 && j <= ny - 1) : (i >= 2 && i <= nx - 1 && j >= 2));
+        const bool my =
+            IN || (r ? (i <= nx - 1 && j <= ny - 1) : (i >= 2 && j >= 2 && j <= ny - 1));
+        const float brk = L.linchk == 1 ? breaking_coef(L, s, i, j) : 1.0f;
+        L.m2[g] = mx ? nl_x<SPH, IN>(L, s, i, j, brk) : 0.0f;
+        L.n2[g] = my ? nl_y<SPH, IN>(L, s, i, j, brk) : 0.0f;
       } else {
         // MOMT_S / MOMT_C ranges (moment.f90:78-88, 117-119, 174-192, 233-235)
         const int lo = r ? 1 : 2;
-        const bool mx = i >= lo && i <= nx - 1 && j >= lo;
-        const bool my = j >= lo && j <= ny - 1 && i >= lo;
-        L.m2[g] = mx ? lin_x<SPH>(L, s, i, j) : 0.0f;
-        L.n2[g] = my ? lin_y<SPH>(L, s, i, j) : 0.0f;
+        const bool mx = IN || (i >= lo && i <= nx - 1 && j >= lo);
+        const bool my = IN || (j >= lo && j <= ny - 1 && i >= lo);
+        L.m2[g] = mx ? lin_x<SPH, IN>(L, s, i, j) : 0.0f;
+        L.n2[g] = my ? lin_y<SPH, IN>(L, s, i, j) : 0.0f;
       }
-      if (L.zmax && (r || (i >= 2 && j >= 2))) {
+      if (L.zmax && (IN || r || (i >= 2 && j >= 2))) {
         const float z2 = s.Z2(i, j);
         if (s.H(i, j) + z2 > kGx) {
           const float zm = L.zmax[g];
@@ -608,6 +612,16 @@ __device__ __forceinline__ void tile_step(const DevLayer &L, int t, int tiles_x,
       }
     }
   }
+}
+
+template <int MODE, bool NL, bool SPH, int TX, int TY>
+__device__ __forceinline__ void tile_step(const DevLayer &L, int t, int tiles_x, float *sm,
+                                          const gcdet::Tables *stab) {
+  const int ti0 = (t % tiles_x) * TX + 1, tj0 = (t / tiles_x) * TY + 1;
+  const bool interior = ti0 - 2 > kSpan + 2 && ti0 + TX + 2 < L.nx - kSpan &&
+                        tj0 - 2 > kSpan + 2 && tj0 + TY + 2 < L.ny - kSpan;
+  if (interior) tile_body<MODE, NL, SPH, TX, TY, true>(L, t, ti0, tj0, sm, stab);
+  else tile_body<MODE, NL, SPH, TX, TY, false>(L, t, ti0, tj0, sm, stab);
 }
 
 template <int MODE, bool NL, bool SPH, int TX, int TY>
@@ -689,7 +703,7 @@ __global__ void __launch_bounds__(kThreads)
     float z2 = 0.0f, dz2 = 0.0f;
     if (col_in && lane >= 1 && j >= 1 && j <= ny) {
       r.z1own = z1;
-      mass_cell<NL, SPH>(L, r, gi, j, z2, dz2);
+      mass_cell<NL, SPH, false>(L, r, gi, j, z2, dz2);
     }
     const int o = r.o5(gi, j);
     r.z2[o] = z2;
@@ -737,14 +751,15 @@ __global__ void __launch_bounds__(kThreads)
       if (NL) {
         const bool mx = rt ? (i <= nx - 1 && j <= ny - 1) : (i >= 2 && i <= nx - 1 && j >= 2);
         const bool my = rt ? (i <= nx - 1 && j <= ny - 1) : (i >= 2 && j >= 2 && j <= ny - 1);
-        L.m2[g] = mx ? nl_x<SPH>(L, r, i, j) : 0.0f;
-        L.n2[g] = my ? nl_y<SPH>(L, r, i, j) : 0.0f;
+        const float brk = L.linchk == 1 ? breaking_coef(L, r, i, j) : 1.0f;
+        L.m2[g] = mx ? nl_x<SPH, false>(L, r, i, j, brk) : 0.0f;
+        L.n2[g] = my ? nl_y<SPH, false>(L, r, i, j, brk) : 0.0f;
       } else {
         const int lo = rt ? 1 : 2;
         const bool mx = i >= lo && i <= nx - 1 && j >= lo;
         const bool my = j >= lo && j <= ny - 1 && i >= lo;
-        L.m2[g] = mx ? lin_x<SPH>(L, r, i, j) : 0.0f;
-        L.n2[g] = my ? lin_y<SPH>(L, r, i, j) : 0.0f;
+        L.m2[g] = mx ? lin_x<SPH, false>(L, r, i, j) : 0.0f;
+        L.n2[g] = my ? lin_y<SPH, false>(L, r, i, j) : 0.0f;
       }
       if (L.zmax && (rt || (i >= 2 && j >= 2))) {
         const float z2 = r.Z2(i, j);
@@ -985,16 +1000,16 @@ __global__ void k_fill_i32(int *p, int n, int v) {
   if (g < n) p[g] = v;
 }
 
-// Persistent step kernel of the batch engine: a grid sized to the chip
-// pulls (layer, tile) work items.  Items are split into 8 contiguous
-// queues, one per XCD (workgroup b runs on XCD b % 8), each drained
-// through an atomic head; a workgroup whose own queue is empty takes items
-// from the others, so uneven scenario sizes still balance.  A claim takes
-// a run of consecutive items sized for about three claims per workgroup
-// and queue: device-scope atomics on one address are serialised across
-// the XCDs, so one claim per tile would cost more than the tile.  The
-// next claim is issued before the current run is processed.  Every
-// workgroup leaves once all 8 heads have passed their ends.
+// Persistent step kernel of the batch engine: a grid no larger than the
+// chip holds at once pulls (layer, tile) work items.  Items are split into
+// 8 contiguous queues, one per XCD (workgroup b runs on XCD b % 8), each
+// drained through an atomic head; a workgroup whose own queue is empty
+// takes items from the others, so uneven scenario sizes still balance.
+// Device-scope atomics on one address are serialised across the XCDs, so
+// a claim takes a run of consecutive items (about three claims per
+// workgroup and queue) and a queue is probed with a plain load before any
+// claim on it.  Every workgroup leaves once all 8 heads have passed their
+// ends.
 template <int MODE, bool NL, bool SPH, int TX, int TY>
 __global__ void __launch_bounds__(kThreads)
     k_batch(const BatchDesc *__restrict__ descs, const BatchItem *__restrict__ items,
@@ -1004,19 +1019,21 @@ __global__ void __launch_bounds__(kThreads)
   __shared__ int s_item;
   if (NL || !SPH) stage_tables(stab, threadIdx.x);
   const int x = blockIdx.x & 7;
-  const int nbq = ((int)gridDim.x + 7 - x) / 8;  // workgroups on this XCD
   for (int c = 0; c < 8; ++c) {
     const int q = (x + c) & 7;
     const int lo = qstart[q], hi = qstart[q + 1];
     if (hi <= lo) continue;
+    const int nbq = ((int)gridDim.x + 7 - q) / 8;  // workgroups of that queue's XCD
     const int per = max(1, (hi - lo + 3 * nbq - 1) / (3 * nbq));
-    if (threadIdx.x == 0) s_item = lo + atomicAdd(&heads[q], per);
     for (;;) {
-      __syncthreads();  // s_item written; previous tile done with LDS
+      __syncthreads();  // previous run done with LDS and s_item
+      if (threadIdx.x == 0) {
+        const int seen = __hip_atomic_load(&heads[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_item = lo + seen >= hi ? hi : lo + atomicAdd(&heads[q], per);
+      }
+      __syncthreads();
       const int i0 = __builtin_amdgcn_readfirstlane(s_item);
-      __syncthreads();  // every lane has read s_item
       if (i0 >= hi) break;
-      if (threadIdx.x == 0) s_item = lo + atomicAdd(&heads[q], per);
       const int i1 = min(i0 + per, hi);
       for (int it = i0; it < i1; ++it) {
         if (it > i0) __syncthreads();  // previous tile done with LDS
@@ -1029,6 +1046,26 @@ __global__ void __launch_bounds__(kThreads)
     }
   }
 }
+
+// The same work as one workgroup per item (the hardware dispatcher does
+// the balancing): item (b % 8) * per_xcd + b / 8, so each XCD takes one
+// contiguous eighth of the items, as the queues split them.
+template <int MODE, bool NL, bool SPH, int TX, int TY>
+__global__ void __launch_bounds__(kThreads)
+    k_batch_grid(const BatchDesc *__restrict__ descs, const BatchItem *__restrict__ items,
+                 int nitems, int per_xcd) {
+  __shared__ float sm[6 * (TX + 4) * (TY + 4)];
+  __shared__ gcdet::Tables stab;
+  const int it = tile_of_block(per_xcd);
+  if (it >= nitems) return;
+  if (NL || !SPH) stage_tables(stab, threadIdx.x);
+  const BatchItem w = items[it];
+  const int slot = __builtin_amdgcn_readfirstlane(w.slot);
+  const int tile = __builtin_amdgcn_readfirstlane(w.tile);
+  const DevLayer L = descs[slot].L;
+  tile_step<MODE, NL, SPH, TX, TY>(L, tile, descs[slot].tiles_x, sm, &stab);
+}
+
 
 template <int MODE, bool NL, bool SPH, int TX, int TY>
 hipError_t launch_t(const DevLayer &L, hipStream_t st) {
@@ -1122,7 +1159,7 @@ int device_cus() {
 }
 
 template <int MODE, bool NL, bool SPH, int TX, int TY>
-hipError_t launch_batch_t(const BatchLaunch &B, hipStream_t st, bool prepare) {
+hipError_t launch_batch_t(const BatchLaunch &B, hipStream_t st, bool prepare, bool grid_form) {
   static int resident = 0;  // workgroups that fit on the chip at once
   if (!resident) {
     int per_cu = 0;
@@ -1132,8 +1169,16 @@ hipError_t launch_batch_t(const BatchLaunch &B, hipStream_t st, bool prepare) {
     resident = (per_cu > 0 ? per_cu : 1) * device_cus();
   }
   if (prepare) return hipSuccess;
+  if (grid_form) {
+    const int per = (B.nitems + 7) / 8;
+    hipLaunchKernelGGL((k_batch_grid<MODE, NL, SPH, TX, TY>), dim3(8 * per), dim3(kThreads), 0,
+                       st, B.descs, B.items, B.nitems, per);
+    return hipGetLastError();
+  }
+  // at least four items per workgroup: fewer, and the claims cost more
+  // than they balance
   int grid = resident;
-  const int want = (B.nitems + 7) / 8 * 8;
+  const int want = (B.nitems / 4 + 7) / 8 * 8;
   if (want < grid) grid = want;
   if (grid < 8) grid = 8;
   hipLaunchKernelGGL((k_batch<MODE, NL, SPH, TX, TY>), dim3(grid), dim3(kThreads), 0, st,
@@ -1143,31 +1188,38 @@ hipError_t launch_batch_t(const BatchLaunch &B, hipStream_t st, bool prepare) {
 
 template <int TX, int TY>
 hipError_t batch_shape_t(const BatchLaunch &B, int mode, bool nl, bool sph, hipStream_t st,
-                         bool prepare) {
+                         bool prepare, bool grid_form) {
 #define GC_CASE(MD, A, C) \
-  if (mode == MD && nl == A && sph == C) return launch_batch_t<MD, A, C, TX, TY>(B, st, prepare);
+  if (mode == MD && nl == A && sph == C) \
+    return launch_batch_t<MD, A, C, TX, TY>(B, st, prepare, grid_form);
   GC_CASE(kFused, false, true) GC_CASE(kFused, false, false)
   GC_CASE(kFused, true, true) GC_CASE(kFused, true, false)
 #undef GC_CASE
   return hipErrorInvalidValue;
 }
 
+// fused tile shapes; shape k < 3 runs the persistent kernel with tile k,
+// k >= 3 the one-workgroup-per-tile form with tile k - 3
 const int kBatchShapes[][2] = {{64, 8}, {32, 16}, {64, 16}};
+constexpr int kNumBatchTiles = 3;
 
 }  // namespace
 
-int batch_num_shapes() { return (int)(sizeof(kBatchShapes) / sizeof(kBatchShapes[0])); }
+int batch_num_shapes() { return 2 * kNumBatchTiles; }
 void batch_shape(int k, int *tx, int *ty) {
-  *tx = kBatchShapes[k][0];
-  *ty = kBatchShapes[k][1];
+  *tx = kBatchShapes[k % kNumBatchTiles][0];
+  *ty = kBatchShapes[k % kNumBatchTiles][1];
 }
 
 hipError_t launch_batch(const BatchLaunch &B, int mode, bool nl, bool sph, int shape,
                         hipStream_t st, bool prepare) {
   if (B.nitems <= 0 && !prepare) return hipSuccess;
+  if (shape < 0 || shape >= batch_num_shapes()) return hipErrorInvalidValue;
+  const bool grid_form = shape >= kNumBatchTiles;
   if (mode != kFused) {  // continuity / momentum halves of nested parents: 64 x 8
 #define GC_CASE(MD, A, C) \
-    if (mode == MD && nl == A && sph == C) return launch_batch_t<MD, A, C, 64, 8>(B, st, prepare);
+    if (mode == MD && nl == A && sph == C) \
+      return launch_batch_t<MD, A, C, 64, 8>(B, st, prepare, grid_form);
     GC_CASE(kMassOnly, false, true) GC_CASE(kMassOnly, false, false)
     GC_CASE(kMassOnly, true, true) GC_CASE(kMassOnly, true, false)
     GC_CASE(kMomentOnly, false, true) GC_CASE(kMomentOnly, false, false)
@@ -1175,10 +1227,10 @@ hipError_t launch_batch(const BatchLaunch &B, int mode, bool nl, bool sph, int s
 #undef GC_CASE
     return hipErrorInvalidValue;
   }
-  switch (shape) {
-    case 0: return batch_shape_t<64, 8>(B, mode, nl, sph, st, prepare);
-    case 1: return batch_shape_t<32, 16>(B, mode, nl, sph, st, prepare);
-    case 2: return batch_shape_t<64, 16>(B, mode, nl, sph, st, prepare);
+  switch (shape % kNumBatchTiles) {
+    case 0: return batch_shape_t<64, 8>(B, mode, nl, sph, st, prepare, grid_form);
+    case 1: return batch_shape_t<32, 16>(B, mode, nl, sph, st, prepare, grid_form);
+    case 2: return batch_shape_t<64, 16>(B, mode, nl, sph, st, prepare, grid_form);
     default: return hipErrorInvalidValue;
   }
 }

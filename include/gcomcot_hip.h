@@ -210,8 +210,10 @@ long long gcomcot_batch_device_bytes_(int *sid);
 /* device bytes one layer of nx x ny cells takes (admission control) */
 long long gcomcot_layer_footprint_(int *nx, int *ny, int *nonlinear);
 void gcomcot_batch_set_arrival_threshold_(float *thr);
-/* *autotune != 0: time the tile shapes on the live batch; *shape >= 0
- * forces that fused tile shape (0: 64x8, 1: 32x16, 2: 64x16) */
+/* *autotune != 0: time the launch forms and tile shapes on the live batch;
+ * *shape >= 0 forces one for the fused launches: 0/1/2 = persistent kernel
+ * with 64x8 / 32x16 / 64x16 tiles, 3/4/5 = one workgroup per tile with
+ * the same tiles */
 int gcomcot_batch_set_tuning_(int *autotune, int *shape);
 /* *on == 0: enqueue every root step instead of replaying HIP graphs (for
  * tools that cannot follow graph launches) */

@@ -99,7 +99,7 @@ int main(int argc, char **argv) {
   std::mt19937 rng(seed);
   auto U = [&](int a, int b) { return std::uniform_int_distribution<int>(a, b)(rng); };
   int zero = 0, on = 1;
-  int shape = U(-1, 2);
+  int shape = U(-1, 5);
   if (gcomcot_batch_reset_()) return 3;
   if (gcomcot_batch_set_tuning_(&on, &shape)) return 3;
   std::map<int, Case> live;

@@ -26,7 +26,8 @@ namespace {
   std::abort();
 }
 
-const int kShapes[][2] = {{64, 8}, {32, 16}, {64, 16}};  // kernels.hip kBatchShapes
+// kernels.hip kBatchShapes; shapes 3-5 are the same tiles in the other launch form
+const int kShapes[][2] = {{64, 8}, {32, 16}, {64, 16}};
 
 void tile_op(const DevLayer &d, int mode, int t, int tiles_x, int TX, int TY,
              std::vector<unsigned char> &cov) {
@@ -78,7 +79,7 @@ void run_batch(void *a) {
     if (B.heads[q] != 0) die("queue head not cleared before the launch");
   }
   int TX = 64, TY = 8;
-  if (x->mode == kFused) TX = kShapes[x->shape][0], TY = kShapes[x->shape][1];
+  if (x->mode == kFused) TX = kShapes[x->shape % 3][0], TY = kShapes[x->shape % 3][1];
   int nslot = 0;
   for (int it = 0; it < B.nitems; ++it) nslot = std::max(nslot, B.items[it].slot + 1);
   std::vector<std::vector<unsigned char>> cov(nslot);
@@ -187,12 +188,12 @@ hipError_t launch_jnz(const DevNest &N, hipStream_t st) {
   emu_submit(st, run_jnz, &a, sizeof(a));
   return hipSuccess;
 }
-int batch_num_shapes() { return 3; }
-void batch_shape(int k, int *tx, int *ty) { *tx = kShapes[k][0]; *ty = kShapes[k][1]; }
+int batch_num_shapes() { return 6; }
+void batch_shape(int k, int *tx, int *ty) { *tx = kShapes[k % 3][0]; *ty = kShapes[k % 3][1]; }
 hipError_t launch_batch(const BatchLaunch &B, int mode, bool, bool, int shape, hipStream_t st,
                         bool prepare) {
   if (prepare || B.nitems <= 0) return hipSuccess;
-  if (mode == kFused && (shape < 0 || shape >= 3)) return hipErrorInvalidValue;
+  if (shape < 0 || shape >= 6) return hipErrorInvalidValue;
   BatchArg a{B, mode, shape};
   emu_submit(st, run_batch, &a, sizeof(a));
   return hipSuccess;

@@ -56,7 +56,7 @@ def test_mixed_batch_matches_oracle():
     b.close()
 
 
-@pytest.mark.parametrize("shape", [0, 1, 2])
+@pytest.mark.parametrize("shape", [0, 1, 2, 3, 4, 5])
 def test_every_batch_tile_shape(shape):
     picks = [CASES[2], CASES[3], CASES[6], CASES[7]]
     print(f"[gpu] batch tile shape {shape}", flush=True)
