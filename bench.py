@@ -61,6 +61,8 @@ def parse():
                     help="steps of the bounded CPU-baseline sample (0 = skip)")
     ap.add_argument("--tile-variant", type=int, default=-1,
                     help="tile variant of the step kernel (-1: autotune)")
+    ap.add_argument("--others", type=int, default=1,
+                    help="also time the linear single layer and the two-level nest (0 = skip)")
     ap.add_argument("--batch", type=int, default=1, help="run the 64-scenario batch (0 = skip)")
     ap.add_argument("--batch-cases", type=int, default=64)
     ap.add_argument("--dropin-sample", type=int, default=100,
@@ -180,6 +182,30 @@ def bench_c1(a):
         traffic = tr.get("bytes_per_launch")
     return sc, cells, gcups, wall, kern_ms, achieved, traffic, arrived, ms_total, \
         (tx[variant], ty[variant])
+
+
+def bench_other_configs(a):
+    """The single-layer linear configuration (1024x768, 2000 steps) and the
+    two-level nest (2048x1536 linear parent + 1801x1801 nonlinear Cartesian
+    child, REL_SIZE 4, REL_TIME 4, 3000 steps) through the whole-run device
+    loop (HIP graph): GCUPS over all layers' cell updates."""
+    from gcomcot import DeviceRun, config0, config2
+    out = {}
+    for name, sc, bpc in (("config0", config0(), BYTES_PER_CELL["linear"]),
+                          ("config2", config2(), BYTES_PER_CELL["linear"])):
+        log(f"{name}: {sc.name}, {sc.steps} steps")
+        run = DeviceRun(sc)
+        run.run(10, graph=bool(a.graphs))
+        t0 = time.perf_counter()
+        run.run(sc.steps, graph=bool(a.graphs))
+        wall = time.perf_counter() - t0
+        cells = sc.cell_updates_per_step()
+        run.close()
+        gcups = cells * sc.steps / wall / 1e9
+        out[name] = {"workload": sc.name, "steps": sc.steps, "cell_updates_per_step": cells,
+                     "gcups": round(gcups, 3), "ms_per_step": round(wall * 1e3 / sc.steps, 5),
+                     "algorithmic_GBs": round(gcups * bpc, 1)}
+    return out
 
 
 def bench_batch(a):
@@ -311,6 +337,7 @@ def main():
         cpu = cpu_baseline(config1(nx=a.nx, ny=a.ny, steps=a.cpu_steps), a.cpu_steps)
         if a.accuracy:
             cpu["accuracy"] = cpu_accuracy()
+    others = bench_other_configs(a) if a.others else None
     batch = bench_batch(a) if a.batch else None
     serve = bench_serve(a) if a.serve else None
     copy = copy_probe(1024)
@@ -339,6 +366,7 @@ def main():
         "event_ms_total": round(ms_total, 3),
         "cells_arrived": arrived,
         "hbm_copy": copy,
+        "other_configs": others,
         "batch": batch,
         "serve": serve,
     }

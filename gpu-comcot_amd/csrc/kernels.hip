@@ -34,7 +34,12 @@ struct Lds {
   float *z1, *m1, *n1, *h, *z2, *dz;
   int gi0, gj0;  // global (1-based) index of LDS element 0
   const gcdet::Tables *tab;  // pow/exp tables, LDS copy
+  const float *crow;         // per-row coefficients of rows gj0.., LDS copy (kNumCoef x R)
+  int R;
   __device__ __forceinline__ int o(int i, int j) const { return (j - gj0) * P + (i - gi0); }
+  __device__ __forceinline__ float CF(const DevLayer &, int k, int j) const {
+    return crow[k * R + (j - gj0)];
+  }
   __device__ __forceinline__ float Z1(int i, int j) const { return z1[o(i, j)]; }
   __device__ __forceinline__ float Z2(int i, int j) const { return z2[o(i, j)]; }
   __device__ __forceinline__ float M(int i, int j) const { return m1[o(i, j)]; }
@@ -42,10 +47,6 @@ struct Lds {
   __device__ __forceinline__ float H(int i, int j) const { return h[o(i, j)]; }
   __device__ __forceinline__ float DZ(int i, int j) const { return dz[o(i, j)]; }
 };
-
-__device__ __forceinline__ float cf(const DevLayer &L, int k, int j) {
-  return L.coef[k * L.ny + (j - 1)];
-}
 
 // PQ_DEPTH (initialization.f90:2512-2523).  IN: the cell is known to lie
 // away from the grid edges, so no index clamp can apply.
@@ -151,8 +152,8 @@ __device__ __forceinline__ void mass_cell(const DevLayer &L, const A &s, int i, 
   const float h = s.H(i, j);
   float zz;
   if (SPH) {
-    const float r1 = cf(L, kR1, j), r11 = cf(L, kR11, j);
-    const float r6 = cf(L, kR6, j), r6m = cf(L, kR6, j - 1);
+    const float r1 = s.CF(L, kR1, j), r11 = s.CF(L, kR11, j);
+    const float r6 = s.CF(L, kR6, j), r6m = s.CF(L, kR6, j - 1);
     zz = s.Z1(i, j) - r1 * (s.M(i, j) - s.M(i - 1, j)) -
          r11 * (s.N(i, j) * r6 - s.N(i, j - 1) * r6m);
   } else {
@@ -189,8 +190,8 @@ __device__ float lin_x(const DevLayer &L, const A &s, int i, int j) {
     int jm1 = j - 1;
     if (!IN && jm1 <= 1) jm1 = 1;
     float tot_n = s.N(i, j) + s.N(i + 1, j) + s.N(i, jm1) + s.N(i + 1, jm1);
-    float r2 = L.r24 ? L.r24[gidx(L, i, j)] : cf(L, kF2, j) * fmaxf(hp_at<IN>(L, s, i, j), 0.0f);
-    float xm = s.M(i, j) - r2 * (s.Z2(i + 1, j) - s.Z2(i, j)) + cf(L, kR3, j) * tot_n;
+    float r2 = L.r24 ? L.r24[gidx(L, i, j)] : s.CF(L, kF2, j) * fmaxf(hp_at<IN>(L, s, i, j), 0.0f);
+    float xm = s.M(i, j) - r2 * (s.Z2(i + 1, j) - s.Z2(i, j)) + s.CF(L, kR3, j) * tot_n;
     if (fabsf(xm) < kEps) xm = 0.0f;
     return xm;
   } else {
@@ -229,8 +230,8 @@ __device__ float lin_y(const DevLayer &L, const A &s, int i, int j) {
     if (!IN && im1 <= 1) im1 = 1;
     float tot_m = s.M(im1, j) + s.M(im1, j + 1) + s.M(i, j) + s.M(i, j + 1);
     float r4 = L.r24 ? L.r24[(size_t)L.nx * L.ny + gidx(L, i, j)]
-                     : cf(L, kF4, j) * fmaxf(hq_at<IN>(L, s, i, j), 0.0f);
-    float xn = s.N(i, j) - r4 * (s.Z2(i, j + 1) - s.Z2(i, j)) - cf(L, kR5, j) * tot_m;
+                     : s.CF(L, kF4, j) * fmaxf(hq_at<IN>(L, s, i, j), 0.0f);
+    float xn = s.N(i, j) - r4 * (s.Z2(i, j + 1) - s.Z2(i, j)) - s.CF(L, kR5, j) * tot_m;
     if (fabsf(xn) < kEps) xn = 0.0f;
     return xn;
   } else {
@@ -339,7 +340,7 @@ __device__ float nl_x(const DevLayer &L, const A &s, int i, int j, float brk) {
   if (L.fric) FF = L.kfric * sqrtf(p * p + xqq * xqq) / gcdet::powf_det(DF, kManExp, *s.tab);
   float XP;
   if (SPH) {
-    const float r2 = cf(L, kF2, j);
+    const float r2 = s.CF(L, kF2, j);
     if (L.modscm == 0 && (IN || j < JY)) {
       XP = p - r2 * DD * (s.Z2(ip1, j) - s.Z2(i, j)) -
            r2 * DD / 12.0f *
@@ -349,7 +350,7 @@ __device__ float nl_x(const DevLayer &L, const A &s, int i, int j, float brk) {
       XP = p - r2 * DD * (s.Z2(ip1, j) - s.Z2(i, j));
     }
     if (L.fric) XP = XP - FF * p;
-    XP = XP + 4.0f * cf(L, kR3, j) * xqq;
+    XP = XP + 4.0f * s.CF(L, kR3, j) * xqq;
   } else {
     XP = p - L.grx * DD * (s.Z2(ip1, j) - s.Z2(i, j));
     if (L.modscm == 0 && (IN || j < JY)) {
@@ -362,8 +363,8 @@ __device__ float nl_x(const DevLayer &L, const A &s, int i, int j, float brk) {
   }
   if (L.linchk == 1 && dp1_at<IN>(L, s, i, j) >= kGx &&
       (IN || (i > kSpan && i < IX - kSpan && j > kSpan && j < JY - kSpan))) {
-    const float fx = SPH ? cf(L, kR21, j) : L.rx;
-    const float fy = SPH ? cf(L, kR0, j) : L.ry;
+    const float fx = SPH ? s.CF(L, kR21, j) : L.rx;
+    const float fy = SPH ? s.CF(L, kR0, j) : L.ry;
     const float d = dp1_at<IN>(L, s, i, j);
     // Upwind terms without divergent branches: each Fortran case is one of
     // fx*(-A), fx*A, fx*(B-A), fx*(A-B) of the same two quotients.
@@ -428,7 +429,7 @@ __device__ float nl_y(const DevLayer &L, const A &s, int i, int j, float brk) {
   if (L.fric) FF = L.kfric * sqrtf(q * q + xpp * xpp) / gcdet::powf_det(DF, kManExp, *s.tab);
   float XQ;
   if (SPH) {
-    const float r4 = cf(L, kF4, j);
+    const float r4 = s.CF(L, kF4, j);
     if (L.modscm == 0 && (IN || i < IX)) {
       XQ = q - r4 * DD * (s.Z2(i, jp1) - s.Z2(i, j)) -
            r4 * DD / 12.0f *
@@ -438,7 +439,7 @@ __device__ float nl_y(const DevLayer &L, const A &s, int i, int j, float brk) {
       XQ = q - r4 * DD * (s.Z2(i, jp1) - s.Z2(i, j));
     }
     if (L.fric) XQ = XQ - FF * q;
-    XQ = XQ - 4.0f * cf(L, kR5, j) * xpp;
+    XQ = XQ - 4.0f * s.CF(L, kR5, j) * xpp;
   } else {
     XQ = q - L.gry * DD * (s.Z2(i, jp1) - s.Z2(i, j));
     if (L.modscm == 0 && (IN || i < IX)) {
@@ -450,8 +451,8 @@ __device__ float nl_y(const DevLayer &L, const A &s, int i, int j, float brk) {
   }
   if (L.linchk == 1 && dq1_at<IN>(L, s, i, j) >= kGx &&
       (IN || (i > kSpan && i < IX - kSpan && j > kSpan && j < JY - kSpan))) {
-    const float fy = SPH ? cf(L, kR0, j) : L.ry;
-    const float fx = SPH ? cf(L, kR22, j) : L.rx;
+    const float fy = SPH ? s.CF(L, kR0, j) : L.ry;
+    const float fx = SPH ? s.CF(L, kR22, j) : L.rx;
     const float d = dq1_at<IN>(L, s, i, j);
     const bool qn = q < 0.0f;
     const int ja = qn ? jp1 : jm1;
@@ -497,7 +498,8 @@ __device__ __forceinline__ int tile_of_block(int per_xcd) {
 }
 
 // One TX x TY tile of layer L (first cell ti0, tj0): stage, continuity,
-// momentum, outputs.  sm holds 6 (TX+4)(TY+4) planes; stab the pow/exp
+// momentum, outputs.  sm holds 6 (TX+4)(TY+4) planes and the per-row
+// coefficients of the TY+4 rows; stab the pow/exp
 // tables, staged by the caller before the call (the first barrier below
 // publishes them).  IN: the tile and its halo lie more than SPAN+2 cells
 // inside the grid, so no index clamp, boundary ring, ghost row or range
@@ -506,8 +508,18 @@ template <int MODE, bool NL, bool SPH, int TX, int TY, bool IN>
 __device__ __forceinline__ void tile_body(const DevLayer &L, int t, int ti0, int tj0, float *sm,
                                           const gcdet::Tables *stab) {
   constexpr int P = TX + 4, R = TY + 4, A = P * R;
-  Lds<P> s{sm, sm + A, sm + 2 * A, sm + 3 * A, sm + 4 * A, sm + 5 * A, ti0 - 2, tj0 - 2, stab};
+  Lds<P> s{sm,     sm + A,     sm + 2 * A, sm + 3 * A, sm + 4 * A, sm + 5 * A, ti0 - 2,
+           tj0 - 2, stab, sm + 6 * A, R};
   const int nx = L.nx, ny = L.ny, tid = threadIdx.x;
+
+  // ---- per-row coefficients of the tile's rows (spherical layers)
+  if (SPH) {
+    float *crow = sm + 6 * A;
+    for (int idx = tid; idx < kNumCoef * R; idx += kThreads) {
+      const int k = idx / R, j = tj0 - 2 + idx % R;
+      crow[idx] = (IN || (j >= 1 && j <= ny)) ? L.coef[k * ny + (j - 1)] : 0.0f;
+    }
+  }
 
   // ---- stage inputs with a two-cell halo
   for (int idx = tid; idx < A; idx += kThreads) {
@@ -627,7 +639,7 @@ __device__ __forceinline__ void tile_step(const DevLayer &L, int t, int tiles_x,
 template <int MODE, bool NL, bool SPH, int TX, int TY>
 __global__ void __launch_bounds__(kThreads)
     k_step(DevLayer L, int tiles_x, int ntiles, int per_xcd) {
-  __shared__ float sm[6 * (TX + 4) * (TY + 4)];
+  __shared__ float sm[6 * (TX + 4) * (TY + 4) + kNumCoef * (TY + 4)];
   __shared__ gcdet::Tables stab;
   const int t = tile_of_block(per_xcd);
   if (t >= ntiles) return;
@@ -663,6 +675,9 @@ struct Ring {
   __device__ __forceinline__ float DZ(int i, int j) const { return dz[o5(i, j)]; }
   // Z1 is only ever read at the thread's own cell (continuity, breaking)
   __device__ __forceinline__ float Z1(int, int) const { return z1own; }
+  __device__ __forceinline__ float CF(const DevLayer &L, int k, int j) const {
+    return L.coef[k * L.ny + (j - 1)];
+  }
 };
 
 template <bool NL, bool SPH>
@@ -1014,7 +1029,7 @@ template <int MODE, bool NL, bool SPH, int TX, int TY>
 __global__ void __launch_bounds__(kThreads)
     k_batch(const BatchDesc *__restrict__ descs, const BatchItem *__restrict__ items,
             const int *__restrict__ qstart, int *heads) {
-  __shared__ float sm[6 * (TX + 4) * (TY + 4)];
+  __shared__ float sm[6 * (TX + 4) * (TY + 4) + kNumCoef * (TY + 4)];
   __shared__ gcdet::Tables stab;
   __shared__ int s_item;
   if (NL || !SPH) stage_tables(stab, threadIdx.x);
@@ -1054,7 +1069,7 @@ template <int MODE, bool NL, bool SPH, int TX, int TY>
 __global__ void __launch_bounds__(kThreads)
     k_batch_grid(const BatchDesc *__restrict__ descs, const BatchItem *__restrict__ items,
                  int nitems, int per_xcd) {
-  __shared__ float sm[6 * (TX + 4) * (TY + 4)];
+  __shared__ float sm[6 * (TX + 4) * (TY + 4) + kNumCoef * (TY + 4)];
   __shared__ gcdet::Tables stab;
   const int it = tile_of_block(per_xcd);
   if (it >= nitems) return;
