@@ -46,6 +46,8 @@ int launch(const LayerRec &L, int mode, bool with_max, bool advance) {
 int create_layer(int id, int parent, int level, int laycord, int laygov, int rel_size,
                  int rel_time, const int *corners, int fric_switch, float fric_coef, float dt,
                  float dx, float dy, int nx, int ny, const float *H, const float *Z) {
+  if (int e = tree_validate_layer(g_tree, id, parent, rel_size, rel_time, corners, nx, ny))
+    return e;
   if (int e = ensure_init()) return e;
   return tree_create_layer(g_tree, g_stream, id, parent, level, laycord, laygov, rel_size,
                            rel_time, corners, fric_switch, fric_coef, dt, dx, dy, nx, ny, H, Z);
@@ -104,15 +106,14 @@ int gcomcot_init_layer_ex_(int *layerid, int *parent, int *level, int *laycord, 
   if (*fric_switch != 0 && *fric_switch != 1)
     return fail(12, "FRIC_SWITCH %d not supported (0 constant Manning, 1 none)", *fric_switch);
   if (*dt <= 0.0f || *dx <= 0.0f || *dy <= 0.0f) return fail(13, "DT/DX/DY must be positive");
+  if (*laycord == 0 && !coef)
+    return fail(14, "spherical layer %d needs the PARA row table", *layerid);
   if (int e = create_layer(*layerid, *parent, *level, *laycord, *laygov, *rel_size, *rel_time,
                            corners, *fric_switch, *fric_coef, *dt, *dx, *dy, *nx, *ny, H, Z))
     return e;
   LayerRec &L = g_tree.layers[*layerid];
   std::vector<float> c((size_t)kNumCoef * *ny, 0.0f);
-  if (*laycord == 0) {
-    if (!coef) return fail(14, "spherical layer %d needs the PARA row table", *layerid);
-    std::memcpy(c.data(), coef, c.size() * sizeof(float));
-  }
+  if (*laycord == 0) std::memcpy(c.data(), coef, c.size() * sizeof(float));
   return tree_upload_coef(L, c);
 }
 
@@ -233,10 +234,10 @@ void cuda_shutdown_(void) {
 
 // ------------------------------------------------------------- extended
 int gcomcot_run_steps_(int *nsteps, int *use_graph) {
-  if (int e = ensure_init()) return e;
-  if (!layer(1)) return fail(8, "no root layer");
   int n = *nsteps;
   if (n < 0) return fail(23, "negative step count");
+  if (!layer(1)) return fail(8, "no root layer");
+  if (int e = ensure_init()) return e;
   if (*use_graph && n >= 2 && tree_launches_per_step(g_tree) <= kGraphMaxLaunches) {
     // Two root steps bring every layer's level parity back to where it
     // started (children with odd REL_TIME flip once per root step), so a
@@ -262,10 +263,10 @@ int gcomcot_run_steps_(int *nsteps, int *use_graph) {
 }
 
 int gcomcot_time_steps_(int *nsteps, float *ms_total, float *ms_each) {
-  if (int e = ensure_init()) return e;
-  if (!layer(1)) return fail(8, "no root layer");
   const int n = *nsteps;
   if (n < 1) return fail(23, "step count must be >= 1");
+  if (!layer(1)) return fail(8, "no root layer");
+  if (int e = ensure_init()) return e;
   std::vector<hipEvent_t> ev(n + 1);
   for (auto &e : ev) HIPCHK(hipEventCreate(&e));
   HIPCHK(hipEventRecord(ev[0], g_stream));

@@ -530,7 +530,6 @@ int gcomcot_batch_reset_(void) {
 }
 
 int gcomcot_batch_new_scenario_(long long *steps, int *sid) {
-  if (int e = eng_init()) return e;
   if (*steps < 0) return fail(30, "batch: negative step count");
   auto s = std::make_unique<Scenario>();
   s->sid = g_eng.next_sid++;
@@ -545,7 +544,6 @@ int gcomcot_batch_add_layer_(int *sid, int *layerid, int *parent, int *laycord, 
                              int *rel_size, int *rel_time, int *corners, int *fric_switch,
                              float *fric_coef, float *dt, float *dx, float *dy, float *coef,
                              float *H, float *Z, int *nx, int *ny) {
-  if (int e = eng_init()) return e;
   Scenario *s = scenario(*sid);
   if (!s) return fail(31, "batch: no scenario %d", *sid);
   if (*layerid != 1 && *parent != 1)
@@ -557,6 +555,10 @@ int gcomcot_batch_add_layer_(int *sid, int *layerid, int *parent, int *laycord, 
     return fail(12, "FRIC_SWITCH %d not supported", *fric_switch);
   if (*dt <= 0.0f || *dx <= 0.0f || *dy <= 0.0f) return fail(13, "DT/DX/DY must be positive");
   if (*laycord == 0 && !coef) return fail(14, "spherical layer needs the PARA row table");
+  if (int e = tree_validate_layer(s->tree, *layerid, *layerid == 1 ? -1 : *parent, *rel_size,
+                                  *rel_time, corners, *nx, *ny))
+    return e;
+  if (int e = eng_init()) return e;
   invalidate();
   const int level = *layerid == 1 ? 1 : 2;
   if (int e = tree_create_layer(s->tree, g_eng.st, *layerid, *layerid == 1 ? -1 : *parent,

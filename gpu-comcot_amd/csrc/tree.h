@@ -72,6 +72,9 @@ size_t layer_bytes(int nx, int ny, bool nl, bool r24);
 // reference's SUBGRID_MATCHING does (initialization.f90:882-1131), derives
 // the ALLOC scalars (comcot.f90:537-552, 602), allocates and clears the
 // device planes, uploads H and Z.  A new root frees the old tree.
+// The argument checks of tree_create_layer alone (no device work).
+int tree_validate_layer(Tree &T, int id, int parent, int rel_size, int rel_time,
+                        const int *corners, int nx, int ny);
 int tree_create_layer(Tree &T, hipStream_t st, int id, int parent, int level, int laycord,
                       int laygov, int rel_size, int rel_time, const int *corners,
                       int fric_switch, float fric_coef, float dt, float dx, float dy, int nx,

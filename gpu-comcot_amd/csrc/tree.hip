@@ -97,10 +97,8 @@ void tree_free(Tree &T) {
   T.ctl_in = 0;
 }
 
-int tree_create_layer(Tree &T, hipStream_t st, int id, int parent, int level, int laycord,
-                      int laygov, int rel_size, int rel_time, const int *corners,
-                      int fric_switch, float fric_coef, float dt, float dx, float dy, int nx,
-                      int ny, const float *H, const float *Z) {
+int tree_validate_layer(Tree &T, int id, int parent, int rel_size, int rel_time,
+                        const int *corners, int nx, int ny) {
   if (id < 1 || id >= kMaxLayers) return fail(1, "invalid layer id %d (1..%d)", id, kMaxLayers - 1);
   if (nx < 3 || ny < 3) return fail(2, "layer %d: grid %dx%d too small", id, nx, ny);
   if (id == 1 && parent != -1) return fail(3, "layer 1 must be the root (parent -1)");
@@ -116,6 +114,14 @@ int tree_create_layer(Tree &T, hipStream_t st, int id, int parent, int level, in
                   rel_size);
     if (rel_time < 1) return fail(7, "layer %d: REL_TIME %d < 1", id, rel_time);
   }
+  return 0;
+}
+
+int tree_create_layer(Tree &T, hipStream_t st, int id, int parent, int level, int laycord,
+                      int laygov, int rel_size, int rel_time, const int *corners,
+                      int fric_switch, float fric_coef, float dt, float dx, float dy, int nx,
+                      int ny, const float *H, const float *Z) {
+  if (int e = tree_validate_layer(T, id, parent, rel_size, rel_time, corners, nx, ny)) return e;
   if (!T.ctl) {
     HIPCHK(hipMalloc(&T.ctl, 2 * sizeof(int)));
     HIPCHK(hipMemsetAsync(T.ctl, 0, 2 * sizeof(int), st));
