@@ -156,6 +156,30 @@ GC_HD float powf_det(float x, float y, const TAB &T) {
   return (float)exp2_core((double)y * log2_pos(x, T), T);
 }
 
+// x^y for the depth terms of Manning friction, whose callers guarantee
+// x >= 1e-5 (the momentum routines clamp or test the depth first) and
+// y = 2.3333333: there t = y*log2(x) lies in [-39, 299], so none of the
+// range cases of exp2_core or powf_det can occur and the same arithmetic
+// runs without them.  Equal to powf_det(x, y) for every x >= 1e-5 and for
+// NaN and +inf (NaN passes, +inf and the largest floats give +inf).
+template <class TAB>
+GC_HD float powf_manning(float x, float y, const TAB &T) {
+  const double t = (double)y * log2_pos(x, T);
+  const double s = t * 32.0;
+  const int k = (int)(s + (s >= 0.0 ? 0.5 : -0.5));
+  const double g = t - (double)k * (1.0 / 32.0);
+  const int j = k & 31;
+  const int e2 = k >> 5;
+  double q = dfma(g, kE6, kE5);
+  q = dfma(g, q, kE4);
+  q = dfma(g, q, kE3);
+  q = dfma(g, q, kE2);
+  q = dfma(g, q, kE1);
+  q = dfma(g, q, 1.0);
+  const float r = (float)((T.exp2j[j] * q) * bits_to_double((uint64_t)(uint32_t)(e2 + 1023) << 52));
+  return x != x ? x : r;
+}
+
 // e^x
 template <class TAB>
 GC_HD float expf_det(float x, const TAB &T) {
@@ -166,6 +190,7 @@ GC_HD float expf_det(float x, const TAB &T) {
 // host entry points on the compile-time tables
 static inline float powf_det(float x, float y) { return powf_det(x, y, kTab); }
 static inline float expf_det(float x) { return expf_det(x, kTab); }
+static inline float powf_manning(float x, float y) { return powf_manning(x, y, kTab); }
 #endif
 
 }  // namespace gcdet

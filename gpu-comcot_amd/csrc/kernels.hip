@@ -204,7 +204,7 @@ __device__ float lin_x(const DevLayer &L, const A &s, int i, int j) {
     if (L.fric) {
       float xqq = 0.25f * (s.N(i, j) + s.N(i + 1, j) + s.N(i, jm1) + s.N(i + 1, jm1));
       if (hm >= 1.0e-3f)
-        ff = L.kfric * sqrtf(s.M(i, j) * s.M(i, j) + xqq * xqq) / gcdet::powf_det(hm, kManExp, *s.tab);
+        ff = L.kfric * sqrtf(s.M(i, j) * s.M(i, j) + xqq * xqq) / gcdet::powf_manning(hm, kManExp, *s.tab);
       else
         ff = 0.0f;
       xm = xm - ff * s.M(i, j);
@@ -244,7 +244,7 @@ __device__ float lin_y(const DevLayer &L, const A &s, int i, int j) {
     if (L.fric) {
       float xpp = 0.25f * (s.M(i, j) + s.M(i, j + 1) + s.M(im1, j) + s.M(im1, j + 1));
       if (hn >= 1.0e-5f)
-        ff = L.kfric * sqrtf(s.N(i, j) * s.N(i, j) + xpp * xpp) / gcdet::powf_det(hn, kManExp, *s.tab);
+        ff = L.kfric * sqrtf(s.N(i, j) * s.N(i, j) + xpp * xpp) / gcdet::powf_manning(hn, kManExp, *s.tab);
       else
         ff = 0.0f;
       xn = xn - ff * s.N(i, j);
@@ -337,7 +337,7 @@ __device__ float nl_x(const DevLayer &L, const A &s, int i, int j, float brk) {
   const float p = s.M(i, j);
   const float xqq = 0.25f * (s.N(i, j) + s.N(ip1, j) + s.N(i, jm1) + s.N(ip1, jm1));
   float FF = 0.0f;
-  if (L.fric) FF = L.kfric * sqrtf(p * p + xqq * xqq) / gcdet::powf_det(DF, kManExp, *s.tab);
+  if (L.fric) FF = L.kfric * sqrtf(p * p + xqq * xqq) / gcdet::powf_manning(DF, kManExp, *s.tab);
   float XP;
   if (SPH) {
     const float r2 = s.CF(L, kF2, j);
@@ -426,7 +426,7 @@ __device__ float nl_y(const DevLayer &L, const A &s, int i, int j, float brk) {
   const float q = s.N(i, j);
   const float xpp = 0.25f * (s.M(i, j) + s.M(i, jp1) + s.M(im1, j) + s.M(im1, jp1));
   float FF = 0.0f;
-  if (L.fric) FF = L.kfric * sqrtf(q * q + xpp * xpp) / gcdet::powf_det(DF, kManExp, *s.tab);
+  if (L.fric) FF = L.kfric * sqrtf(q * q + xpp * xpp) / gcdet::powf_manning(DF, kManExp, *s.tab);
   float XQ;
   if (SPH) {
     const float r4 = s.CF(L, kF4, j);

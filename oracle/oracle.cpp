@@ -1179,3 +1179,4 @@ double orc_cr_dt(int laycord, int laygov, int nx, int ny, const double *h, doubl
 // Test hooks: the scalar pow/exp of pow_mode 1, for checks against libm.
 extern "C" float orc_powf_det(float x, float y) { return gcdet::powf_det(x, y); }
 extern "C" float orc_expf_det(float x) { return gcdet::expf_det(x); }
+extern "C" float orc_powf_manning(float x, float y) { return gcdet::powf_manning(x, y); }

@@ -103,6 +103,25 @@ def test_det_pow_is_correctly_rounded(oracle_built):
     assert np.mean(lm != det) < 2e-3
 
 
+def test_manning_pow_equals_det_pow_on_its_domain(oracle_built):
+    """powf_manning (the branch-free form the kernels call) returns the bits
+    of powf_det for every depth the momentum routines can pass (>= 1e-5),
+    up to the largest float, and for +inf and NaN."""
+    L = Oracle.lib()
+    for f in (L.orc_powf_det, L.orc_powf_manning):
+        f.restype = ctypes.c_float
+        f.argtypes = [ctypes.c_float, ctypes.c_float]
+    rng = np.random.default_rng(11)
+    xs = np.exp(rng.uniform(np.log(1e-5), np.log(3.4e38), 40000)).astype(np.float32)
+    edges = np.array([1e-5, 1e-3, 1.0, 3.0e38, 3.0000002e38, 3.4028235e38, np.inf, np.nan],
+                     np.float32)
+    y = float(np.float32(2.3333333))
+    for x in np.concatenate([xs, edges]):
+        a = L.orc_powf_det(float(x), y)
+        b = L.orc_powf_manning(float(x), y)
+        assert bits(np.float32(a)) == bits(np.float32(b)) or (np.isnan(a) and np.isnan(b)), x
+
+
 def test_det_exp_matches_libm_on_edges(oracle_built):
     L = Oracle.lib()
     L.orc_expf_det.restype = ctypes.c_float
