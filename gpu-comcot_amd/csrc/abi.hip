@@ -238,6 +238,31 @@ int gcomcot_run_steps_(int *nsteps, int *use_graph) {
   if (n < 0) return fail(23, "negative step count");
   if (!layer(1)) return fail(8, "no root layer");
   if (int e = ensure_init()) return e;
+  if (tree_step2_eligible(g_tree) && n >= 2) {
+    // a linear single layer: two root steps per launch; a graph of two
+    // launches (four steps) brings the level parity back for replay
+    if (*use_graph && n >= 4) {
+      hipGraph_t graph = nullptr;
+      hipGraphExec_t exec = nullptr;
+      HIPCHK(hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal));
+      int r = tree_enqueue_root_steps2(g_tree, g_stream);
+      if (!r) r = tree_enqueue_root_steps2(g_tree, g_stream);
+      hipError_t ce = hipStreamEndCapture(g_stream, &graph);
+      if (r) return r;
+      HIPCHK(ce);
+      HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+      for (int k = 0; k + 3 < n; k += 4) HIPCHK(hipGraphLaunch(exec, g_stream));
+      HIPCHK(hipStreamSynchronize(g_stream));
+      (void)hipGraphExecDestroy(exec);
+      (void)hipGraphDestroy(graph);
+      n = n % 4;
+    }
+    for (; n >= 2; n -= 2)
+      if (int r = tree_enqueue_root_steps2(g_tree, g_stream)) return r;
+    for (; n > 0; --n)
+      if (int r = tree_enqueue_root_step(g_tree, g_stream)) return r;
+    return sync();
+  }
   if (*use_graph && n >= 2 && tree_launches_per_step(g_tree) <= kGraphMaxLaunches) {
     // Two root steps bring every layer's level parity back to where it
     // started (children with odd REL_TIME flip once per root step), so a
@@ -317,6 +342,12 @@ int gcomcot_last_error_(char *buf, int *len) {
     buf[*len - 1] = 0;
   }
   return take_last_code();
+}
+
+int gcomcot_set_temporal_blocking_(int *mode) {
+  if (*mode < 0 || *mode > kStep2Shapes) return fail(28, "temporal blocking mode %d out of range", *mode);
+  g_tree.blocking = *mode;
+  return 0;
 }
 
 int gcomcot_hbm_copy_(int *mib, int *reps, float *ms_per_copy) {

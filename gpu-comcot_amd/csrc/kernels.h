@@ -39,6 +39,13 @@ hipError_t launch_step(const DevLayer &L, int mode, bool nl, bool sph, int varia
 int num_tile_variants();
 void tile_variant_shape(int v, int *tx, int *ty);
 
+// Two root steps of a linear outermost layer without children in one
+// launch (temporal blocking); shape 0/1/2 = 64x8 / 32x16 / 64x16 tiles.
+// Reads level 1, writes level 2 (two steps later); with zmax set, the
+// wet maximum and arrival take both steps and the counter advances by 2.
+hipError_t launch_step2(const DevLayer &L, bool sph, int shape, hipStream_t st);
+constexpr int kStep2Shapes = 3;
+
 // Reference-semantics maximum (no wet test) plus the wet max/arrival.
 hipError_t launch_max(const DevLayer &L, float *zmax_raw, hipStream_t st);
 

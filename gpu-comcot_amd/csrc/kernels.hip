@@ -647,6 +647,143 @@ __global__ void __launch_bounds__(kThreads)
   tile_step<MODE, NL, SPH, TX, TY>(L, t, tiles_x, sm, &stab);
 }
 
+// ------------------------------------------------ two steps per launch
+// Temporal blocking for a linear outermost layer without nested layers:
+// one launch advances two root steps.  A tile stages Z, M, N, H with a
+// four-cell halo, computes the first step's Z (continuity + open boundary)
+// on the tile plus three cells and its M, N on the tile plus two cells,
+// all in LDS, then the second step on the tile from those, and writes
+// Z, M, N once.  Every cell value is produced by the same functions from
+// the same operands as in two single-step launches, so the result is
+// bitwise the same; HBM traffic and launches per step halve, at the cost
+// of recomputing the halo ring (cheap for the linear equations).  The wet
+// maximum and arrival see both steps in order.
+template <bool SPH, int TX, int TY, bool IN>
+__device__ __forceinline__ void tile_body2(const DevLayer &L, int t, int ti0, int tj0,
+                                           float *sm, const gcdet::Tables *stab) {
+  constexpr int HW = 4, P = TX + 2 * HW, R = TY + 2 * HW, A = P * R;
+  float *ph = sm, *pz0 = sm + A, *pm0 = sm + 2 * A, *pn0 = sm + 3 * A;
+  float *pz1 = sm + 4 * A, *pm1 = sm + 5 * A, *pn1 = sm + 6 * A, *crow = sm + 7 * A;
+  const int gi0 = ti0 - HW, gj0 = tj0 - HW;
+  // first step: level 1 = staged inputs, new Z into pz1; second step:
+  // level 1 = first step's planes, new Z into pz0 (dead after step one)
+  Lds<P> s1{pz0, pm0, pn0, ph, pz1, nullptr, gi0, gj0, stab, crow, R};
+  Lds<P> s2{pz1, pm1, pn1, ph, pz0, nullptr, gi0, gj0, stab, crow, R};
+  const int nx = L.nx, ny = L.ny, tid = threadIdx.x;
+  auto inside = [&](int i, int j) { return IN || (i >= 1 && i <= nx && j >= 1 && j <= ny); };
+
+  if (SPH) {
+    for (int idx = tid; idx < kNumCoef * R; idx += kThreads) {
+      const int k = idx / R, j = gj0 + idx % R;
+      crow[idx] = (IN || (j >= 1 && j <= ny)) ? L.coef[k * ny + (j - 1)] : 0.0f;
+    }
+  }
+  for (int idx = tid; idx < A; idx += kThreads) {
+    const int gi = gi0 + idx % P, gj = gj0 + idx / P;
+    const bool in = inside(gi, gj);
+    const size_t g = in ? (size_t)(gj - 1) * nx + (gi - 1) : 0;
+    ph[idx] = in ? L.h[g] : 0.0f;
+    pz0[idx] = in ? L.z1[g] : 0.0f;
+    pm0[idx] = in ? L.m1[g] : 0.0f;
+    pn0[idx] = in ? L.n1[g] : 0.0f;
+  }
+  __syncthreads();
+
+  // step one, continuity on the tile + 3
+  {
+    constexpr int CX = TX + 6, CY = TY + 6;
+    for (int idx = tid; idx < CX * CY; idx += kThreads) {
+      const int i = ti0 - 3 + idx % CX, j = tj0 - 3 + idx / CX;
+      float z2 = 0.0f, dz2;
+      if (inside(i, j)) mass_cell<false, SPH, IN>(L, s1, i, j, z2, dz2);
+      pz1[s1.o(i, j)] = z2;
+    }
+  }
+  __syncthreads();
+  // step one, momentum on the tile + 2 (0 outside the MOMT ranges and the grid)
+  {
+    constexpr int CX = TX + 4, CY = TY + 4;
+    for (int idx = tid; idx < CX * CY; idx += kThreads) {
+      const int i = ti0 - 2 + idx % CX, j = tj0 - 2 + idx / CX;
+      float m = 0.0f, n = 0.0f;
+      if (inside(i, j)) {
+        const bool mx = IN || (i <= nx - 1);
+        const bool my = IN || (j <= ny - 1);
+        if (mx) m = lin_x<SPH, IN>(L, s1, i, j);
+        if (my) n = lin_y<SPH, IN>(L, s1, i, j);
+      }
+      pm1[s1.o(i, j)] = m;
+      pn1[s1.o(i, j)] = n;
+    }
+  }
+  __syncthreads();
+  // step two, continuity on the tile + 1
+  {
+    constexpr int CX = TX + 2, CY = TY + 2;
+    for (int idx = tid; idx < CX * CY; idx += kThreads) {
+      const int i = ti0 - 1 + idx % CX, j = tj0 - 1 + idx / CX;
+      float z2 = 0.0f, dz2;
+      if (inside(i, j)) mass_cell<false, SPH, IN>(L, s2, i, j, z2, dz2);
+      pz0[s2.o(i, j)] = z2;
+    }
+  }
+  __syncthreads();
+
+  // step two, momentum and the outputs of the tile
+  int step = 0;
+  if (L.zmax) {
+    step = L.ctl[L.ctl_in] + 1;
+    if (L.advance && t == 0 && tid == 0) L.ctl[1 - L.ctl_in] = step + 1;
+  }
+#pragma unroll 1
+  for (int idx = tid; idx < TX * TY; idx += kThreads) {
+    const int i = ti0 + idx % TX, j = tj0 + idx / TX;
+    if (!IN && (i > nx || j > ny)) continue;
+    const size_t g = (size_t)(j - 1) * nx + (i - 1);
+    const float za = pz1[s1.o(i, j)], zb = pz0[s2.o(i, j)];
+    L.z2[g] = zb;
+    const bool mx = IN || (i <= nx - 1);
+    const bool my = IN || (j <= ny - 1);
+    L.m2[g] = mx ? lin_x<SPH, IN>(L, s2, i, j) : 0.0f;
+    L.n2[g] = my ? lin_y<SPH, IN>(L, s2, i, j) : 0.0f;
+    if (L.zmax) {
+      const float h = ph[s1.o(i, j)];
+      const bool wa = h + za > kGx, wb = h + zb > kGx;
+      if (wa || wb) {  // the maximum is only read and written where wet
+        const float z0 = L.zmax[g];
+        float zm = z0;
+        int arr = -2;
+        if (wa) {
+          if (za >= L.arr_thr && !(zm >= L.arr_thr)) arr = step;
+          if (za > zm) zm = za;
+        }
+        if (wb) {
+          if (zb >= L.arr_thr && !(zm >= L.arr_thr)) arr = step + 1;
+          if (zb > zm) zm = zb;
+        }
+        if (arr != -2) L.arrival[g] = arr;
+        if (zm != z0) L.zmax[g] = zm;
+      }
+    }
+  }
+}
+
+template <bool SPH, int TX, int TY>
+__global__ void __launch_bounds__(kThreads)
+    k_step2(DevLayer L, int tiles_x, int ntiles, int per_xcd) {
+  constexpr int HW = 4;
+  __shared__ float sm[7 * (TX + 2 * HW) * (TY + 2 * HW) + kNumCoef * (TY + 2 * HW)];
+  __shared__ gcdet::Tables stab;
+  const int t = tile_of_block(per_xcd);
+  if (t >= ntiles) return;
+  if (!SPH) stage_tables(stab, threadIdx.x);
+  const int ti0 = (t % tiles_x) * TX + 1, tj0 = (t / tiles_x) * TY + 1;
+  const bool interior = ti0 - HW > kSpan + 2 && ti0 + TX + HW < L.nx - kSpan &&
+                        tj0 - HW > kSpan + 2 && tj0 + TY + HW < L.ny - kSpan;
+  if (interior) tile_body2<SPH, TX, TY, true>(L, t, ti0, tj0, sm, &stab);
+  else tile_body2<SPH, TX, TY, false>(L, t, ti0, tj0, sm, &stab);
+}
+
 // ---------------------------------------------------------- strip kernel
 // The fused step as a strip marching along J: a workgroup owns 252
 // interior columns (256 lanes, two halo lanes each side) and a segment of
@@ -1105,6 +1242,15 @@ hipError_t launch_shape(const DevLayer &L, int mode, bool nl, bool sph, hipStrea
   return hipErrorInvalidValue;
 }
 
+template <bool SPH, int TX, int TY>
+hipError_t launch2_t(const DevLayer &L, hipStream_t st) {
+  const int tx = (L.nx + TX - 1) / TX, ty = (L.ny + TY - 1) / TY;
+  const int nt = tx * ty, per = (nt + 7) / 8;
+  hipLaunchKernelGGL((k_step2<SPH, TX, TY>), dim3(8 * per), dim3(kThreads), 0, st, L, tx, nt,
+                     per);
+  return hipGetLastError();
+}
+
 // Fused-step launch shapes: LDS tiles TX x TY (variants 0-5; 0 is the
 // default), then the strip kernel with SEG rows per segment (6-10).
 const int kVariants[][3] = {{1, 64, 8},   {1, 64, 16},  {1, 32, 32},  {1, 128, 8},
@@ -1136,6 +1282,15 @@ hipError_t launch_step(const DevLayer &L, int mode, bool nl, bool sph, int varia
     case 32032: return launch_shape<32, 32>(L, mode, nl, sph, st);
     case 64032: return launch_shape<64, 32>(L, mode, nl, sph, st);
     case 32016: return launch_shape<32, 16>(L, mode, nl, sph, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_step2(const DevLayer &L, bool sph, int shape, hipStream_t st) {
+  switch (shape) {
+    case 0: return sph ? launch2_t<true, 64, 8>(L, st) : launch2_t<false, 64, 8>(L, st);
+    case 1: return sph ? launch2_t<true, 32, 16>(L, st) : launch2_t<false, 32, 16>(L, st);
+    case 2: return sph ? launch2_t<true, 64, 16>(L, st) : launch2_t<false, 64, 16>(L, st);
     default: return hipErrorInvalidValue;
   }
 }

@@ -45,6 +45,7 @@ struct Tree {
   int ctl_in = 0;      // which counter holds the completed step count
   float arr_thr = 0.01f;
   int variant = 0;     // tile variant of the step kernels
+  int blocking = 1;    // two steps per launch for linear single layers: 0 off, 1+shape
   LayerRec *layer(int id) {
     return (id >= 1 && id < kMaxLayers && layers[id].used) ? &layers[id] : nullptr;
   }
@@ -93,6 +94,10 @@ int tree_launch(const Tree &T, const LayerRec &L, int mode, bool with_max, bool 
 int tree_enqueue_children(Tree &T, LayerRec &P, bool with_max, hipStream_t st);
 // One complete root step: mass+open, nests, momentum + max, rotation.
 int tree_enqueue_root_step(Tree &T, hipStream_t st);
+// True when the tree is one linear layer (two steps per launch apply).
+bool tree_step2_eligible(const Tree &T);
+// Two root steps of such a tree in one launch (temporal blocking).
+int tree_enqueue_root_steps2(Tree &T, hipStream_t st);
 // Kernel launches one root step of the tree enqueues.
 long long tree_launches_per_step(Tree &T);
 

@@ -344,6 +344,22 @@ int tree_enqueue_root_step(Tree &T, hipStream_t st) {
   return 0;
 }
 
+bool tree_step2_eligible(const Tree &T) {
+  const LayerRec &R = T.layers[1];
+  return T.blocking > 0 && R.used && R.children.empty() && !R.nl;
+}
+
+int tree_enqueue_root_steps2(Tree &T, hipStream_t st) {
+  LayerRec *R = T.layer(1);
+  if (!R) return fail(8, "no root layer");
+  DevLayer d = dev_view(T, *R, true, true);
+  hipError_t e = launch_step2(d, R->sph != 0, T.blocking - 1, st);
+  if (e != hipSuccess) return fail(200 + (int)e, "two-step launch failed: %s", hipGetErrorString(e));
+  R->cur = 1 - R->cur;
+  T.ctl_in = 1 - T.ctl_in;
+  return 0;
+}
+
 static long long launches_below(Tree &T, const LayerRec &P) {
   long long n = 0;
   for (int cid : P.children) {

@@ -163,6 +163,14 @@ int gcomcot_last_error_(char *buf, int *len);
 int gcomcot_set_tile_variant_(int *variant);
 int gcomcot_tile_variants_(int *count, int *tx, int *ty);
 
+/* Two root steps per launch (temporal blocking) in gcomcot_run_steps_ for
+ * a linear outermost layer without nested layers: *mode 0 = off, 1/2/3 =
+ * on with 64x8 / 32x16 / 64x16 tiles (default 1).  Results are bitwise
+ * those of single steps; afterwards level 2 holds the state two steps
+ * before level 1 when the last launch was a two-step one.  Returns 0 if
+ * valid. */
+int gcomcot_set_temporal_blocking_(int *mode);
+
 /* Measurement aid: *reps copies of a *mib MiB buffer with a plain
  * streaming kernel; *ms_per_copy = mean duration (HIP events).  Gives the
  * chip's practical HBM bandwidth and the scale of rocprofv3's memory

@@ -175,6 +175,11 @@ hipError_t launch_step(const DevLayer &L, int mode, bool, bool, int, hipStream_t
   emu_submit(st, run_step, &a, sizeof(a));
   return hipSuccess;
 }
+hipError_t launch_step2(const DevLayer &L, bool, int, hipStream_t st) {
+  StepArg a{L, kFused};
+  emu_submit(st, run_step, &a, sizeof(a));  // the planner tests never take this path
+  return hipSuccess;
+}
 int num_tile_variants() { return 1; }
 void tile_variant_shape(int, int *tx, int *ty) { *tx = 64; *ty = 8; }
 hipError_t launch_max(const DevLayer &, float *, hipStream_t) { return hipSuccess; }

@@ -110,3 +110,29 @@ def test_config1_reduced_matches_oracle():
     dev.run(300)
     _compare(sc, dev, orc, "config1-256")
     dev.close()
+
+
+LINEAR = [CASES[0], CASES[1],
+          ("ragged-linear-77x61", S.custom(laycord=0, laygov=0, nx=77, ny=61, cx=0.45), 90)]
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2, 3], ids=["single", "2step-64x8", "2step-32x16",
+                                                    "2step-64x16"])
+def test_linear_layers_one_or_two_steps_per_launch(mode):
+    """Linear single layers run two root steps per launch by default; every
+    tile shape of that kernel and the single-step kernel must give the
+    oracle's bits, for odd and even step counts, with and without graphs."""
+    from gcomcot._lib import ip, lib
+    lib().gcomcot_set_temporal_blocking_(ip(mode))
+    try:
+        for name, sc, steps in LINEAR:
+            for n, graph in ((steps + 1, True), (steps, False)):
+                print(f"[gpu] {name} blocking={mode} steps={n} graph={graph}", flush=True)
+                orc = Oracle(sc, 32, pow_mode=1)
+                orc.step(n)
+                dev = DeviceRun(sc)
+                dev.run(n, graph=graph)
+                _compare(sc, dev, orc, f"{name} mode {mode}")
+                dev.close()
+    finally:
+        lib().gcomcot_set_temporal_blocking_(ip(1))
