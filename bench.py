@@ -112,25 +112,6 @@ def cpu_baseline(sc, nsteps):
                       f"{dt:.1f} s"}
 
 
-def autotune_variant(run, steps=6):
-    """Time every tile variant of the fused step on the live layer (the
-    device loop, a few steps each) and keep the fastest."""
-    import ctypes
-    from gcomcot._lib import ip, lib
-    n = ctypes.c_int(0)
-    lib().gcomcot_tile_variants_(ctypes.byref(n), None, None)
-    best, best_ms = 0, float("inf")
-    for v in range(n.value):
-        lib().gcomcot_set_tile_variant_(ip(v))
-        run.run(2, graph=False)
-        _, each = run.timed(steps)
-        ms = float(np.median(each))
-        if ms < best_ms:
-            best, best_ms = v, ms
-    lib().gcomcot_set_tile_variant_(ip(best))
-    return best
-
-
 def traffic_entry(workload_key):
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
@@ -152,10 +133,7 @@ def bench_c1(a):
         lib().gcomcot_set_tile_variant_(ip(a.tile_variant))
         variant = a.tile_variant
     else:
-        variant = autotune_variant(run)
-    run.close()
-    run = DeviceRun(sc)  # fresh state after tuning
-    lib().gcomcot_set_tile_variant_(ip(variant))
+        variant = run.autotune()[0]  # state-neutral library autotuner
     run.run(a.warmup, graph=False)
     lib().gcomcot_run_steps_(ip(0), ip(0))  # sync
     log(f"config1: timing {a.steps} steps (tile variant {variant})")
@@ -195,6 +173,7 @@ def bench_other_configs(a):
                           ("config2", config2(), BYTES_PER_CELL["linear"])):
         log(f"{name}: {sc.name}, {sc.steps} steps")
         run = DeviceRun(sc)
+        tuned = run.autotune() if len(sc.layers) == 1 else None
         run.run(10, graph=bool(a.graphs))
         t0 = time.perf_counter()
         run.run(sc.steps, graph=bool(a.graphs))
@@ -203,6 +182,8 @@ def bench_other_configs(a):
         run.close()
         gcups = cells * sc.steps / wall / 1e9
         out[name] = {"workload": sc.name, "steps": sc.steps, "cell_updates_per_step": cells,
+                     "tuned": None if tuned is None else
+                     {"tile_variant": tuned[0], "two_step_shape": tuned[1]},
                      "gcups": round(gcups, 3), "ms_per_step": round(wall * 1e3 / sc.steps, 5),
                      "algorithmic_GBs": round(gcups * bpc, 1)}
     return out

@@ -344,6 +344,58 @@ int gcomcot_last_error_(char *buf, int *len) {
   return take_last_code();
 }
 
+int gcomcot_autotune_(int *reps, int *variant, int *blocking, float *ms_per_step) {
+  LayerRec *R = layer(1);
+  if (!R) return fail(8, "no root layer");
+  if (*reps < 1) return fail(29, "autotune: reps must be >= 1");
+  if (int e = ensure_init()) return e;
+  // dry launches (no maximum, no counter advance) read level 1 and write
+  // level 2 only, which the next real step rewrites: the state is untouched
+  DevLayer d = dev_view(g_tree, *R, false, false);
+  const bool fused_only = R->children.empty();
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  float best = 1e30f;
+  int bv = g_tree.variant, bb = 0;
+  auto time_it = [&](auto launch, int steps_per_launch, float *out) -> int {
+    HIPCHK(launch());
+    HIPCHK(hipEventRecord(e0, g_stream));
+    for (int k = 0; k < *reps; ++k) HIPCHK(launch());
+    HIPCHK(hipEventRecord(e1, g_stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    *out = ms / (*reps * steps_per_launch);
+    return 0;
+  };
+  if (fused_only) {
+    for (int v = 0; v < num_tile_variants(); ++v) {
+      float ms;
+      if (int r = time_it([&] { return launch_step(d, kFused, R->nl != 0, R->sph != 0, v, g_stream); },
+                          1, &ms))
+        return r;
+      if (ms < best) { best = ms; bv = v; bb = 0; }
+    }
+    if (!R->nl) {
+      for (int sh = 0; sh < kStep2Shapes; ++sh) {
+        float ms;
+        if (int r = time_it([&] { return launch_step2(d, R->sph != 0, sh, g_stream); }, 2, &ms))
+          return r;
+        if (ms < best) { best = ms; bb = sh + 1; }
+      }
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  g_tree.variant = bv;
+  g_tree.blocking = bb;
+  if (variant) *variant = bv;
+  if (blocking) *blocking = bb;
+  if (ms_per_step) *ms_per_step = best;
+  return 0;
+}
+
 int gcomcot_set_temporal_blocking_(int *mode) {
   if (*mode < 0 || *mode > kStep2Shapes) return fail(28, "temporal blocking mode %d out of range", *mode);
   g_tree.blocking = *mode;

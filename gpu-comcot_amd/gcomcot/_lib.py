@@ -40,6 +40,7 @@ def lib():
         L.gcomcot_tile_variants_.restype = ctypes.c_int
         L.gcomcot_hbm_copy_.restype = ctypes.c_int
         L.gcomcot_set_temporal_blocking_.restype = ctypes.c_int
+        L.gcomcot_autotune_.restype = ctypes.c_int
         _lib = L
     return _lib
 

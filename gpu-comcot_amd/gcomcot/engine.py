@@ -88,6 +88,14 @@ class DeviceRun:
                                           fp(s.dx), fp(s.dy), cptr, hp, zp, ip(s.nx), ip(s.ny))
             check(rc, f"init layer {s.id}")
 
+    def autotune(self, reps=5):
+        """Pick the fastest step kernel shape for this layer (library autotuner);
+        returns (tile variant, two-step shape or 0, ms per root step)."""
+        v, b, ms = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_float(0)
+        check(lib().gcomcot_autotune_(ip(reps), ctypes.byref(v), ctypes.byref(b),
+                                      ctypes.byref(ms)), "autotune")
+        return v.value, b.value, ms.value
+
     # ------------------------------------------------------------ running
     def run(self, steps, graph=True):
         rc = lib().gcomcot_run_steps_(ip(steps), ip(1 if graph else 0))

@@ -163,6 +163,15 @@ int gcomcot_last_error_(char *buf, int *len);
 int gcomcot_set_tile_variant_(int *variant);
 int gcomcot_tile_variants_(int *count, int *tx, int *ty);
 
+/* Autotuner: for an outermost layer without nested layers, time every
+ * tile variant of the single-step kernel and, for linear layers, every
+ * shape of the two-step kernel, *reps launches each, on the live layer
+ * with the maximum and the step counter switched off (such launches read
+ * level 1 and write only level 2, so the state is unchanged); keep the
+ * fastest for gcomcot_run_steps_ / gcomcot_time_steps_.  Reports the
+ * choice and its time per root step.  Returns 0 or an error code. */
+int gcomcot_autotune_(int *reps, int *variant, int *blocking, float *ms_per_step);
+
 /* Two root steps per launch (temporal blocking) in gcomcot_run_steps_ for
  * a linear outermost layer without nested layers: *mode 0 = off, 1/2/3 =
  * on with 64x8 / 32x16 / 64x16 tiles (default 1).  Results are bitwise

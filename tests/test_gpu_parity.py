@@ -136,3 +136,27 @@ def test_linear_layers_one_or_two_steps_per_launch(mode):
                 dev.close()
     finally:
         lib().gcomcot_set_temporal_blocking_(ip(1))
+
+
+@pytest.mark.parametrize("name,sc,steps", [CASES[0], CASES[2], CASES[4]],
+                         ids=[CASES[0][0], CASES[2][0], CASES[4][0]])
+def test_autotune_mid_run_keeps_the_state(name, sc, steps):
+    """The library autotuner times dry launches on the live layer: a run
+    tuned half way must end exactly where the oracle does."""
+    from gcomcot._lib import ip, lib
+    print(f"[gpu] autotune mid-run: {name}", flush=True)
+    orc = Oracle(sc, 32, pow_mode=1)
+    orc.step(77)
+    dev = DeviceRun(sc)
+    try:
+        dev.run(37)
+        v, b, ms = dev.autotune(reps=2)
+        assert ms > 0 and v >= 0 and 0 <= b <= 3
+        if sc.layers[0].laygov != 0:
+            assert b == 0  # two steps per launch are for linear layers only
+        dev.run(40)
+        _compare(sc, dev, orc, f"{name} tuned")
+    finally:
+        dev.close()
+        lib().gcomcot_set_tile_variant_(ip(0))
+        lib().gcomcot_set_temporal_blocking_(ip(1))
