@@ -36,15 +36,7 @@ struct Lds {
   const gcdet::Tables *tab;  // pow/exp tables, LDS copy
   const float *crow;         // per-row coefficients of rows gj0.., LDS copy (kNumCoef x R)
   int R;
-  // advection quotients of every cell (nonlinear momentum): M^2/DP1,
-  // M*X/DP1, N^2/DQ1, N*Y/DQ1, planes of P*R (see advection_quotients)
-  const float *adv = nullptr;
-  static constexpr bool kPre = true;
   __device__ __forceinline__ int o(int i, int j) const { return (j - gj0) * P + (i - gi0); }
-  __device__ __forceinline__ float AX(int i, int j) const { return adv[o(i, j)]; }
-  __device__ __forceinline__ float AY(int i, int j) const { return adv[P * R + o(i, j)]; }
-  __device__ __forceinline__ float QY(int i, int j) const { return adv[2 * P * R + o(i, j)]; }
-  __device__ __forceinline__ float QX(int i, int j) const { return adv[3 * P * R + o(i, j)]; }
   __device__ __forceinline__ float CF(const DevLayer &, int k, int j) const {
     return crow[k * R + (j - gj0)];
   }
@@ -380,31 +372,17 @@ __device__ float nl_x(const DevLayer &L, const A &s, int i, int j, float brk) {
     const int ia = pn ? ip1 : im1;
     const float da = dp1_at<IN>(L, s, ia, j);
     const bool degx = pn ? (da < kGx || s.DZ(ip1, j) < kGx) : (da < kGx || s.DZ(i, j) < kGx);
-    // upwind quotients; the neighbour's are that cell's own quotients
-    // (same operands in the same order), read from the per-cell planes
-    float ax, bx;
-    if constexpr (A::kPre) {
-      ax = s.AX(i, j);
-      bx = s.AX(ia, j);
-    } else {
-      ax = p * p / d;
-      const float pa = s.M(ia, j);
-      bx = pa * pa / da;
-    }
+    const float ax = p * p / d;
+    const float pa = s.M(ia, j);
+    const float bx = pa * pa / da;
     const float advx = fx * (degx ? (pn ? -ax : ax) : (pn ? bx - ax : ax - bx));
     const bool qn = xqq < 0.0f;
     const int jn = qn ? jp1 : jm1, jf = qn ? j : j - 2;
     const float dn = dp1_at<IN>(L, s, i, jn);
     const bool degy = s.DZ(i, jn) < kGx || s.DZ(ip1, jn) < kGx || dn < kGx;
-    float ay, by;
-    if constexpr (A::kPre) {
-      ay = s.AY(i, j);
-      by = s.AY(i, jn);
-    } else {
-      ay = p * xqq / d;
-      const float xqe = 0.25f * (s.N(i, jn) + s.N(ip1, jn) + s.N(i, jf) + s.N(ip1, jf));
-      by = s.M(i, jn) * xqe / dn;
-    }
+    const float ay = p * xqq / d;
+    const float xqe = 0.25f * (s.N(i, jn) + s.N(ip1, jn) + s.N(i, jf) + s.N(ip1, jf));
+    const float by = s.M(i, jn) * xqe / dn;
     float advy = fy * (degy ? (qn ? -ay : ay) : (qn ? by - ay : ay - by));
     XP = XP - brk * advx - brk * advy;
   }
@@ -480,30 +458,18 @@ __device__ float nl_y(const DevLayer &L, const A &s, int i, int j, float brk) {
     const int ja = qn ? jp1 : jm1;
     const float da = dq1_at<IN>(L, s, i, ja);
     const bool degy = qn ? (da < kGx || s.DZ(i, jp1) < kGx) : (da < kGx || s.DZ(i, j) < kGx);
-    float ay, by;
-    if constexpr (A::kPre) {
-      ay = s.QY(i, j);
-      by = s.QY(i, ja);
-    } else {
-      ay = q * q / d;
-      const float qa = s.N(i, ja);
-      by = qa * qa / da;
-    }
+    const float ay = q * q / d;
+    const float qa = s.N(i, ja);
+    const float by = qa * qa / da;
     float advy = fy * (degy ? (qn ? -ay : ay) : (qn ? by - ay : ay - by));
     const bool pn = xpp < 0.0f;
     const int in = pn ? ip1 : im1, ifar = pn ? i : i - 2;
     const float dn = dq1_at<IN>(L, s, pn ? ip1 : i - 1, j);
     const bool degx = pn ? (s.DZ(ip1, j) < kGx || s.DZ(ip1, jp1) < kGx || dn < kGx)
                          : (s.DZ(im1, j) < kGx || s.DZ(im1, jp1) < kGx || dn < kGx);
-    float ax, bx;
-    if constexpr (A::kPre) {
-      ax = s.QX(i, j);
-      bx = s.QX(in, j);
-    } else {
-      ax = q * xpp / d;
-      const float xpe = 0.25f * (s.M(in, j) + s.M(in, jp1) + s.M(ifar, j) + s.M(ifar, jp1));
-      bx = s.N(in, j) * xpe / dq1_at<IN>(L, s, in, j);
-    }
+    const float ax = q * xpp / d;
+    const float xpe = 0.25f * (s.M(in, j) + s.M(in, jp1) + s.M(ifar, j) + s.M(ifar, jp1));
+    const float bx = s.N(in, j) * xpe / dq1_at<IN>(L, s, in, j);
     float advx = fx * (degx ? (pn ? -ax : ax) : (pn ? bx - ax : ax - bx));
     XQ = XQ - brk * advx - brk * advy;
   }
@@ -513,34 +479,6 @@ __device__ float nl_y(const DevLayer &L, const A &s, int i, int j, float brk) {
   if (XQ > lim) XQ = lim;
   if (XQ < -lim) XQ = -lim;
   return XQ;
-}
-
-// The four upwind quotients of cell (i, j) that CONMOME(_S)'s convection
-// terms divide (moment.f90:996-1045, 1185-1237 and their Cartesian twins):
-// X: M^2/DP1 and M*X/DP1 with X the cell's averaged N; Y: N^2/DQ1 and N*Y/DQ1
-// with Y its averaged M.  nl_x / nl_y take the cell's own and its upwind
-// neighbour's values from these planes; inside the convection range no
-// index clamp applies, so the neighbour's operands are exactly the ones the
-// Fortran forms for it.  Where a depth is 0 the quotient is inf or NaN and
-// is never selected.
-template <bool IN, class A>
-__device__ __forceinline__ void advection_quotients(const DevLayer &L, const A &s, int i, int j,
-                                                    float *adv, int plane) {
-  const int IX = L.nx, JY = L.ny;
-  int ip1 = i + 1, im1 = i - 1, jp1 = j + 1, jm1 = j - 1;
-  if (!IN && im1 <= 1) im1 = 1;
-  if (!IN && ip1 >= IX) ip1 = IX;
-  if (!IN && jm1 <= 1) jm1 = 1;
-  if (!IN && jp1 >= JY) jp1 = JY;
-  const float p = s.M(i, j), q = s.N(i, j);
-  const float xqq = 0.25f * (s.N(i, j) + s.N(ip1, j) + s.N(i, jm1) + s.N(ip1, jm1));
-  const float xpp = 0.25f * (s.M(i, j) + s.M(i, jp1) + s.M(im1, j) + s.M(im1, jp1));
-  const float d = dp1_at<IN>(L, s, i, j), e = dq1_at<IN>(L, s, i, j);
-  const int o = s.o(i, j);
-  adv[o] = p * p / d;
-  adv[plane + o] = p * xqq / d;
-  adv[2 * plane + o] = q * q / e;
-  adv[3 * plane + o] = q * xpp / e;
 }
 
 // Copy the pow/exp tables into LDS (read before the first __syncthreads).
@@ -572,8 +510,6 @@ __device__ __forceinline__ void tile_body(const DevLayer &L, int t, int ti0, int
   constexpr int P = TX + 4, R = TY + 4, A = P * R;
   Lds<P> s{sm,     sm + A,     sm + 2 * A, sm + 3 * A, sm + 4 * A, sm + 5 * A, ti0 - 2,
            tj0 - 2, stab, sm + 6 * A, R};
-  float *adv = sm + 6 * A + kNumCoef * R;  // 4 planes, nonlinear momentum only
-  s.adv = adv;
   const int nx = L.nx, ny = L.ny, tid = threadIdx.x;
 
   // ---- per-row coefficients of the tile's rows (spherical layers)
@@ -632,17 +568,6 @@ __device__ __forceinline__ void tile_body(const DevLayer &L, int t, int ti0, int
       mass_cell<NL, SPH, IN>(L, s, i, j, z2, dz2);
       s.z2[s.o(i, j)] = z2;
       if (NL) s.dz[s.o(i, j)] = dz2;
-    }
-    __syncthreads();
-  }
-
-  // ---- advection quotients of the tile + 1 (nonlinear momentum)
-  if (NL && MODE != kMassOnly) {
-    constexpr int QX = TX + 2, QY = TY + 2;
-    for (int idx = tid; idx < QX * QY; idx += kThreads) {
-      const int i = ti0 - 1 + idx % QX, j = tj0 - 1 + idx / QX;
-      if (!IN && (i < 1 || i > nx || j < 1 || j > ny)) continue;
-      advection_quotients<IN>(L, s, i, j, adv, A);
     }
     __syncthreads();
   }
@@ -714,7 +639,7 @@ __device__ __forceinline__ void tile_step(const DevLayer &L, int t, int tiles_x,
 template <int MODE, bool NL, bool SPH, int TX, int TY>
 __global__ void __launch_bounds__(kThreads)
     k_step(DevLayer L, int tiles_x, int ntiles, int per_xcd) {
-  __shared__ float sm[(NL ? 10 : 6) * (TX + 4) * (TY + 4) + kNumCoef * (TY + 4)];
+  __shared__ float sm[6 * (TX + 4) * (TY + 4) + kNumCoef * (TY + 4)];
   __shared__ gcdet::Tables stab;
   const int t = tile_of_block(per_xcd);
   if (t >= ntiles) return;
@@ -890,11 +815,6 @@ struct Ring {
   __device__ __forceinline__ float CF(const DevLayer &L, int k, int j) const {
     return L.coef[k * L.ny + (j - 1)];
   }
-  static constexpr bool kPre = false;  // advection quotients computed in place
-  __device__ __forceinline__ float AX(int, int) const { return 0.0f; }
-  __device__ __forceinline__ float AY(int, int) const { return 0.0f; }
-  __device__ __forceinline__ float QY(int, int) const { return 0.0f; }
-  __device__ __forceinline__ float QX(int, int) const { return 0.0f; }
 };
 
 template <bool NL, bool SPH>
@@ -1246,7 +1166,7 @@ template <int MODE, bool NL, bool SPH, int TX, int TY>
 __global__ void __launch_bounds__(kThreads)
     k_batch(const BatchDesc *__restrict__ descs, const BatchItem *__restrict__ items,
             const int *__restrict__ qstart, int *heads) {
-  __shared__ float sm[(NL ? 10 : 6) * (TX + 4) * (TY + 4) + kNumCoef * (TY + 4)];
+  __shared__ float sm[6 * (TX + 4) * (TY + 4) + kNumCoef * (TY + 4)];
   __shared__ gcdet::Tables stab;
   __shared__ int s_item;
   if (NL || !SPH) stage_tables(stab, threadIdx.x);
@@ -1286,7 +1206,7 @@ template <int MODE, bool NL, bool SPH, int TX, int TY>
 __global__ void __launch_bounds__(kThreads)
     k_batch_grid(const BatchDesc *__restrict__ descs, const BatchItem *__restrict__ items,
                  int nitems, int per_xcd) {
-  __shared__ float sm[(NL ? 10 : 6) * (TX + 4) * (TY + 4) + kNumCoef * (TY + 4)];
+  __shared__ float sm[6 * (TX + 4) * (TY + 4) + kNumCoef * (TY + 4)];
   __shared__ gcdet::Tables stab;
   const int it = tile_of_block(per_xcd);
   if (it >= nitems) return;
