@@ -396,6 +396,41 @@ int gcomcot_autotune_(int *reps, int *variant, int *blocking, float *ms_per_step
   return 0;
 }
 
+int gcomcot_set_layer_coords_(int *lid, const float *x, const float *y) {
+  LayerRec *L = layer(*lid);
+  if (!L) return fail(41, "gcomcot_set_layer_coords_: no layer %d", *lid);
+  g_tree.xs[*lid].assign(x, x + L->nx);
+  g_tree.ys[*lid].assign(y, y + L->ny);
+  return 0;
+}
+
+int gcomcot_gauges_set_(int *n, const float *x0, const float *y0, const int *lid, int *capacity) {
+  if (int e = ensure_init()) return e;
+  return tree_set_gauges(g_tree, g_stream, *n, x0, y0, lid, *capacity);
+}
+
+int gcomcot_gauges_read_(float *series, int *max_steps, int *nout) {
+  *nout = 0;
+  if (g_tree.ngauges == 0) return 0;
+  if (sync()) return 1;
+  int done = 0;
+  HIPCHK(hipMemcpy(&done, g_tree.ctl + g_tree.ctl_in, sizeof(int), hipMemcpyDeviceToHost));
+  const int cap = g_tree.gauge_cap, n = g_tree.ngauges;
+  long long recorded = done - g_tree.gauge_step0;
+  long long count = recorded;
+  if (count > cap) count = cap;
+  if (count > *max_steps) count = *max_steps;
+  if (count <= 0) return 0;
+  std::vector<float> ring((size_t)n * cap);
+  HIPCHK(hipMemcpy(ring.data(), g_tree.ring, ring.size() * sizeof(float), hipMemcpyDeviceToHost));
+  const long long first = done - count + 1;  // oldest step kept
+  for (int g = 0; g < n; ++g)
+    for (long long k = 0; k < count; ++k)
+      series[(size_t)g * *max_steps + k] = ring[(size_t)g * cap + (size_t)((first + k - 1) % cap)];
+  *nout = (int)count;
+  return 0;
+}
+
 int gcomcot_set_temporal_blocking_(int *mode) {
   if (*mode < 0 || *mode > kStep2Shapes) return fail(28, "temporal blocking mode %d out of range", *mode);
   g_tree.blocking = *mode;

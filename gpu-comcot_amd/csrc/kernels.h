@@ -97,6 +97,20 @@ hipError_t launch_jnq_multi(const DevNest *nests, const int *start, int nnest, i
 hipError_t launch_jnz_multi(const DevNest *nests, const int *start, int nnest, int nblocks,
                             hipStream_t st);
 hipError_t launch_fill_i32(int *p, int n, int v, hipStream_t st);
+// ---- gauges: bilinear samples of the elevation after each root step
+struct GaugeDev {
+  int lid, ki, kj, nx;   // layer, 1-based lower-left stencil cell, layer width
+  float cx, cy, tide;    // GET_TS weights and tide-level offset
+  int pad;
+};
+struct ZNow {
+  const float *z[20];    // current elevation plane of each layer id (index = id)
+};
+// Sample n gauges into ring[g * cap + (step - 1) % cap], step = *step_now
+// (the root steps completed), with GET_TS's arithmetic (output.f90:1085-1094).
+hipError_t launch_gauges(const GaugeDev *g, int n, const ZNow &z, const int *step_now,
+                         float *ring, int cap, hipStream_t st);
+
 // Streaming copy of bytes (a multiple of 16) from src to dst.
 hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st);
 

@@ -1147,6 +1147,26 @@ __global__ void __launch_bounds__(256) k_copy(const float4 *__restrict__ src,
     dst[i] = src[i];
 }
 
+// Gauge samples (GET_TS, output.f90:1085-1094): the 2x2 stencil of the
+// current elevation weighted in the Fortran's operation order.
+__global__ void k_gauges(const GaugeDev *__restrict__ gs, int n, ZNow zs,
+                         const int *__restrict__ step_now, float *ring, int cap) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const GaugeDev G = gs[k];
+  const float *z = zs.z[G.lid];
+  const size_t o = (size_t)(G.kj - 1) * G.nx + (G.ki - 1);
+  const float l1 = z[o], l2 = z[o + 1], l3 = z[o + G.nx], l4 = z[o + G.nx + 1];
+  const float z1 = l1 * (1.0f - G.cx) * (1.0f - G.cy);
+  const float z2 = l2 * G.cx * (1.0f - G.cy);
+  const float z3 = l3 * (1.0f - G.cx) * G.cy;
+  const float z4 = l4 * G.cx * G.cy;
+  float dat = z1 + z2 + z3 + z4;
+  if (fabsf(G.tide) > kGx) dat = dat + G.tide;
+  const int step = *step_now;
+  ring[(size_t)k * cap + (step - 1) % cap] = dat;
+}
+
 __global__ void k_fill_i32(int *p, int n, int v) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g < n) p[g] = v;
@@ -1430,6 +1450,13 @@ hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st)
   const size_t n = bytes / 16;
   hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, st, static_cast<const float4 *>(src),
                      static_cast<float4 *>(dst), n);
+  return hipGetLastError();
+}
+
+hipError_t launch_gauges(const GaugeDev *g, int n, const ZNow &z, const int *step_now,
+                         float *ring, int cap, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_gauges, dim3((n + 63) / 64), dim3(64), 0, st, g, n, z, step_now, ring, cap);
   return hipGetLastError();
 }
 

@@ -44,6 +44,13 @@ struct Tree {
   int *ctl = nullptr;  // device int[2]: root-step counters for the arrival field
   int ctl_in = 0;      // which counter holds the completed step count
   float arr_thr = 0.01f;
+  // grid coordinates LAYER%X / %Y per layer (gauge stencil search), host
+  std::vector<float> xs[kMaxLayers], ys[kMaxLayers];
+  // gauges sampled after every root step of the device loop
+  GaugeDev *gauges = nullptr;
+  float *ring = nullptr;
+  int ngauges = 0, gauge_cap = 0;
+  long long gauge_step0 = 0;  // root steps completed when the gauges were set
   int variant = 0;     // tile variant of the step kernels
   int blocking = 1;    // two steps per launch for linear single layers: 0 off, 1+shape
   LayerRec *layer(int id) {
@@ -98,6 +105,11 @@ int tree_enqueue_root_step(Tree &T, hipStream_t st);
 bool tree_step2_eligible(const Tree &T);
 // Two root steps of such a tree in one launch (temporal blocking).
 int tree_enqueue_root_steps2(Tree &T, hipStream_t st);
+// Gauges: stencil cell and weights of GET_TS (output.f90:1035-1155) for
+// (x0, y0) on layer lid from its coordinates; sampling launch per step.
+int tree_set_gauges(Tree &T, hipStream_t st, int n, const float *x0, const float *y0,
+                    const int *lid, int capacity);
+int tree_enqueue_gauges(const Tree &T, hipStream_t st);
 // Kernel launches one root step of the tree enqueues.
 long long tree_launches_per_step(Tree &T);
 

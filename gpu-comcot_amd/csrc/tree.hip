@@ -92,6 +92,13 @@ static void free_layer(LayerRec &L) {
 void tree_free(Tree &T) {
   for (auto &L : T.layers)
     if (L.used) free_layer(L);
+  if (T.gauges) (void)hipFree(T.gauges);
+  if (T.ring) (void)hipFree(T.ring);
+  T.gauges = nullptr;
+  T.ring = nullptr;
+  T.ngauges = 0;
+  for (auto &v : T.xs) v.clear();
+  for (auto &v : T.ys) v.clear();
   if (T.ctl) (void)hipFree(T.ctl);
   T.ctl = nullptr;
   T.ctl_in = 0;
@@ -130,6 +137,13 @@ int tree_create_layer(Tree &T, hipStream_t st, int id, int parent, int level, in
     if (id == 1) {  // a new root starts a new layer tree
       for (int k = 1; k < kMaxLayers; ++k)
         if (T.layers[k].used) free_layer(T.layers[k]);
+      if (T.gauges) (void)hipFree(T.gauges);
+      if (T.ring) (void)hipFree(T.ring);
+      T.gauges = nullptr;
+      T.ring = nullptr;
+      T.ngauges = 0;
+      for (auto &v : T.xs) v.clear();
+      for (auto &v : T.ys) v.clear();
     } else {
       if (LayerRec *op = T.layer(T.layers[id].parent)) {
         auto &ch = op->children;
@@ -341,12 +355,76 @@ int tree_enqueue_root_step(Tree &T, hipStream_t st) {
   }
   R->cur = 1 - R->cur;
   T.ctl_in = 1 - T.ctl_in;
-  return 0;
+  return tree_enqueue_gauges(T, st);
 }
 
 bool tree_step2_eligible(const Tree &T) {
   const LayerRec &R = T.layers[1];
-  return T.blocking > 0 && R.used && R.children.empty() && !R.nl;
+  // gauges sample every step, so they keep one step per launch
+  return T.blocking > 0 && R.used && R.children.empty() && !R.nl && T.ngauges == 0;
+}
+
+// GET_TS's stencil search over X(IS..IE) (output.f90:1062-1069, 1103-1110)
+static int gauge_cell(const std::vector<float> &X, int is, int ie, float x0) {
+  int k = 0;
+  for (int i = is; i <= ie - 1; ++i) {
+    if (x0 >= X[i - 1] && x0 < X[i]) k = i;
+    if (x0 < X[is - 1]) k = is;
+    if (x0 >= X[ie - 1]) k = ie - 1;
+  }
+  return k;
+}
+
+int tree_set_gauges(Tree &T, hipStream_t st, int n, const float *x0, const float *y0,
+                    const int *lid, int capacity) {
+  if (n < 0 || (n > 0 && capacity < 1)) return fail(40, "gauges: bad count or capacity");
+  std::vector<GaugeDev> g((size_t)n);
+  for (int k = 0; k < n; ++k) {
+    const LayerRec *L = T.layer(lid[k]);
+    if (!L) return fail(41, "gauge %d: no layer %d", k, lid[k]);
+    const auto &X = T.xs[lid[k]], &Y = T.ys[lid[k]];
+    if ((int)X.size() != L->nx || (int)Y.size() != L->ny)
+      return fail(42, "gauge %d: coordinates of layer %d not set", k, lid[k]);
+    const int is = lid[k] == 1 ? 1 : 2;  // inner layers skip the ghost row/column
+    const int ki = gauge_cell(X, is, L->nx, x0[k]), kj = gauge_cell(Y, is, L->ny, y0[k]);
+    if (ki < 1 || kj < 1) return fail(43, "gauge %d: position not comparable (NaN?)", k);
+    const float dxv = X[ki] - X[ki - 1], dyv = Y[kj] - Y[kj - 1];
+    g[k].lid = lid[k];
+    g[k].ki = ki;
+    g[k].kj = kj;
+    g[k].nx = L->nx;
+    g[k].cx = (x0[k] - X[ki - 1]) / dxv;
+    g[k].cy = (y0[k] - Y[kj - 1]) / dyv;
+    g[k].tide = 0.0f;  // the layers run at mean sea level (TIDE_LEVEL = 0)
+    g[k].pad = 0;
+  }
+  if (T.gauges) (void)hipFree(T.gauges);
+  if (T.ring) (void)hipFree(T.ring);
+  T.gauges = nullptr;
+  T.ring = nullptr;
+  T.ngauges = 0;
+  if (n == 0) return 0;
+  HIPCHK(hipMalloc(&T.gauges, (size_t)n * sizeof(GaugeDev)));
+  HIPCHK(hipMalloc(&T.ring, (size_t)n * capacity * sizeof(float)));
+  HIPCHK(hipMemcpy(T.gauges, g.data(), (size_t)n * sizeof(GaugeDev), hipMemcpyHostToDevice));
+  HIPCHK(hipMemsetAsync(T.ring, 0, (size_t)n * capacity * sizeof(float), st));
+  int done = 0;
+  HIPCHK(hipStreamSynchronize(st));
+  if (T.ctl) HIPCHK(hipMemcpy(&done, T.ctl + T.ctl_in, sizeof(int), hipMemcpyDeviceToHost));
+  T.gauge_step0 = done;
+  T.ngauges = n;
+  T.gauge_cap = capacity;
+  return 0;
+}
+
+int tree_enqueue_gauges(const Tree &T, hipStream_t st) {
+  if (T.ngauges == 0) return 0;
+  ZNow z{};
+  for (int id = 1; id < kMaxLayers; ++id)
+    if (T.layers[id].used) z.z[id] = T.layers[id].z[T.layers[id].cur];
+  hipError_t e = launch_gauges(T.gauges, T.ngauges, z, T.ctl + T.ctl_in, T.ring, T.gauge_cap, st);
+  if (e != hipSuccess) return fail(204, "gauge launch failed: %s", hipGetErrorString(e));
+  return 0;
 }
 
 int tree_enqueue_root_steps2(Tree &T, hipStream_t st) {

@@ -41,6 +41,8 @@ def lib():
         L.gcomcot_hbm_copy_.restype = ctypes.c_int
         L.gcomcot_set_temporal_blocking_.restype = ctypes.c_int
         L.gcomcot_autotune_.restype = ctypes.c_int
+        for name in ("gcomcot_set_layer_coords_", "gcomcot_gauges_set_", "gcomcot_gauges_read_"):
+            getattr(L, name).restype = ctypes.c_int
         _lib = L
     return _lib
 

@@ -123,6 +123,39 @@ class DeviceRun:
         L.cuda_update_layer_(ctypes.byref(one))
         _ok("drop-in step")
 
+    # ------------------------------------------------------------ gauges
+    def set_gauges(self, points, capacity):
+        """Gauges [(x0, y0, layer id), ...] sampled on the device after every
+        root step; each layer's coordinates are its LayerSpec lon/lat grid
+        (`coords`)."""
+        L = lib()
+        for lid in sorted({p[2] for p in points}):
+            x, y = self.coords(lid)
+            xa, xp = arr(x)
+            ya, yp = arr(y)
+            check(L.gcomcot_set_layer_coords_(ip(lid), xp, yp), f"coords of layer {lid}")
+        x0 = np.array([p[0] for p in points], np.float32)
+        y0 = np.array([p[1] for p in points], np.float32)
+        ids = np.array([p[2] for p in points], np.int32)
+        check(L.gcomcot_gauges_set_(ip(len(points)), x0.ctypes.data_as(ctypes.c_void_p),
+                                    y0.ctypes.data_as(ctypes.c_void_p),
+                                    ids.ctypes.data_as(ctypes.c_void_p), ip(capacity)),
+              "set gauges")
+        self.ngauges = len(points)
+
+    def gauges(self, max_steps):
+        out = np.zeros((self.ngauges, max_steps), np.float32)
+        n = ctypes.c_int(0)
+        check(lib().gcomcot_gauges_read_(out.ctypes.data_as(ctypes.c_void_p), ip(max_steps),
+                                         ctypes.byref(n)), "read gauges")
+        return out[:, :n.value]
+
+    def coords(self, lid):
+        """X (NX) and Y (NY) of a layer as float32: its longitudes/latitudes
+        (spherical) or metre offsets (Cartesian) from the scenario."""
+        s = self.specs[lid]
+        return scenarios.layer_coords(s)
+
     # ------------------------------------------------------------ state
     def state(self, lid=1):
         """Level-1 and level-2 planes of a layer, as (ny, nx) float32."""

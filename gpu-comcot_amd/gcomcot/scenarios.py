@@ -255,6 +255,20 @@ def config2(pnx=2048, pny=1536, dx=2.0, footprint=450, rel_size=4, rel_time=4, c
     return Scenario(f"config2-{pnx}x{pny}+{cnx}x{cny}", [P, C], steps, {"seed": seed})
 
 
+def layer_coords(s):
+    """Grid coordinates X(NX), Y(NY) of a layer (the LAYER%X / %Y that gauges
+    are located on), float32: spherical layers use degrees east of their
+    first column and their row latitudes; Cartesian layers metres from
+    their first cell."""
+    if s.laycord == 0:
+        x = (np.arange(s.nx, dtype=np.float64) * s.dx / 60.0).astype(np.float32)
+        y = np.asarray(s.y, np.float32)
+    else:
+        x = (np.arange(s.nx, dtype=np.float64) * s.dx).astype(np.float32)
+        y = (np.arange(s.ny, dtype=np.float64) * s.dy).astype(np.float32)
+    return x, y
+
+
 def child_dt(laycord, laygov, h, dx, dy, lat_first, lat_last, dt_parent):
     """CR_CHECK for a nested layer (initialization.f90:1206-1247): (REL_TIME, DT)."""
     ha, hp = arr(h)

@@ -163,6 +163,21 @@ int gcomcot_last_error_(char *buf, int *len);
 int gcomcot_set_tile_variant_(int *variant);
 int gcomcot_tile_variants_(int *count, int *tx, int *ty);
 
+/* Gauges on the device (reference: GET_TS, output.f90:1035-1155, which
+ * calls gcomcot_get_z_ and copies a 2x2 stencil to the host per gauge per
+ * step).  gcomcot_set_layer_coords_ gives a layer's LAYER%X(NX), %Y(NY);
+ * gcomcot_gauges_set_ places *n gauges at (x0[k], y0[k]) on layers lid[k]
+ * (stencil search and weights as GET_TS, inner layers from index 2) with a
+ * ring of *capacity samples each (*n = 0 removes them).  The device loop
+ * (gcomcot_run_steps_, gcomcot_time_steps_) then samples every gauge
+ * after every root step on the device, GET_TS's arithmetic bit for bit;
+ * gcomcot_gauges_read_ copies out the last min(steps, capacity,
+ * *max_steps) samples in time order, series[k * *max_steps + s], count in
+ * *nout.  With gauges set, linear layers run one step per launch. */
+int gcomcot_set_layer_coords_(int *lid, const float *x, const float *y);
+int gcomcot_gauges_set_(int *n, const float *x0, const float *y0, const int *lid, int *capacity);
+int gcomcot_gauges_read_(float *series, int *max_steps, int *nout);
+
 /* Autotuner: for an outermost layer without nested layers, time every
  * tile variant of the single-step kernel and, for linear layers, every
  * shape of the two-step kernel, *reps launches each, on the live layer

@@ -247,6 +247,19 @@ class Reference:
                     r=rpl.reshape(9, s.ny, s.nx), r6=r6, rx=sc[0], ry=sc[1], grx=sc[2],
                     gry=sc[3], dt=sc[4])
 
+    def set_coords(self, lid, x, y):
+        """LAYER%X(NX), %Y(NY) of a layer, the coordinates GET_TS searches."""
+        xa = np.ascontiguousarray(x, np.float32)
+        ya = np.ascontiguousarray(y, np.float32)
+        self.lib().ref_set_coords(_i(lid), _p(xa), _p(ya))
+
+    def get_ts(self, x0, y0, lid):
+        """The reference's gauge sample GET_TS (output.f90:1035-1155) of the
+        current elevation of layer lid at (x0, y0)."""
+        dat = ctypes.c_float(0)
+        self.lib().ref_get_ts(_f(x0), _f(y0), _i(lid), ctypes.byref(dat))
+        return np.float32(dat.value)
+
     def flux(self, lid=1):
         s = self.specs[lid]
         xf, yf = np.empty(2 * s.nx, np.float32), np.empty(2 * s.ny, np.float32)

@@ -327,4 +327,49 @@ subroutine ref_cr_check(dt_root, rel_time, dt_child) bind(C, name="ref_cr_check"
      dt_child(i) = la(i)%dt
   end do
 end subroutine ref_cr_check
+!.....grid coordinates X(NX), Y(NY) of a layer (what GET_TS searches)
+subroutine ref_set_coords(id, x, y) bind(C, name="ref_set_coords")
+  implicit none
+  integer :: id
+  real :: x(*), y(*)
+  integer :: k
+  call ref_pick(id, k)
+  if (k .eq. 0) then
+     lo%x(1:lo%nx) = x(1:lo%nx)
+     lo%y(1:lo%ny) = y(1:lo%ny)
+  else
+     la(k)%x(1:la(k)%nx) = x(1:la(k)%nx)
+     la(k)%y(1:la(k)%ny) = y(1:la(k)%ny)
+  end if
+end subroutine ref_set_coords
+
+!.....the reference's gauge sample (output.f90:1035-1155) of the current
+!     level-2 elevation of layer id at (x0, y0)
+subroutine ref_get_ts(x0, y0, id, dat) bind(C, name="ref_get_ts")
+  implicit none
+  real :: x0, y0, dat
+  integer :: id
+  call get_ts(dat, x0, y0, lo, la, id)
+end subroutine ref_get_ts
 end module ref_state
+
+!.....the CPU side of GCOMCOT_GET_Z (GPUOutput.cu:29-44) that GET_TS calls:
+!     level-2 Z at (i,j), (i+1,j), (i,j+1), (i+1,j+1) of layer lid
+subroutine ref_get_z(lid, l1, l2, l3, l4, i, j) bind(C, name="gcomcot_get_z_")
+  use ref_state
+  implicit none
+  integer :: lid, i, j, k
+  real :: l1, l2, l3, l4
+  call ref_pick(lid, k)
+  if (k .eq. 0) then
+     l1 = lo%z(i, j, 2)
+     l2 = lo%z(i + 1, j, 2)
+     l3 = lo%z(i, j + 1, 2)
+     l4 = lo%z(i + 1, j + 1, 2)
+  else
+     l1 = la(k)%z(i, j, 2)
+     l2 = la(k)%z(i + 1, j, 2)
+     l3 = la(k)%z(i, j + 1, 2)
+     l4 = la(k)%z(i + 1, j + 1, 2)
+  end if
+end subroutine ref_get_z

@@ -180,6 +180,10 @@ hipError_t launch_step2(const DevLayer &L, bool, int, hipStream_t st) {
   emu_submit(st, run_step, &a, sizeof(a));  // the planner tests never take this path
   return hipSuccess;
 }
+hipError_t launch_gauges(const GaugeDev *, int, const ZNow &, const int *, float *, int,
+                         hipStream_t) {
+  return hipSuccess;  // gauges belong to the single-tree device loop, not the batch planner
+}
 int num_tile_variants() { return 1; }
 void tile_variant_shape(int, int *tx, int *ty) { *tx = 64; *ty = 8; }
 hipError_t launch_max(const DevLayer &, float *, hipStream_t) { return hipSuccess; }
