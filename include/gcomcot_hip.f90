@@ -162,6 +162,48 @@ module gcomcot_hip
       integer(c_int) :: length
     end function
 
+    integer(c_int) function gcomcot_time_steps(nsteps, ms_total, ms_each) &
+        bind(C, name="gcomcot_time_steps_")
+      import :: c_int, c_float
+      integer(c_int) :: nsteps
+      real(c_float) :: ms_total, ms_each
+    end function
+
+    integer(c_int) function gcomcot_autotune(reps, variant, blocking, ms_per_step) &
+        bind(C, name="gcomcot_autotune_")
+      import :: c_int, c_float
+      integer(c_int) :: reps, variant, blocking
+      real(c_float) :: ms_per_step
+    end function
+
+    integer(c_int) function gcomcot_set_temporal_blocking(mode) &
+        bind(C, name="gcomcot_set_temporal_blocking_")
+      import :: c_int
+      integer(c_int) :: mode
+    end function
+
+    ! Gauges sampled on the device (the GET_TS series, output.f90:1035-1155)
+    integer(c_int) function gcomcot_set_layer_coords(lid, x, y) &
+        bind(C, name="gcomcot_set_layer_coords_")
+      import :: c_int, c_float
+      integer(c_int) :: lid
+      real(c_float) :: x(*), y(*)
+    end function
+
+    integer(c_int) function gcomcot_gauges_set(n, x0, y0, lid, capacity) &
+        bind(C, name="gcomcot_gauges_set_")
+      import :: c_int, c_float
+      integer(c_int) :: n, lid(*), capacity
+      real(c_float) :: x0(*), y0(*)
+    end function
+
+    integer(c_int) function gcomcot_gauges_read(series, max_steps, nout) &
+        bind(C, name="gcomcot_gauges_read_")
+      import :: c_int, c_float
+      real(c_float) :: series(*)
+      integer(c_int) :: max_steps, nout
+    end function
+
     ! ---------------------------------------------------------- Part 3
     integer(c_int) function gcomcot_batch_reset() bind(C, name="gcomcot_batch_reset_")
       import :: c_int
