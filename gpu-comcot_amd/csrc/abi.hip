@@ -14,6 +14,7 @@
 #include "comcot_consts.h"
 #include "gcomcot_hip_internal.h"
 #include "kernels.h"
+#include "output.h"
 #include "tree.h"
 
 using namespace gc;
@@ -49,6 +50,8 @@ int create_layer(int id, int parent, int level, int laycord, int laygov, int rel
   if (int e = tree_validate_layer(g_tree, id, parent, rel_size, rel_time, corners, nx, ny))
     return e;
   if (int e = ensure_init()) return e;
+  if (id == 1 && out_active())  // a new tree: queued files are finished first
+    if (int e = out_disable(g_tree)) return e;
   return tree_create_layer(g_tree, g_stream, id, parent, level, laycord, laygov, rel_size,
                            rel_time, corners, fric_switch, fric_coef, dt, dx, dy, nx, ny, H, Z);
 }
@@ -226,6 +229,7 @@ void gcomcot_get_z_(int *lid, float *l1, float *l2, float *l3, float *l4, int *i
 void cuda_shutdown_(void) {
   if (!g_init) return;
   (void)hipStreamSynchronize(g_stream);
+  (void)out_disable(g_tree);
   tree_free(g_tree);
   (void)hipStreamDestroy(g_stream);
   g_stream = nullptr;
@@ -233,15 +237,12 @@ void cuda_shutdown_(void) {
 }
 
 // ------------------------------------------------------------- extended
-int gcomcot_run_steps_(int *nsteps, int *use_graph) {
-  int n = *nsteps;
-  if (n < 0) return fail(23, "negative step count");
-  if (!layer(1)) return fail(8, "no root layer");
-  if (int e = ensure_init()) return e;
+// n root steps of the device loop, no output files
+static int run_plain(int n, int use_graph) {
   if (tree_step2_eligible(g_tree) && n >= 2) {
     // a linear single layer: two root steps per launch; a graph of two
     // launches (four steps) brings the level parity back for replay
-    if (*use_graph && n >= 4) {
+    if (use_graph && n >= 4) {
       hipGraph_t graph = nullptr;
       hipGraphExec_t exec = nullptr;
       HIPCHK(hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal));
@@ -263,7 +264,7 @@ int gcomcot_run_steps_(int *nsteps, int *use_graph) {
       if (int r = tree_enqueue_root_step(g_tree, g_stream)) return r;
     return sync();
   }
-  if (*use_graph && n >= 2 && tree_launches_per_step(g_tree) <= kGraphMaxLaunches) {
+  if (use_graph && n >= 2 && tree_launches_per_step(g_tree) <= kGraphMaxLaunches) {
     // Two root steps bring every layer's level parity back to where it
     // started (children with odd REL_TIME flip once per root step), so a
     // captured pair can be replayed.
@@ -285,6 +286,49 @@ int gcomcot_run_steps_(int *nsteps, int *use_graph) {
   for (int k = 0; k < n; ++k)
     if (int r = tree_enqueue_root_step(g_tree, g_stream)) return r;
   return sync();
+}
+
+int gcomcot_run_steps_(int *nsteps, int *use_graph) {
+  int n = *nsteps;
+  if (n < 0) return fail(23, "negative step count");
+  if (!layer(1)) return fail(8, "no root layer");
+  if (int e = ensure_init()) return e;
+  if (!out_active()) return run_plain(n, *use_graph);
+  // the run is cut at every root step that writes files; the files of a
+  // step are packed on the step stream and written while the loop goes on
+  if (sync()) return 1;
+  int done = 0;
+  HIPCHK(hipMemcpy(&done, g_tree.ctl + g_tree.ctl_in, sizeof(int), hipMemcpyDeviceToHost));
+  if (done == 0 && out_due(g_tree, 0))
+    if (int r = out_write(g_tree, 0, g_stream)) return r;
+  while (n > 0) {
+    const int m = (int)out_steps_to_next(g_tree, done, n);
+    if (int r = run_plain(m, *use_graph)) return r;
+    done += m;
+    n -= m;
+    if (out_due(g_tree, done))
+      if (int r = out_write(g_tree, done, g_stream)) return r;
+  }
+  return sync();
+}
+
+int gcomcot_output_set_(const char *dir, int *dirlen, int *iprt, int *out_opt, float *tend,
+                        float *tide) {
+  if (int e = ensure_init()) return e;
+  if (sync()) return 1;
+  return out_configure(g_tree, dir, *dirlen, *iprt, *out_opt, *tend, *tide);
+}
+
+int gcomcot_output_layer_(int *lid, int *fluxswitch) {
+  return out_set_fluxswitch(*lid, *fluxswitch);
+}
+
+int gcomcot_output_flush_(long long *nfiles) {
+  long long n = 0;
+  if (g_init && sync()) return 1;
+  const int r = out_flush(&n);
+  if (nfiles) *nfiles = n;
+  return r;
 }
 
 int gcomcot_time_steps_(int *nsteps, float *ms_total, float *ms_each) {

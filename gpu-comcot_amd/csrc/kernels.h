@@ -29,6 +29,9 @@ struct DevLayer {
   int *ctl;            // root-step counters: read ctl[ctl_in], write ctl[1-ctl_in]
   int ctl_in, advance; // advance: this launch writes the next counter
   float arr_thr;
+  // MAX_AMP's maximum (GPUOutput.cu:46-53: no wet test, NaN replaces),
+  // fused into the step of the outermost layer when != null
+  float *zraw;
 };
 
 enum StepMode { kFused = 0, kMassOnly = 1, kMomentOnly = 2 };
@@ -48,6 +51,18 @@ constexpr int kStep2Shapes = 3;
 
 // Reference-semantics maximum (no wet test) plus the wet max/arrival.
 hipError_t launch_max(const DevLayer &L, float *zmax_raw, hipStream_t st);
+
+// Output planes of one layer for the reference's file writers, in one pass:
+// oz = elevation with DATA_PRT's dry-land mask and tide (output.f90:781-793),
+// om/on = copies of the fluxes, ozmax = MAX_AMP's masked maximum
+// (output.f90:918-931).  Any output may be null.
+struct OutPack {
+  const float *z, *h, *m, *n, *zraw;
+  float *oz, *om, *on, *ozmax;
+  float tide;
+  long long count;
+};
+hipError_t launch_out_pack(const OutPack &p, hipStream_t st);
 
 // Nesting: child boundary fluxes for sub-step t (1 = JNQ, >1 = NEWQ)
 struct DevNest {

@@ -19,6 +19,8 @@
 // results equal the FP32 Fortran bit for bit; pow/exp use detmath.h.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "comcot_consts.h"
 #include "detmath.h"
 #include "kernels.h"
@@ -622,6 +624,10 @@ __device__ __forceinline__ void tile_body(const DevLayer &L, int t, int ti0, int
           if (z2 > zm) L.zmax[g] = z2;
         }
       }
+      if (L.zraw) {  // outermost layer only: every cell is its own
+        const float z2 = s.Z2(i, j);
+        if (!(L.zraw[g] >= z2)) L.zraw[g] = z2;
+      }
     }
   }
 }
@@ -764,6 +770,13 @@ __device__ __forceinline__ void tile_body2(const DevLayer &L, int t, int ti0, in
         if (arr != -2) L.arrival[g] = arr;
         if (zm != z0) L.zmax[g] = zm;
       }
+    }
+    if (L.zraw) {  // both steps in order, as two MAX_AMP launches would
+      float zr = L.zraw[g];
+      bool upd = false;
+      if (!(zr >= za)) { zr = za; upd = true; }
+      if (!(zr >= zb)) { zr = zb; upd = true; }
+      if (upd) L.zraw[g] = zr;
     }
   }
 }
@@ -921,6 +934,10 @@ __global__ void __launch_bounds__(kThreads)
           if (z2 > zm) L.zmax[g] = z2;
         }
       }
+      if (L.zraw) {
+        const float z2 = r.Z2(i, j);
+        if (!(L.zraw[g] >= z2)) L.zraw[g] = z2;
+      }
     }
     zq0 = zq1;
     zq1 = zq2;
@@ -946,7 +963,7 @@ __global__ void k_max(DevLayer L, float *zmax_raw) {
   if (g >= n) return;
   const int i = (int)(g % L.nx) + 1, j = (int)(g / L.nx) + 1;
   const float z2 = L.z2[g];
-  if (zmax_raw[g] < z2) zmax_raw[g] = z2;
+  if (!(zmax_raw[g] >= z2)) zmax_raw[g] = z2;  // a NaN replaces, as GPUOutput.cu:49-51
   if (L.zmax && (L.is_root || (i >= 2 && j >= 2)) && L.h[g] + z2 > kGx) {
     const float zm = L.zmax[g];
     if (z2 >= L.arr_thr && !(zm >= L.arr_thr)) L.arrival[g] = step;
@@ -1145,6 +1162,25 @@ __global__ void __launch_bounds__(256) k_copy(const float4 *__restrict__ src,
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
     dst[i] = src[i];
+}
+
+// Dry land shows 0, else the elevation plus the tide correction when it is
+// not negligible: DATA_PRT (output.f90:781-793) and MAX_AMP (:918-931).
+__device__ __forceinline__ float shown_elevation(float z, float h, float tide) {
+  if (z + h <= kGx && h < 0.0f) return 0.0f;
+  return fabsf(tide) > kGx ? z + tide : z;
+}
+
+__global__ void __launch_bounds__(256) k_out_pack(OutPack p) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < (size_t)p.count;
+       g += stride) {
+    const float h = p.h[g];
+    if (p.oz) p.oz[g] = shown_elevation(p.z[g], h, p.tide);
+    if (p.om) p.om[g] = p.m[g];
+    if (p.on) p.on[g] = p.n[g];
+    if (p.ozmax) p.ozmax[g] = shown_elevation(p.zraw[g], h, p.tide);
+  }
 }
 
 // Gauge samples (GET_TS, output.f90:1085-1094): the 2x2 stencil of the
@@ -1450,6 +1486,13 @@ hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st)
   const size_t n = bytes / 16;
   hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, st, static_cast<const float4 *>(src),
                      static_cast<float4 *>(dst), n);
+  return hipGetLastError();
+}
+
+hipError_t launch_out_pack(const OutPack &p, hipStream_t st) {
+  if (p.count <= 0) return hipSuccess;
+  const long long blocks = std::min<long long>((p.count + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_out_pack, dim3((unsigned)blocks), dim3(256), 0, st, p);
   return hipGetLastError();
 }
 

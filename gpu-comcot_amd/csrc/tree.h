@@ -53,6 +53,7 @@ struct Tree {
   long long gauge_step0 = 0;  // root steps completed when the gauges were set
   int variant = 0;     // tile variant of the step kernels
   int blocking = 1;    // two steps per launch for linear single layers: 0 off, 1+shape
+  int raw_max = 0;     // device loop also keeps MAX_AMP's maximum of the outermost layer
   LayerRec *layer(int id) {
     return (id >= 1 && id < kMaxLayers && layers[id].used) ? &layers[id] : nullptr;
   }

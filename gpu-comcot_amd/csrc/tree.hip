@@ -271,6 +271,7 @@ DevLayer dev_view(const Tree &T, const LayerRec &L, bool with_max, bool advance)
   d.ctl_in = T.ctl_in;
   d.advance = advance ? 1 : 0;
   d.arr_thr = T.arr_thr;
+  d.zraw = (T.raw_max && with_max && advance && L.id == 1) ? L.zmax_raw : nullptr;
   return d;
 }
 

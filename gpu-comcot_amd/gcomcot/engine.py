@@ -8,6 +8,7 @@ the order of the Fortran main loop (comcot.f90:359-439), which is what the
 unmodified Fortran driver does when linked against libgcomcot_hip.so.
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -95,6 +96,26 @@ class DeviceRun:
         check(lib().gcomcot_autotune_(ip(reps), ctypes.byref(v), ctypes.byref(b),
                                       ctypes.byref(ms)), "autotune")
         return v.value, b.value, ms.value
+
+    # ------------------------------------------------------------ outputs
+    def set_output(self, directory, iprt, out_opt=2, tend=0.0, tide=0.0, fluxswitch=None):
+        """Reference-format files written by the device loop into `directory`:
+        snapshots every `iprt` root steps (and at step 0), the maximum files
+        of layer 1 when out_opt is 0 or 2, with TEND `tend` and TIDE_LEVEL
+        `tide`; fluxswitch maps layer id -> FLUXSWITCH (default 1, z only)."""
+        L = lib()
+        for lid, fs in (fluxswitch or {}).items():
+            check(L.gcomcot_output_layer_(ip(lid), ip(fs)), f"output layer {lid}")
+        d = os.fsencode(os.fspath(directory))
+        buf = ctypes.create_string_buffer(d)
+        check(L.gcomcot_output_set_(buf, ip(len(d)), ip(iprt), ip(out_opt), fp(tend), fp(tide)),
+              "output set")
+
+    def flush_output(self):
+        """Wait for every queued file; returns the number written so far."""
+        n = ctypes.c_longlong(0)
+        check(lib().gcomcot_output_flush_(ctypes.byref(n)), "output flush")
+        return n.value
 
     # ------------------------------------------------------------ running
     def run(self, steps, graph=True):

@@ -182,6 +182,26 @@ module gcomcot_hip
       integer(c_int) :: mode
     end function
 
+    ! Reference-format output files of the device loop (DATA_PRT, MAX_AMP)
+    integer(c_int) function gcomcot_output_set(dir, dirlen, iprt, out_opt, tend, tide) &
+        bind(C, name="gcomcot_output_set_")
+      import :: c_int, c_float, c_char
+      character(kind=c_char) :: dir(*)
+      integer(c_int) :: dirlen, iprt, out_opt
+      real(c_float) :: tend, tide
+    end function
+
+    integer(c_int) function gcomcot_output_layer(lid, fluxswitch) &
+        bind(C, name="gcomcot_output_layer_")
+      import :: c_int
+      integer(c_int) :: lid, fluxswitch
+    end function
+
+    integer(c_int) function gcomcot_output_flush(nfiles) bind(C, name="gcomcot_output_flush_")
+      import :: c_int, c_long_long
+      integer(c_long_long) :: nfiles
+    end function
+
     ! Gauges sampled on the device (the GET_TS series, output.f90:1035-1155)
     integer(c_int) function gcomcot_set_layer_coords(lid, x, y) &
         bind(C, name="gcomcot_set_layer_coords_")

@@ -178,6 +178,31 @@ int gcomcot_set_layer_coords_(int *lid, const float *x, const float *y);
 int gcomcot_gauges_set_(int *n, const float *x0, const float *y0, const int *lid, int *capacity);
 int gcomcot_gauges_read_(float *series, int *max_steps, int *nout);
 
+/* Reference-format output files of the device loop (reference: ALL_PRT /
+ * DATA_PRT, output.f90:737-851, at K = 0 and every IPRT root steps,
+ * comcot.f90:308-310, 415-417; MAX_AMP, output.f90:855-1031, after every
+ * root step when OUT_OPT is 0 or 2, comcot.f90:420-423).
+ * gcomcot_output_set_ turns them on for gcomcot_run_steps_: files go to the
+ * directory dir[0..*dirlen) (0 = the working directory, which must exist);
+ * *iprt <= 0 writes no snapshots; *out_opt 0 or 2 writes the maximum files
+ * and makes the step kernel keep MAX_AMP's maximum of layer 1 (the one
+ * cuda_getzmax_ returns); *tend is the run's TEND in seconds and *tide the
+ * TIDE_LEVEL.  Snapshots z_LL_KKKKKK.dat (dry land 0, tide added) and, for
+ * FLUXSWITCH 0, m_/n_LL_KKKKKK.dat; maxima zmax_layer01_MMMMmins.dat at each
+ * half hour of model time and zmax_layer01.dat from TEND - 2 DT on; each file
+ * one Fortran unformatted sequential record (4-byte length, the (NX,NY)
+ * REAL plane, 4-byte length), as the reference writes them.
+ * gcomcot_output_layer_ sets a layer's FLUXSWITCH (0: z, m, n; 1 or 9: z;
+ * others: none; default 1).  Files are written by a thread of the library
+ * from pinned host copies while the loop continues; gcomcot_output_flush_
+ * waits until every file is on disk and reports how many were written
+ * (cuda_shutdown_ and a new layer 1 flush too).  Each returns 0 or an
+ * error code. */
+int gcomcot_output_set_(const char *dir, int *dirlen, int *iprt, int *out_opt, float *tend,
+                        float *tide);
+int gcomcot_output_layer_(int *lid, int *fluxswitch);
+int gcomcot_output_flush_(long long *nfiles);
+
 /* Autotuner: for an outermost layer without nested layers, time every
  * tile variant of the single-step kernel and, for linear layers, every
  * shape of the two-step kernel, *reps launches each, on the live layer

@@ -265,3 +265,23 @@ class Reference:
         xf, yf = np.empty(2 * s.nx, np.float32), np.empty(2 * s.ny, np.float32)
         self.lib().ref_flux_io(_i(lid), _p(xf), _p(yf))
         return xf.reshape(2, s.nx), yf.reshape(2, s.ny)
+
+    def maxamp(self):
+        """MAXAMP_LAUNCH after a root step: the recorder maximum of layer 1."""
+        self.lib().ref_maxamp()
+
+    def raw_max(self):
+        s = self.specs[1]
+        out = np.empty(s.nx * s.ny, np.float32)
+        self.lib().cuda_getzmax_(_p(out), _i(1))
+        return out.reshape(s.ny, s.nx)
+
+    def data_prt(self, k, lid, fluxswitch, tide=0.0):
+        """The reference's DATA_PRT (output.f90:758-851): writes z_/m_/n_
+        files of layer lid at step k into the working directory."""
+        self.lib().ref_data_prt(_i(k), _i(lid), _i(fluxswitch), _f(tide))
+
+    def max_amp(self, time, tend, tide=0.0):
+        """The reference's MAX_AMP (output.f90:855-1031) at model time
+        `time` (REAL): writes zmax files into the working directory when due."""
+        self.lib().ref_max_amp(_f(time), _f(tend), _f(tide))

@@ -884,7 +884,7 @@ struct State {
       if (L.parent != -1 && ((int)(k % L.nx) == 0 || (int)(k / L.nx) == 0)) continue;
       T z2 = L.z[n + k];
       bool wet = L.h[k] + z2 > GX;
-      if (L.zmax_raw[k] < z2) L.zmax_raw[k] = z2;
+      if (!(L.zmax_raw[k] >= z2)) L.zmax_raw[k] = z2;  // a NaN replaces too
       if (wet) {
         if (z2 >= arrival_thr && !(L.zmax_wet[k] >= arrival_thr)) L.arrival[k] = nstep;
         if (z2 > L.zmax_wet[k]) L.zmax_wet[k] = z2;

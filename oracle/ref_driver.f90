@@ -25,6 +25,10 @@ module ref_state
   type(layer), save :: lo
   type(layer), allocatable, save :: la(:)
   integer, save :: nla = 0
+  ! the maximum that the GPU half keeps for the outermost layer and that
+  ! MAX_AMP reads back with CUDA_GETZMAX: starts at the initial elevation
+  ! (GPUcomcot.cu:79), raised after each root step by MAXAMP_LAUNCH
+  real, allocatable, save :: zmaxr(:,:)
 contains
 
 
@@ -114,6 +118,9 @@ subroutine ref_init_root(nx, ny, laycord, laygov, dx, dy, dt, y, dely,  &
   call alloc(lo, 1)
   lo%h = reshape(h, (/nx, ny/))
   lo%z(:,:,1) = reshape(z0, (/nx, ny/))
+  if (allocated(zmaxr)) deallocate(zmaxr)
+  allocate(zmaxr(nx, ny))
+  zmaxr = lo%z(:,:,1)
   call pq_depth(lo)
   if (laycord .eq. 0) call para(lo)
 end subroutine ref_init_root
@@ -351,6 +358,44 @@ subroutine ref_get_ts(x0, y0, id, dat) bind(C, name="ref_get_ts")
   integer :: id
   call get_ts(dat, x0, y0, lo, la, id)
 end subroutine ref_get_ts
+
+!.....MAXAMP_LAUNCH's recorder (GPUOutput.cu:46-53) on the level-2
+!     elevation of the outermost layer: a value that is not below the
+!     maximum leaves it, anything else (a NaN too) replaces it
+subroutine ref_maxamp() bind(C, name="ref_maxamp")
+  implicit none
+  where (.not. (zmaxr .ge. lo%z(:,:,2))) zmaxr = lo%z(:,:,2)
+end subroutine ref_maxamp
+
+!.....the reference's snapshot writer DATA_PRT (output.f90:758-851) for
+!     layer id at step k, into the working directory
+subroutine ref_data_prt(k, id, fluxswitch, tide) bind(C, name="ref_data_prt")
+  implicit none
+  integer :: k, id, fluxswitch
+  real :: tide
+  integer :: j
+  call ref_pick(id, j)
+  if (j .eq. 0) then
+     lo%fluxswitch = fluxswitch
+     lo%tide_level = tide
+     lo%sedi_switch = 1
+     call data_prt(k, lo)
+  else
+     la(j)%fluxswitch = fluxswitch
+     la(j)%tide_level = tide
+     la(j)%sedi_switch = 1
+     call data_prt(k, la(j))
+  end if
+end subroutine ref_data_prt
+
+!.....the reference's maximum writer MAX_AMP (output.f90:855-1031) at model
+!     time `time` of a run ending at `tend`, into the working directory
+subroutine ref_max_amp(time, tend, tide) bind(C, name="ref_max_amp")
+  implicit none
+  real :: time, tend, tide
+  lo%tide_level = tide
+  call max_amp(lo, la, time, tend)
+end subroutine ref_max_amp
 end module ref_state
 
 !.....the CPU side of GCOMCOT_GET_Z (GPUOutput.cu:29-44) that GET_TS calls:
@@ -373,3 +418,47 @@ subroutine ref_get_z(lid, l1, l2, l3, l4, i, j) bind(C, name="gcomcot_get_z_")
      l4 = la(k)%z(i + 1, j + 1, 2)
   end if
 end subroutine ref_get_z
+
+!.....CUDA_GETZ / CUDA_GETMN / CUDA_GETZMAX (GPUOutput.cu:65-103) as DATA_PRT
+!     and MAX_AMP call them: the level-2 elevation and fluxes of a layer
+!     (what the GPU's Zout/MNout hold before CHANGE), and the recorder's
+!     maximum of the outermost layer
+subroutine ref_cuda_getz(z, lid) bind(C, name="cuda_getz_")
+  use ref_state
+  implicit none
+  integer :: lid, k
+  real :: z(*)
+  call ref_pick(lid, k)
+  if (k .eq. 0) then
+     z(1:lo%nx*lo%ny) = reshape(lo%z(:,:,2), (/lo%nx*lo%ny/))
+  else
+     z(1:la(k)%nx*la(k)%ny) = reshape(la(k)%z(:,:,2), (/la(k)%nx*la(k)%ny/))
+  end if
+end subroutine ref_cuda_getz
+
+subroutine ref_cuda_getmn(m, n, lid) bind(C, name="cuda_getmn_")
+  use ref_state
+  implicit none
+  integer :: lid, k
+  real :: m(*), n(*)
+  call ref_pick(lid, k)
+  if (k .eq. 0) then
+     m(1:lo%nx*lo%ny) = reshape(lo%m(:,:,2), (/lo%nx*lo%ny/))
+     n(1:lo%nx*lo%ny) = reshape(lo%n(:,:,2), (/lo%nx*lo%ny/))
+  else
+     m(1:la(k)%nx*la(k)%ny) = reshape(la(k)%m(:,:,2), (/la(k)%nx*la(k)%ny/))
+     n(1:la(k)%nx*la(k)%ny) = reshape(la(k)%n(:,:,2), (/la(k)%nx*la(k)%ny/))
+  end if
+end subroutine ref_cuda_getmn
+
+subroutine ref_cuda_getzmax(zmax, lid) bind(C, name="cuda_getzmax_")
+  use ref_state
+  implicit none
+  integer :: lid
+  real :: zmax(*)
+  if (lid .ne. 1) then
+     write(*,*) 'ref_driver: the recorder maximum is kept for layer 1 only'
+     stop 1
+  end if
+  zmax(1:lo%nx*lo%ny) = reshape(zmaxr, (/lo%nx*lo%ny/))
+end subroutine ref_cuda_getzmax
