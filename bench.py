@@ -70,6 +70,10 @@ def parse():
     ap.add_argument("--serve", type=int, default=1, help="run the serving streams (0 = skip)")
     ap.add_argument("--serve-requests", type=int, default=24)
     ap.add_argument("--serve-rates", default="2,5,10,20")
+    ap.add_argument("--outputs", type=int, default=1,
+                    help="time the device loop with reference-format output files (0 = skip)")
+    ap.add_argument("--outputs-steps", type=int, default=1000)
+    ap.add_argument("--outputs-iprt", type=int, default=250)
     ap.add_argument("--accuracy", type=int, default=1,
                     help="with the CPU baseline: FP32 vs FP64-oracle error on a reduced grid")
     ap.add_argument("--graphs", type=int, default=1,
@@ -186,6 +190,48 @@ def bench_other_configs(a):
                      {"tile_variant": tuned[0], "two_step_shape": tuned[1]},
                      "gcups": round(gcups, 3), "ms_per_step": round(wall * 1e3 / sc.steps, 5),
                      "algorithmic_GBs": round(gcups * bpc, 1)}
+    return out
+
+
+def bench_outputs(a):
+    """The headline workload through the device loop (graphs), without and
+    with the reference's output files: snapshots z/m/n every --outputs-iprt
+    root steps and at step 0, MAX_AMP's maximum kept by the step kernel and
+    written when due (TEND at the last step).  ms_per_step_* time the loop
+    call; files still being written when it returns are waited for after it
+    (final_flush_ms).  Files go to a temporary directory, removed afterwards."""
+    import shutil
+    import tempfile
+
+    from gcomcot import DeviceRun, config1
+    n = a.outputs_steps
+    sc = config1(nx=a.nx, ny=a.ny, steps=n)
+    dt = np.float32(sc.layers[0].dt)
+    out = {"steps": n, "iprt": a.outputs_iprt, "fluxswitch": 0, "out_opt": 2}
+    walls = {}
+    d = tempfile.mkdtemp(prefix="gcomcot_out_")
+    try:
+        for mode in ("off", "on"):
+            log(f"outputs {mode}: {n} steps")
+            run = DeviceRun(sc)
+            run.autotune()
+            if mode == "on":
+                run.set_output(d, a.outputs_iprt, 2, np.float32((n - 1) * dt), 0.0, {1: 0})
+            t0 = time.perf_counter()
+            run.run(n, graph=bool(a.graphs))
+            walls[mode] = time.perf_counter() - t0
+            if mode == "on":
+                t1 = time.perf_counter()
+                out["files_written"] = run.flush_output()
+                out["final_flush_ms"] = round((time.perf_counter() - t1) * 1e3, 2)
+                out["bytes_on_disk"] = sum(os.path.getsize(os.path.join(d, f))
+                                           for f in os.listdir(d))
+            run.close()
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    out["ms_per_step_off"] = round(walls["off"] * 1e3 / n, 5)
+    out["ms_per_step_on"] = round(walls["on"] * 1e3 / n, 5)
+    out["overhead_frac"] = round(walls["on"] / walls["off"] - 1.0, 4)
     return out
 
 
@@ -319,6 +365,7 @@ def main():
         if a.accuracy:
             cpu["accuracy"] = cpu_accuracy()
     others = bench_other_configs(a) if a.others else None
+    outputs = bench_outputs(a) if a.outputs else None
     batch = bench_batch(a) if a.batch else None
     serve = bench_serve(a) if a.serve else None
     copy = copy_probe(1024)
@@ -348,6 +395,7 @@ def main():
         "cells_arrived": arrived,
         "hbm_copy": copy,
         "other_configs": others,
+        "outputs": outputs,
         "batch": batch,
         "serve": serve,
     }

@@ -13,9 +13,11 @@
 //
 // Here the masking runs on the device into a staging area (one pass per
 // layer), the staging area goes to pinned host memory on a copy stream of
-// its own, and a writer thread puts the records on disk, so the step loop
-// only pays for the pack pass.  Two staging slots are used in turn; a
-// third output waits for the older one.
+// its own, a host function on that stream hands the files to a pool of
+// writer threads as soon as the copy ends, and the writers put the records
+// on disk in parallel, so the step loop only pays for the pack pass.  Two
+// staging slots are used in turn; a third output waits for the older one
+// (the loop then runs at the speed of the disk).
 #pragma once
 
 #include <hip/hip_runtime.h>

@@ -3,11 +3,14 @@
 // and masking follow output.f90 (DATA_PRT :758-851, MAX_AMP :855-940).
 #include "output.h"
 
+#include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <condition_variable>
 #include <cstdint>
 #include <cstdio>
 #include <deque>
+#include <functional>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -26,12 +29,21 @@ enum SlotState { kFree, kCopying, kWriting };
 struct Slot {
   void *dev = nullptr, *host = nullptr;
   size_t cap = 0;
-  hipEvent_t packed = nullptr, copied = nullptr;
+  hipEvent_t packed = nullptr;
   SlotState state = kFree;
+  size_t remaining = 0;  // files of the slot not yet on disk
   std::vector<FileRec> files;
 };
 
 constexpr int kSlots = 2;
+// Writer threads; the files of one output go down in parallel.  A file
+// name always goes to the same writer, so rewrites of one name (the final
+// maximum file) land in step order.
+constexpr int kWriters = 4;
+
+struct Task {
+  int slot, file;
+};
 
 struct Output {
   bool active = false;
@@ -40,6 +52,7 @@ struct Output {
   float tend = 0.0f, tide = 0.0f;
   int flux[kMaxLayers];
   bool initial_pending = false;
+  bool trace = false;
   // model time after root step clk_k, accumulated in REAL as the main
   // loop does (comcot.f90:304, 323)
   long long clk_k = 0;
@@ -47,33 +60,25 @@ struct Output {
   hipStream_t copy_st = nullptr;
   Slot slots[kSlots];
   int next = 0;
-  // shared with the writer thread
+  bool started = false;
+  // shared with the copy callbacks and the writer threads
   std::mutex mu;
   std::condition_variable cv;
-  std::deque<int> queue;
-  std::thread th;
-  bool stop = false;
+  std::deque<Task> queue[kWriters];
   long long nfiles = 0;
   int err = 0;
   std::string errmsg;
 
   Output() {
     for (int &f : flux) f = 1;
-  }
-  ~Output() {
-    // the writer only does file I/O, so it can finish what it holds here
-    if (th.joinable()) {
-      {
-        std::lock_guard<std::mutex> lk(mu);
-        stop = true;
-      }
-      cv.notify_all();
-      th.join();
-    }
+    const char *e = std::getenv("GCOMCOT_OUTPUT_TRACE");
+    trace = e && *e && *e != '0';
   }
 };
 
-Output g_out;
+// Never destroyed: a copy callback or a writer may still run while the
+// process ends (files not flushed by then are not guaranteed).
+Output &g_out = *new Output;
 
 // Fortran Iw.w edit descriptor: zero-padded to w digits, w asterisks when
 // the value does not fit.
@@ -102,66 +107,64 @@ int write_record(const std::string &path, const char *data, size_t bytes, std::s
   return ok ? 0 : 1;
 }
 
-void writer_main() {
+int writer_of(const std::string &name) {
+  return (int)(std::hash<std::string>{}(name) % kWriters);
+}
+
+void writer_main(int w) {
   Output &O = g_out;
   for (;;) {
-    int s;
+    Task t;
     {
       std::unique_lock<std::mutex> lk(O.mu);
-      O.cv.wait(lk, [&] { return O.stop || !O.queue.empty(); });
-      if (O.queue.empty()) return;
-      s = O.queue.front();
-      O.queue.pop_front();
+      O.cv.wait(lk, [&] { return !O.queue[w].empty(); });
+      t = O.queue[w].front();
+      O.queue[w].pop_front();
     }
-    Slot &S = O.slots[s];
-    int err = 0;
+    // the slot's file list is not touched again until its last file is down
+    Slot &S = O.slots[t.slot];
+    const FileRec &f = S.files[(size_t)t.file];
+    const std::string path = O.dir.empty() ? f.name : O.dir + "/" + f.name;
     std::string why;
-    long long done = 0;
-    for (const FileRec &f : S.files) {
-      const std::string path = O.dir.empty() ? f.name : O.dir + "/" + f.name;
-      if (write_record(path, static_cast<const char *>(S.host) + f.off, f.bytes, &why)) {
-        err = 1;
-        break;
-      }
-      ++done;
+    const auto t0 = std::chrono::steady_clock::now();
+    const int bad = write_record(path, static_cast<const char *>(S.host) + f.off, f.bytes, &why);
+    if (O.trace) {
+      const double ms = std::chrono::duration<double, std::milli>(
+                            std::chrono::steady_clock::now() - t0).count();
+      std::fprintf(stderr, "[gcomcot output] writer %d: %s %zu B %.2f ms\n", w, f.name.c_str(),
+                   f.bytes + 8, ms);
     }
     {
       std::lock_guard<std::mutex> lk(O.mu);
-      O.nfiles += done;
-      if (err && !O.err) {
-        O.err = 1;
-        O.errmsg = why;
+      if (bad) {
+        if (!O.err) {
+          O.err = 1;
+          O.errmsg = why;
+        }
+      } else {
+        ++O.nfiles;
       }
-      S.state = kFree;
+      if (--S.remaining == 0) S.state = kFree;
     }
     O.cv.notify_all();
   }
 }
 
-// A slot whose copy is done goes to the writer.
-int hand_over(int s) {
+// Host function on the copy stream after a slot's copy: its files go to
+// the writers.  Runs on a runtime thread and makes no HIP call.
+void on_copied(void *arg) {
   Output &O = g_out;
-  HIPCHK(hipEventSynchronize(O.slots[s].copied));
+  const int s = (int)(intptr_t)arg;
   {
     std::lock_guard<std::mutex> lk(O.mu);
-    O.slots[s].state = kWriting;
-    O.queue.push_back(s);
+    Slot &S = O.slots[s];
+    S.state = kWriting;
+    S.remaining = S.files.size();
+    for (size_t f = 0; f < S.files.size(); ++f)
+      O.queue[writer_of(S.files[f].name)].push_back({s, (int)f});
+    if (S.files.empty()) S.state = kFree;
   }
   O.cv.notify_all();
-  return 0;
-}
-
-int poll_copies() {
-  for (int s = 0; s < kSlots; ++s) {
-    if (g_out.slots[s].state != kCopying) continue;
-    const hipError_t q = hipEventQuery(g_out.slots[s].copied);
-    if (q == hipSuccess) {
-      if (int r = hand_over(s)) return r;
-    } else if (q != hipErrorNotReady) {
-      HIPCHK(q);
-    }
-  }
-  return 0;
 }
 
 void wait_free(int s) {
@@ -198,6 +201,20 @@ bool snap_due(long long k) {
   const Output &O = g_out;
   if (O.iprt <= 0) return false;
   return k == 0 ? O.initial_pending : k % O.iprt == 0;
+}
+
+// Give slot S at least `total` bytes of staging and pinned memory (the
+// slot must be free).
+int reserve(Slot &S, size_t total) {
+  if (S.cap >= total) return 0;
+  if (S.dev) HIPCHK(hipFree(S.dev));
+  if (S.host) HIPCHK(hipHostFree(S.host));
+  S.dev = S.host = nullptr;
+  S.cap = 0;
+  HIPCHK(hipMalloc(&S.dev, total));
+  HIPCHK(hipHostMalloc(&S.host, total, hipHostMallocDefault));
+  S.cap = total;
+  return 0;
 }
 
 int release_slots() {
@@ -237,11 +254,24 @@ int out_configure(Tree &T, const char *dir, int dirlen, int iprt, int out_opt, f
     O.errmsg.clear();
   }
   if (!O.copy_st) HIPCHK(hipStreamCreateWithFlags(&O.copy_st, hipStreamNonBlocking));
-  for (Slot &S : O.slots) {
+  for (Slot &S : O.slots)
     if (!S.packed) HIPCHK(hipEventCreateWithFlags(&S.packed, hipEventDisableTiming));
-    if (!S.copied) HIPCHK(hipEventCreateWithFlags(&S.copied, hipEventDisableTiming));
+  if (!O.started) {
+    for (int w = 0; w < kWriters; ++w) std::thread(writer_main, w).detach();
+    O.started = true;
   }
-  if (!O.th.joinable()) O.th = std::thread(writer_main);
+  // the largest output of the present tree, reserved now rather than in the loop
+  size_t most = 0;
+  for (int id = 1; id < kMaxLayers; ++id) {
+    const LayerRec *L = T.layer(id);
+    if (!L) continue;
+    const size_t plane = (L->nxy() * sizeof(float) + 255) & ~(size_t)255;
+    const int fs = O.flux[id];
+    if (iprt > 0) most += plane * ((fs <= 1 || fs == 9) ? 1 : 0) + plane * (fs == 0 ? 2 : 0);
+    if (id == 1 && (out_opt == 0 || out_opt == 2)) most += plane;
+  }
+  for (Slot &S : O.slots)
+    if (int r = reserve(S, most)) return r;
   T.raw_max = (out_opt == 0 || out_opt == 2) ? 1 : 0;
   O.active = true;
   return 0;
@@ -330,22 +360,11 @@ int out_write(Tree &T, long long k, hipStream_t st) {
   }
   if (plans.empty()) return 0;
 
-  if (int r = poll_copies()) return r;
   const int s = O.next;
   O.next = (O.next + 1) % kSlots;
   Slot &S = O.slots[s];
-  if (S.state == kCopying)
-    if (int r = hand_over(s)) return r;
   wait_free(s);
-  if (S.cap < total) {
-    if (S.dev) HIPCHK(hipFree(S.dev));
-    if (S.host) HIPCHK(hipHostFree(S.host));
-    S.dev = S.host = nullptr;
-    S.cap = 0;
-    HIPCHK(hipMalloc(&S.dev, total));
-    HIPCHK(hipHostMalloc(&S.host, total, hipHostMallocDefault));
-    S.cap = total;
-  }
+  if (int r = reserve(S, total)) return r;
   char *base = static_cast<char *>(S.dev);
   auto at = [&](size_t off1) { return off1 ? reinterpret_cast<float *>(base + off1 - 1) : nullptr; };
   for (const Plan &p : plans) {
@@ -365,20 +384,20 @@ int out_write(Tree &T, long long k, hipStream_t st) {
     hipError_t e = launch_out_pack(q, st);
     if (e != hipSuccess) return fail(205, "output pack launch failed: %s", hipGetErrorString(e));
   }
+  {
+    std::lock_guard<std::mutex> lk(O.mu);
+    S.files = std::move(files);
+    S.state = kCopying;
+  }
   HIPCHK(hipEventRecord(S.packed, st));
   HIPCHK(hipStreamWaitEvent(O.copy_st, S.packed, 0));
   HIPCHK(hipMemcpyAsync(S.host, S.dev, total, hipMemcpyDeviceToHost, O.copy_st));
-  HIPCHK(hipEventRecord(S.copied, O.copy_st));
-  S.files = std::move(files);
-  S.state = kCopying;
+  HIPCHK(hipLaunchHostFunc(O.copy_st, on_copied, (void *)(intptr_t)s));
   return 0;
 }
 
 int out_flush(long long *nfiles) {
   Output &O = g_out;
-  for (int s = 0; s < kSlots; ++s)
-    if (O.slots[s].state == kCopying)
-      if (int r = hand_over(s)) return r;
   for (int s = 0; s < kSlots; ++s) wait_free(s);
   std::lock_guard<std::mutex> lk(O.mu);
   if (nfiles) *nfiles = O.nfiles;
