@@ -15,6 +15,7 @@
 #include "gcomcot_hip_internal.h"
 #include "kernels.h"
 #include "output.h"
+#include "source.h"
 #include "tree.h"
 
 using namespace gc;
@@ -317,6 +318,33 @@ int gcomcot_output_set_(const char *dir, int *dirlen, int *iprt, int *out_opt, f
   if (int e = ensure_init()) return e;
   if (sync()) return 1;
   return out_configure(g_tree, dir, *dirlen, *iprt, *out_opt, *tend, *tide);
+}
+
+int gcomcot_fault_deform_(const float *params, int *fswitch) {
+  if (!layer(1)) return fail(8, "no root layer");
+  if (int e = ensure_init()) return e;
+  if (sync()) return 1;
+  return tree_apply_fault(g_tree, g_stream, params, *fswitch);
+}
+
+int gcomcot_get_deform_(int *lid, float *deform) {
+  LayerRec *L = layer(*lid);
+  if (!L) return fail(24, "gcomcot_get_deform_: no layer %d", *lid);
+  if (sync()) return 1;
+  if (!L->deform) {
+    std::memset(deform, 0, L->nxy() * sizeof(float));
+    return 0;
+  }
+  HIPCHK(hipMemcpy(deform, L->deform, L->nxy() * sizeof(float), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int gcomcot_get_depth_(int *lid, float *h) {
+  LayerRec *L = layer(*lid);
+  if (!L) return fail(24, "gcomcot_get_depth_: no layer %d", *lid);
+  if (sync()) return 1;
+  HIPCHK(hipMemcpy(h, L->h, L->nxy() * sizeof(float), hipMemcpyDeviceToHost));
+  return 0;
 }
 
 int gcomcot_output_layer_(int *lid, int *fluxswitch) {

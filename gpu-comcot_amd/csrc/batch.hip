@@ -41,6 +41,7 @@
 #include "comcot_consts.h"
 #include "gcomcot_hip_internal.h"
 #include "kernels.h"
+#include "source.h"
 #include "tree.h"
 
 using namespace gc;
@@ -622,6 +623,24 @@ int gcomcot_batch_get_max_(int *sid, int *lid, float *zmax_wet, int *arrival) {
   if (arrival)
     HIPCHK(hipMemcpy(arrival, L->arrival, L->nxy() * sizeof(int), hipMemcpyDeviceToHost));
   return 0;
+}
+
+int gcomcot_batch_set_coords_(int *sid, int *lid, const float *x, const float *y) {
+  Scenario *s = scenario(*sid);
+  if (!s) return fail(31, "batch: no scenario %d", *sid);
+  LayerRec *L = s->tree.layer(*lid);
+  if (!L) return fail(33, "batch: scenario %d has no layer %d", *sid, *lid);
+  s->tree.xs[*lid].assign(x, x + L->nx);
+  s->tree.ys[*lid].assign(y, y + L->ny);
+  return 0;
+}
+
+int gcomcot_batch_fault_deform_(int *sid, const float *params, int *fswitch) {
+  Scenario *s = scenario(*sid);
+  if (!s) return fail(31, "batch: no scenario %d", *sid);
+  if (int e = eng_init()) return e;
+  HIPCHK(hipStreamSynchronize(g_eng.st));
+  return tree_apply_fault(s->tree, g_eng.st, params, *fswitch);
 }
 
 int gcomcot_batch_remove_(int *sid) {

@@ -64,6 +64,30 @@ struct OutPack {
 };
 hipError_t launch_out_pack(const OutPack &p, hipStream_t st);
 
+// One fault plane of the Okada (1985) source as DEFORM_OKADA evaluates it
+// (deform.f90:74-191), in FP64: constants of the fault and, for spherical
+// layers, the stereographic projection split into per-row and per-column
+// factors (deform.f90:494-588 depends on the latitude and the longitude
+// separately until the final combination).
+struct FaultDev {
+  int nx, ny, sph;
+  double sin_t, cos_t, halfl, w, hh, del_x, del_y, sn, cs, ds, dd;
+  double two_rk0, sxi0, cxi0;  // spherical: 2 R K0, sin/cos of XI0
+  const double *row;           // spherical: {sin XI, cos XI} per row J
+  const double *col;           // spherical: {sin, cos}(LM - LM0) per column I
+  double dx, dy, xl, yl;       // Cartesian: spacing, projected grid origin
+};
+// deform = vertical seafloor displacement of every cell of the layer
+hipError_t launch_okada(const FaultDev &f, float *deform, hipStream_t st);
+// CELL_INTERP (all_grids.f90:859-934): a nested layer's deformation from its
+// parent's, in REAL; corners are the child's IS, IE, JS, JE in the parent
+hipError_t launch_cell_interp(const float *parent, int pnx, float *child, int cnx, int cny,
+                              const int *corners, int rel_size, hipStream_t st);
+// Z1 += D and H -= D (deform.f90:55-64, UPDATE_BATH initialization.f90:
+// 2410-2460); zraw != null also restarts MAX_AMP's maximum at the new Z1
+hipError_t launch_apply_deform(float *z1, float *h, const float *deform, float *zraw, size_t n,
+                               hipStream_t st);
+
 // Nesting: child boundary fluxes for sub-step t (1 = JNQ, >1 = NEWQ)
 struct DevNest {
   // parent

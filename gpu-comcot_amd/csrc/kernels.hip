@@ -1183,6 +1183,115 @@ __global__ void __launch_bounds__(256) k_out_pack(OutPack p) {
   }
 }
 
+// ---------------------------------------------------------------- source
+// STRIKE_SLIP (deform.f90:194-217) and DIP_SLIP (:221-238) at one corner of
+// the fault plane, FP64; sn, cs = sin/cos of the dip, dd = the depth.
+__device__ __forceinline__ double okada_strike(double x2, double y1, double y2, double sn,
+                                               double cs, double dd) {
+  const double q = x2 * sn - dd * cs;
+  const double d_bar = y2 * sn - q * cs;
+  const double r = sqrt(y1 * y1 + y2 * y2 + q * q);
+  double t1 = r + d_bar, t2 = r + y2;
+  if (t1 <= (double)kEps) t1 = (double)kEps;
+  if (t2 <= (double)kEps) t2 = (double)kEps;
+  const double a4 = 0.5 * 1 / cs * (log(t1) - sn * log(t2));
+  return -(d_bar * q / r / (r + y2) + q * sn / (r + y2) + a4 * sn) / 2 / (double)3.14159f;
+}
+
+__device__ __forceinline__ double okada_dip(double x2, double y1, double y2, double sn,
+                                            double cs, double dd) {
+  const double q = x2 * sn - dd * cs;
+  const double d_bar = y2 * sn - q * cs;
+  const double r = sqrt(y1 * y1 + y2 * y2 + q * q);
+  const double xx = sqrt(y1 * y1 + q * q);
+  const double a5 = 0.5 * 2 / cs * atan((y2 * (xx + q * cs) + xx * (r + xx) * sn) / y1 / (r + xx) / cs);
+  return -(d_bar * q / r / (r + y1) + sn * atan(y1 * y2 / q / r) - a5 * sn * cs) / 2 /
+         (double)3.14159f;
+}
+
+// DEFORM_OKADA's checks on each part (deform.f90:178-183), REAL limits
+__device__ __forceinline__ double okada_clean(double u) {
+  if (fabs(u) >= 3.4028234663852886e38) return 0.0;  // HUGE(REAL)
+  if (u != u) return 0.0;
+  if (fabs(u) <= (double)kGx) return 0.0;
+  return u;
+}
+
+__global__ void __launch_bounds__(256) k_okada(FaultDev f, float *deform) {
+  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (long long)f.nx * f.ny) return;
+  const int i = (int)(g % f.nx), j = (int)(g / f.nx);  // 0-based I-1, J-1
+  double xs, ys;
+  if (f.sph) {
+    const double sxi = f.row[2 * j], cxi = f.row[2 * j + 1];
+    const double sdl = f.col[2 * i], cdl = f.col[2 * i + 1];
+    const double beta = 1.0 + sxi * f.sxi0 + cxi * f.cxi0 * cdl;
+    const double y = f.two_rk0 * (sxi * f.cxi0 - cxi * f.sxi0 * cdl) / beta;
+    const double x = f.two_rk0 * cxi * sdl / beta;
+    xs = x - f.del_x;
+    ys = y + f.del_y;
+  } else {
+    const double yy = f.dy * (double)j, xx = f.dx * (double)i;
+    xs = xx - f.xl - f.del_x;
+    ys = yy - f.yl + f.del_y;
+  }
+  const double x1 = xs * f.sin_t + ys * f.cos_t;
+  const double x2 = -(xs * f.cos_t - ys * f.sin_t);
+  const double pp = x2 * f.cs + f.hh * f.sn;
+  const double a = x1 + f.halfl, b = x1 - f.halfl, pw = pp - f.w;
+  const double f1 = okada_strike(x2, a, pp, f.sn, f.cs, f.hh);
+  const double f2 = okada_strike(x2, a, pw, f.sn, f.cs, f.hh);
+  const double f3 = okada_strike(x2, b, pp, f.sn, f.cs, f.hh);
+  const double f4 = okada_strike(x2, b, pw, f.sn, f.cs, f.hh);
+  const double g1 = okada_dip(x2, a, pp, f.sn, f.cs, f.hh);
+  const double g2 = okada_dip(x2, a, pw, f.sn, f.cs, f.hh);
+  const double g3 = okada_dip(x2, b, pp, f.sn, f.cs, f.hh);
+  const double g4 = okada_dip(x2, b, pw, f.sn, f.cs, f.hh);
+  const double us = okada_clean((f1 - f2 - f3 + f4) * f.ds);
+  const double ud = okada_clean((g1 - g2 - g3 + g4) * f.dd);
+  deform[g] = (float)(us + ud);
+}
+
+// CELL_INTERP for one child cell (ci, cj), 1-based: the parent cell it lies
+// in, the four corner means and the two linear interpolations, in the
+// Fortran's REAL operation order; the ghost row/column stays 0.
+__global__ void __launch_bounds__(256) k_cell_interp(const float *__restrict__ pd, int pnx,
+                                                     float *cd, int cnx, int cny, int is, int ie,
+                                                     int js, int je, int ir) {
+  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (long long)cnx * cny) return;
+  const int ci = (int)(g % cnx) + 1, cj = (int)(g / cnx) + 1;
+  if (ci < 2 || cj < 2 || ci > (ie - is + 1) * ir + 1 || cj > (je - js + 1) * ir + 1) {
+    cd[g] = 0.0f;
+    return;
+  }
+  const int I = is + (ci - 2) / ir, K = (ci - 2) % ir + 1;
+  const int J = js + (cj - 2) / ir, M = (cj - 2) % ir + 1;
+  auto P = [&](int a, int b) { return pd[(size_t)(b - 1) * pnx + (a - 1)]; };
+  const float tl = (P(I, J) + P(I, J + 1) + P(I - 1, J) + P(I - 1, J + 1)) / 4.0f;
+  const float tr = (P(I, J) + P(I, J + 1) + P(I + 1, J) + P(I + 1, J + 1)) / 4.0f;
+  const float bl = (P(I, J) + P(I, J - 1) + P(I - 1, J) + P(I - 1, J - 1)) / 4.0f;
+  const float br = (P(I, J) + P(I, J - 1) + P(I + 1, J) + P(I + 1, J - 1)) / 4.0f;
+  const float nxf = (float)(2 * ir);
+  const float km = (float)(2 * M) - 1.0f;  // CELL(1, 2M) and CELL(NX+1, 2M)
+  const float cl = km * (tl - bl) / nxf + bl;
+  const float cr = km * (tr - br) / nxf + br;
+  const float ki = (float)(2 * K) - 1.0f;  // CELL(2K, 2M)
+  cd[g] = ki * (cr - cl) / nxf + cl;
+}
+
+__global__ void __launch_bounds__(256) k_apply_deform(float *z1, float *h,
+                                                      const float *__restrict__ d, float *zraw,
+                                                      long long n) {
+  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  const float dv = d[g];
+  h[g] = h[g] - dv;
+  const float z = z1[g] + dv;
+  z1[g] = z;
+  if (zraw) zraw[g] = z;
+}
+
 // Gauge samples (GET_TS, output.f90:1085-1094): the 2x2 stencil of the
 // current elevation weighted in the Fortran's operation order.
 __global__ void k_gauges(const GaugeDev *__restrict__ gs, int n, ZNow zs,
@@ -1493,6 +1602,31 @@ hipError_t launch_out_pack(const OutPack &p, hipStream_t st) {
   if (p.count <= 0) return hipSuccess;
   const long long blocks = std::min<long long>((p.count + 255) / 256, 8192);
   hipLaunchKernelGGL(k_out_pack, dim3((unsigned)blocks), dim3(256), 0, st, p);
+  return hipGetLastError();
+}
+
+hipError_t launch_okada(const FaultDev &f, float *deform, hipStream_t st) {
+  const long long n = (long long)f.nx * f.ny;
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_okada, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, f, deform);
+  return hipGetLastError();
+}
+
+hipError_t launch_cell_interp(const float *parent, int pnx, float *child, int cnx, int cny,
+                              const int *corners, int rel_size, hipStream_t st) {
+  const long long n = (long long)cnx * cny;
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_cell_interp, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, parent,
+                     pnx, child, cnx, cny, corners[0], corners[1], corners[2], corners[3],
+                     rel_size);
+  return hipGetLastError();
+}
+
+hipError_t launch_apply_deform(float *z1, float *h, const float *deform, float *zraw, size_t n,
+                               hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_apply_deform, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, z1, h,
+                     deform, zraw, (long long)n);
   return hipGetLastError();
 }
 

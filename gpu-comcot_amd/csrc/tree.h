@@ -34,6 +34,7 @@ struct LayerRec {
   float *zmax_raw = nullptr, *zmax_wet = nullptr, *coef = nullptr, *r24 = nullptr;
   float *dzp = nullptr, *xflux = nullptr, *yflux = nullptr;
   int *arrival = nullptr;
+  float *deform = nullptr;  // last seafloor deformation applied (on first use)
   int cur = 0;  // buffer index holding time level 1
   std::vector<int> children;
   size_t nxy() const { return (size_t)nx * ny; }

@@ -86,6 +86,7 @@ size_t Tree::device_bytes() const {
 static void free_layer(LayerRec &L) {
   if (L.block) (void)hipFree(L.block);
   if (L.r24) (void)hipFree(L.r24);
+  if (L.deform) (void)hipFree(L.deform);
   L = LayerRec();
 }
 

@@ -86,6 +86,19 @@ class Batch:
         self.specs[sid.value] = sc
         return sid.value
 
+    def fault(self, sid, params, switch=0):
+        """Seafloor deformation of one fault plane for scenario sid, applied on
+        the device (spherical layers get their coordinates from the spec)."""
+        L = lib()
+        for s in self.specs[sid].layers:
+            x, y = scenarios.layer_coords(s)
+            xa, xp = arr(x)
+            ya, yp = arr(y)
+            check(L.gcomcot_batch_set_coords_(ip(sid), ip(s.id), xp, yp), "batch coords")
+        pa = np.ascontiguousarray(params, np.float32)
+        check(L.gcomcot_batch_fault_deform_(ip(sid), pa.ctypes.data_as(ctypes.c_void_p),
+                                            ip(switch)), "batch fault")
+
     def run(self, nsteps):
         """Advance every unfinished scenario by up to nsteps; returns #finished."""
         nf = ctypes.c_int(0)

@@ -97,6 +97,37 @@ class DeviceRun:
                                       ctypes.byref(ms)), "autotune")
         return v.value, b.value, ms.value
 
+    # ------------------------------------------------------------ sources
+    def set_coords(self, lids=None):
+        """Give the library each layer's X (NX) / Y (NY) coordinates."""
+        L = lib()
+        for lid in (lids or sorted(self.specs)):
+            xa, xp = arr(self.coords(lid)[0])
+            ya, yp = arr(self.coords(lid)[1])
+            check(L.gcomcot_set_layer_coords_(ip(lid), xp, yp), f"coords of layer {lid}")
+
+    def fault(self, params, switch=0):
+        """Apply one fault plane's seafloor deformation on the device; params
+        are the FAULT record's 11 values HH, L, W, D, TH, DL, RD, YO, XO, Y0,
+        X0 (gcomcot.source.Fault.params())."""
+        pa = np.ascontiguousarray(params, np.float32)
+        if pa.size != 11:
+            raise GcomcotError("a fault has 11 parameters")
+        check(lib().gcomcot_fault_deform_(pa.ctypes.data_as(ctypes.c_void_p), ip(switch)),
+              "fault deform")
+
+    def deform(self, lid=1):
+        s = self.specs[lid]
+        out = _plane(s.ny, s.nx)
+        check(lib().gcomcot_get_deform_(ip(lid), out.ctypes.data_as(ctypes.c_void_p)), "deform")
+        return out
+
+    def depth(self, lid=1):
+        s = self.specs[lid]
+        out = _plane(s.ny, s.nx)
+        check(lib().gcomcot_get_depth_(ip(lid), out.ctypes.data_as(ctypes.c_void_p)), "depth")
+        return out
+
     # ------------------------------------------------------------ outputs
     def set_output(self, directory, iprt, out_opt=2, tend=0.0, tide=0.0, fluxswitch=None):
         """Reference-format files written by the device loop into `directory`:

@@ -178,6 +178,25 @@ int gcomcot_set_layer_coords_(int *lid, const float *x, const float *y);
 int gcomcot_gauges_set_(int *n, const float *x0, const float *y0, const int *lid, int *capacity);
 int gcomcot_gauges_read_(float *series, int *max_steps, int *nout);
 
+/* Seafloor deformation of one fault plane, computed on the device
+ * (reference: GET_FLOOR_DEFORM's fault-model branch, deform.f90:2-71, with
+ * DEFORM_OKADA :74-191, STRIKE_SLIP/DIP_SLIP :194-238, STEREO_PROJECTION
+ * :494-588, ININTERP/CELL_INTERP all_grids.f90:805-934, UPDATE_BATH
+ * initialization.f90:2410-2460).  params = the FAULT record's 11 REALs HH,
+ * L, W, D, TH, DL, RD, YO, XO, Y0, X0 (type_module.f90:130-147); *fswitch 0
+ * puts the epicentre at the plane's centre, 9 at the centre of its top edge.
+ * The Okada field of layer 1 is evaluated in FP64 (the reference: REAL),
+ * spread to nested layers in REAL as CELL_INTERP does, then every layer's
+ * level-1 elevation rises and its depth falls by it.  Call it before the
+ * first step for a source at T = 0 (the maximum then starts from the raised
+ * surface), or between gcomcot_run_steps_ calls at a later rupture time (the
+ * reference applies those on the host only, deform.f90:29-31).  A spherical
+ * layer 1 needs gcomcot_set_layer_coords_ first.  gcomcot_get_deform_ reads
+ * a layer's last deformation, gcomcot_get_depth_ its still-water depth H. */
+int gcomcot_fault_deform_(const float *params, int *fswitch);
+int gcomcot_get_deform_(int *lid, float *deform);
+int gcomcot_get_depth_(int *lid, float *h);
+
 /* Reference-format output files of the device loop (reference: ALL_PRT /
  * DATA_PRT, output.f90:737-851, at K = 0 and every IPRT root steps,
  * comcot.f90:308-310, 415-417; MAX_AMP, output.f90:855-1031, after every
@@ -272,6 +291,12 @@ void gcomcot_batch_set_arrival_threshold_(float *thr);
  * with 64x8 / 32x16 / 64x16 tiles, 3/4/5 = one workgroup per tile with
  * the same tiles */
 int gcomcot_batch_set_tuning_(int *autotune, int *shape);
+
+/* A fault source for a batched scenario (see gcomcot_fault_deform_), applied
+ * to its layers on the device; spherical roots need their coordinates
+ * first.  Call before the scenario's first step for a source at T = 0. */
+int gcomcot_batch_set_coords_(int *sid, int *lid, const float *x, const float *y);
+int gcomcot_batch_fault_deform_(int *sid, const float *params, int *fswitch);
 /* *on == 0: enqueue every root step instead of replaying HIP graphs (for
  * tools that cannot follow graph launches) */
 int gcomcot_batch_set_graphs_(int *on);

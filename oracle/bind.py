@@ -281,6 +281,25 @@ class Reference:
         files of layer lid at step k into the working directory."""
         self.lib().ref_data_prt(_i(k), _i(lid), _i(fluxswitch), _f(tide))
 
+    def okada(self, params, fswitch=0):
+        """The reference's DEFORM_OKADA on layer 1; returns its DEFORM."""
+        pa = np.ascontiguousarray(params, np.float32)
+        self.lib().ref_okada(_p(pa), _i(fswitch))
+        return self.deform(1)[0]
+
+    def floor_deform(self, params, fswitch=0):
+        """The reference's GET_FLOOR_DEFORM for one fault rupturing at T = 0."""
+        pa = np.ascontiguousarray(params, np.float32)
+        big_stack(self.lib().ref_floor_deform, _p(pa), _i(fswitch))
+
+    def deform(self, lid):
+        """(DEFORM, H) of layer lid as (ny, nx) REAL arrays."""
+        s = self.specs[lid]
+        d = np.empty(s.nx * s.ny, np.float32)
+        h = np.empty(s.nx * s.ny, np.float32)
+        self.lib().ref_deform_io(_i(lid), _p(d), _p(h))
+        return d.reshape(s.ny, s.nx), h.reshape(s.ny, s.nx)
+
     def max_amp(self, time, tend, tide=0.0):
         """The reference's MAX_AMP (output.f90:855-1031) at model time
         `time` (REAL): writes zmax files into the working directory when due."""

@@ -21,6 +21,7 @@
 !-----------------------------------------------------------------------
 module ref_state
   use layer_params
+  use fault_params
   implicit none
   type(layer), save :: lo
   type(layer), allocatable, save :: la(:)
@@ -396,6 +397,80 @@ subroutine ref_max_amp(time, tend, tide) bind(C, name="ref_max_amp")
   lo%tide_level = tide
   call max_amp(lo, la, time, tend)
 end subroutine ref_max_amp
+
+!.....one FAULT record from 11 REALs (HH, L, W, D, TH, DL, RD, YO, XO, Y0,
+!     X0) and the epicentre switch (0 centre, 9 top edge)
+subroutine ref_fault_fill(f, params, fswitch)
+  implicit none
+  type(fault) :: f
+  real :: params(11)
+  integer :: fswitch
+  f%hh = params(1)
+  f%l = params(2)
+  f%w = params(3)
+  f%d = params(4)
+  f%th = params(5)
+  f%dl = params(6)
+  f%rd = params(7)
+  f%yo = params(8)
+  f%xo = params(9)
+  f%y0 = params(10)
+  f%x0 = params(11)
+  f%t0 = 0.0
+  f%switch = fswitch
+  f%num_flt = 1
+  f%fs = 9          ! no WRITE_DEFORM file
+end subroutine ref_fault_fill
+
+!.....the reference's DEFORM_OKADA (deform.f90:74-191) on the outermost
+!     layer alone: LO%DEFORM, nothing applied
+subroutine ref_okada(params, fswitch) bind(C, name="ref_okada")
+  implicit none
+  real :: params(11)
+  integer :: fswitch
+  type(fault) :: f
+  call ref_fault_fill(f, params, fswitch)
+  call deform_okada(lo, f)
+end subroutine ref_okada
+
+!.....the reference's GET_FLOOR_DEFORM (deform.f90:2-71) for one fault
+!     rupturing at T = 0: DEFORM_OKADA, ININTERP to every nested layer,
+!     UPDATE_BATH, and the deformation added to every level-1 elevation
+subroutine ref_floor_deform(params, fswitch) bind(C, name="ref_floor_deform")
+  implicit none
+  real :: params(11)
+  integer :: fswitch
+  type(fault), allocatable :: flt(:)
+  real :: time
+  integer :: nflt
+  real :: elmax, grav, pi, r_earth, gx, eps, zero, one, v_limit, rad_deg, rad_min
+  integer :: ngrid
+  common /cons/ elmax, grav, pi, r_earth, gx, eps, zero, one, ngrid,  &
+                nflt, v_limit, rad_deg, rad_min
+  allocate(flt(nflt))
+  call ref_fault_fill(flt(1), params, fswitch)
+  time = 0.0
+  call get_floor_deform(lo, la, flt, time)
+  deallocate(flt)
+end subroutine ref_floor_deform
+
+!.....read DEFORM and H of layer id
+subroutine ref_deform_io(id, deform, h) bind(C, name="ref_deform_io")
+  implicit none
+  integer :: id
+  real :: deform(*), h(*)
+  integer :: k, n
+  call ref_pick(id, k)
+  if (k .eq. 0) then
+     n = lo%nx * lo%ny
+     deform(1:n) = reshape(lo%deform, (/n/))
+     h(1:n) = reshape(lo%h, (/n/))
+  else
+     n = la(k)%nx * la(k)%ny
+     deform(1:n) = reshape(la(k)%deform, (/n/))
+     h(1:n) = reshape(la(k)%h, (/n/))
+  end if
+end subroutine ref_deform_io
 end module ref_state
 
 !.....the CPU side of GCOMCOT_GET_Z (GPUOutput.cu:29-44) that GET_TS calls:

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "kernels.h"
+#include "source.h"
 
 using namespace gc;
 
@@ -183,6 +184,9 @@ hipError_t launch_step2(const DevLayer &L, bool, int, hipStream_t st) {
 hipError_t launch_gauges(const GaugeDev *, int, const ZNow &, const int *, float *, int,
                          hipStream_t) {
   return hipSuccess;  // gauges belong to the single-tree device loop, not the batch planner
+}
+int tree_apply_fault(Tree &, hipStream_t, const float *, int) {
+  return 1;  // fault sources run on the device only; the planner tests do not use them
 }
 int num_tile_variants() { return 1; }
 void tile_variant_shape(int, int *tx, int *ty) { *tx = 64; *ty = 8; }
