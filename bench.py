@@ -74,6 +74,8 @@ def parse():
                     help="time the device loop with reference-format output files (0 = skip)")
     ap.add_argument("--outputs-steps", type=int, default=1000)
     ap.add_argument("--outputs-iprt", type=int, default=250)
+    ap.add_argument("--source", type=int, default=1,
+                    help="time a fault source on the headline grid (0 = skip)")
     ap.add_argument("--accuracy", type=int, default=1,
                     help="with the CPU baseline: FP32 vs FP64-oracle error on a reduced grid")
     ap.add_argument("--graphs", type=int, default=1,
@@ -235,6 +237,50 @@ def bench_outputs(a):
     return out
 
 
+def bench_source(a):
+    """A fault source on the headline grid: the library's device Okada
+    (FP64) + spreading + update, wall time of gcomcot_fault_deform_ (second
+    call; the first allocates the deformation planes), against the
+    reference's own DEFORM_OKADA (flang, REAL, one core) on a 512x384 sample
+    of the same kind of layer."""
+    from gcomcot import DeviceRun, config1
+    from gcomcot import scenarios as S
+    from gcomcot.source import Fault
+    sc = config1(nx=a.nx, ny=a.ny, steps=1)
+    L = sc.layers[0]
+    x, y = S.layer_coords(L)
+    f = Fault(10000, 200000, 80000, 6, 30, 15, 90, x0=float(x[L.nx // 2]), y0=float(y[L.ny // 2]))
+    run = DeviceRun(sc)
+    run.set_coords()
+    log(f"source: Okada on {L.nx}x{L.ny}")
+    walls = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        run.fault(f.params(), f.switch)
+        walls.append(time.perf_counter() - t0)
+    run.close()
+    cells = L.nx * L.ny
+    ms = min(walls[1:]) * 1e3
+    out = {"workload": f"one fault plane on {L.nx}x{L.ny} spherical", "ms": round(ms, 3),
+           "cells_per_s": round(cells / (ms * 1e-3), 1), "precision": "f64"}
+    try:
+        from oracle.bind import Reference
+        if Reference.available():
+            small = S.custom(laycord=0, laygov=0, nx=512, ny=384, dx=L.dx)
+            sx, sy = S.layer_coords(small.layers[0])
+            fs = Fault(10000, 200000, 80000, 6, 30, 15, 90, x0=float(sx[256]), y0=float(sy[192]))
+            ref = Reference(small)
+            ref.set_coords(1, sx, sy)
+            t0 = time.perf_counter()
+            ref.okada(fs.params(), fs.switch)
+            w = time.perf_counter() - t0
+            out["cpu_reference"] = {"sample": "DEFORM_OKADA on 512x384, flang -O2, 1 core",
+                                    "cells_per_s": round(512 * 384 / w, 1)}
+    except Exception as e:  # the CPU leg is a reported baseline, not the product
+        out["cpu_reference"] = {"error": str(e)[:200]}
+    return out
+
+
 def bench_batch(a):
     from gcomcot import DeviceRun
     from gcomcot import scenarios as S
@@ -366,6 +412,7 @@ def main():
             cpu["accuracy"] = cpu_accuracy()
     others = bench_other_configs(a) if a.others else None
     outputs = bench_outputs(a) if a.outputs else None
+    source = bench_source(a) if a.source else None
     batch = bench_batch(a) if a.batch else None
     serve = bench_serve(a) if a.serve else None
     copy = copy_probe(1024)
@@ -396,6 +443,7 @@ def main():
         "hbm_copy": copy,
         "other_configs": others,
         "outputs": outputs,
+        "source": source,
         "batch": batch,
         "serve": serve,
     }

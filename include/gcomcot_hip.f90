@@ -182,6 +182,26 @@ module gcomcot_hip
       integer(c_int) :: mode
     end function
 
+    ! Fault sources on the device (GET_FLOOR_DEFORM's Okada branch)
+    integer(c_int) function gcomcot_fault_deform(params, fswitch) &
+        bind(C, name="gcomcot_fault_deform_")
+      import :: c_int, c_float
+      real(c_float) :: params(11)
+      integer(c_int) :: fswitch
+    end function
+
+    integer(c_int) function gcomcot_get_deform(lid, deform) bind(C, name="gcomcot_get_deform_")
+      import :: c_int, c_float
+      integer(c_int) :: lid
+      real(c_float) :: deform(*)
+    end function
+
+    integer(c_int) function gcomcot_get_depth(lid, h) bind(C, name="gcomcot_get_depth_")
+      import :: c_int, c_float
+      integer(c_int) :: lid
+      real(c_float) :: h(*)
+    end function
+
     ! Reference-format output files of the device loop (DATA_PRT, MAX_AMP)
     integer(c_int) function gcomcot_output_set(dir, dirlen, iprt, out_opt, tend, tide) &
         bind(C, name="gcomcot_output_set_")
