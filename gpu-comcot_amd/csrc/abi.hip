@@ -24,6 +24,13 @@ namespace {
 
 Tree g_tree;
 bool g_init = false;
+// Faults waiting for their rupture time in the device loop.
+struct PendingFault {
+  float params[11];
+  int fswitch;
+  float t0;
+};
+std::vector<PendingFault> g_faults;
 hipStream_t g_stream = nullptr;
 
 int ensure_init() {
@@ -53,6 +60,7 @@ int create_layer(int id, int parent, int level, int laycord, int laygov, int rel
   if (int e = ensure_init()) return e;
   if (id == 1 && out_active())  // a new tree: queued files are finished first
     if (int e = out_disable(g_tree)) return e;
+  if (id == 1) g_faults.clear();
   return tree_create_layer(g_tree, g_stream, id, parent, level, laycord, laygov, rel_size,
                            rel_time, corners, fric_switch, fric_coef, dt, dx, dy, nx, ny, H, Z);
 }
@@ -289,29 +297,99 @@ static int run_plain(int n, int use_graph) {
   return sync();
 }
 
+// TIME of the main loop after root step k (comcot.f90:304, 323: REAL,
+// one DT added per step), cached for the current layer 1
+static long long g_clk_k = 0;
+static float g_clk_t = 0.0f, g_clk_dt = -1.0f;
+
+static float loop_time(long long k) {
+  const float dt = g_tree.layers[1].dt;
+  if (dt != g_clk_dt || k < g_clk_k) {
+    g_clk_dt = dt;
+    g_clk_k = 0;
+    g_clk_t = 0.0f;
+  }
+  while (g_clk_k < k) {
+    g_clk_t = g_clk_t + dt;
+    ++g_clk_k;
+  }
+  return g_clk_t;
+}
+
+// GET_FLOOR_DEFORM's test (deform.f90:29): a fault ruptures at the call
+// with TIME <= T0 < TIME + DT.  The call before the loop has TIME = 0
+// (initialization.f90:476); the one of root step k, before its MASS, has
+// the TIME after step k (comcot.f90:323, 341).
+static bool fault_fires(const PendingFault &f, long long k) {
+  const float t = loop_time(k), dt = g_tree.layers[1].dt;
+  return f.t0 >= t && f.t0 < t + dt;
+}
+
+// Apply (and drop) every pending fault that fires at call k.
+static int fire_faults(long long k) {
+  for (size_t i = 0; i < g_faults.size();) {
+    if (fault_fires(g_faults[i], k)) {
+      if (int r = tree_apply_fault(g_tree, g_stream, g_faults[i].params, g_faults[i].fswitch))
+        return r;
+      g_faults.erase(g_faults.begin() + (long)i);
+    } else {
+      ++i;
+    }
+  }
+  return 0;
+}
+
 int gcomcot_run_steps_(int *nsteps, int *use_graph) {
   int n = *nsteps;
   if (n < 0) return fail(23, "negative step count");
   if (!layer(1)) return fail(8, "no root layer");
   if (int e = ensure_init()) return e;
-  if (!out_active()) return run_plain(n, *use_graph);
-  // the run is cut at every root step that writes files; the files of a
-  // step are packed on the step stream and written while the loop goes on
+  if (!out_active() && g_faults.empty()) return run_plain(n, *use_graph);
+  // the run is cut at every root step that writes files or before which a
+  // fault ruptures; files are packed on the step stream and written while
+  // the loop goes on
   if (sync()) return 1;
   int done = 0;
   HIPCHK(hipMemcpy(&done, g_tree.ctl + g_tree.ctl_in, sizeof(int), hipMemcpyDeviceToHost));
-  if (done == 0 && out_due(g_tree, 0))
-    if (int r = out_write(g_tree, 0, g_stream)) return r;
+  if (done == 0) {
+    if (int r = fire_faults(0)) return r;
+    if (out_due(g_tree, 0))
+      if (int r = out_write(g_tree, 0, g_stream)) return r;
+  }
   while (n > 0) {
-    const int m = (int)out_steps_to_next(g_tree, done, n);
-    if (int r = run_plain(m, *use_graph)) return r;
-    done += m;
-    n -= m;
+    if (int r = fire_faults(done + 1)) return r;
+    long long m = out_active() ? out_steps_to_next(g_tree, done, n) : n;
+    for (long long k = done + 2; k <= done + m && !g_faults.empty(); ++k) {
+      bool any = false;
+      for (const PendingFault &f : g_faults) any = any || fault_fires(f, k);
+      if (any) {
+        m = k - 1 - done;
+        break;
+      }
+    }
+    if (int r = run_plain((int)m, *use_graph)) return r;
+    done += (int)m;
+    n -= (int)m;
     if (out_due(g_tree, done))
       if (int r = out_write(g_tree, done, g_stream)) return r;
   }
   return sync();
 }
+
+int gcomcot_fault_schedule_(const float *params, int *fswitch, float *t0) {
+  if (*fswitch != 0 && *fswitch != 9) return fail(60, "fault: epicentre switch %d not 0 or 9", *fswitch);
+  for (int k = 0; k < 11; ++k)
+    if (!std::isfinite(params[k])) return fail(61, "fault: parameter %d is not finite", k + 1);
+  if (!std::isfinite(*t0)) return fail(61, "fault: rupture time is not finite");
+  PendingFault f;
+  std::memcpy(f.params, params, sizeof f.params);
+  f.fswitch = *fswitch;
+  f.t0 = *t0;
+  g_faults.push_back(f);
+  return 0;
+}
+
+int gcomcot_fault_pending_(void) { return (int)g_faults.size(); }
 
 int gcomcot_output_set_(const char *dir, int *dirlen, int *iprt, int *out_opt, float *tend,
                         float *tide) {

@@ -116,6 +116,18 @@ class DeviceRun:
         check(lib().gcomcot_fault_deform_(pa.ctypes.data_as(ctypes.c_void_p), ip(switch)),
               "fault deform")
 
+    def schedule_fault(self, params, switch=0, t0=0.0):
+        """A fault that ruptures at model time t0 during run(): applied where
+        the reference's loop calls GET_FLOOR_DEFORM with TIME <= t0 < TIME + DT."""
+        pa = np.ascontiguousarray(params, np.float32)
+        if pa.size != 11:
+            raise GcomcotError("a fault has 11 parameters")
+        check(lib().gcomcot_fault_schedule_(pa.ctypes.data_as(ctypes.c_void_p), ip(switch),
+                                            fp(t0)), "fault schedule")
+
+    def pending_faults(self):
+        return int(lib().gcomcot_fault_pending_())
+
     def deform(self, lid=1):
         s = self.specs[lid]
         out = _plane(s.ny, s.nx)

@@ -190,6 +190,13 @@ module gcomcot_hip
       integer(c_int) :: fswitch
     end function
 
+    integer(c_int) function gcomcot_fault_schedule(params, fswitch, t0) &
+        bind(C, name="gcomcot_fault_schedule_")
+      import :: c_int, c_float
+      real(c_float) :: params(11), t0
+      integer(c_int) :: fswitch
+    end function
+
     integer(c_int) function gcomcot_get_deform(lid, deform) bind(C, name="gcomcot_get_deform_")
       import :: c_int, c_float
       integer(c_int) :: lid

@@ -194,6 +194,15 @@ int gcomcot_gauges_read_(float *series, int *max_steps, int *nout);
  * layer 1 needs gcomcot_set_layer_coords_ first.  gcomcot_get_deform_ reads
  * a layer's last deformation, gcomcot_get_depth_ its still-water depth H. */
 int gcomcot_fault_deform_(const float *params, int *fswitch);
+/* A fault that ruptures at model time *t0 inside gcomcot_run_steps_: it is
+ * applied, on the device, at the call of GET_FLOOR_DEFORM that the
+ * reference's loop would make with TIME <= T0 < TIME + DT (before the loop
+ * with TIME = 0, else before root step k with TIME = k DT accumulated in
+ * REAL; deform.f90:29, comcot.f90:323, 341, initialization.f90:476).  A new
+ * layer 1 clears the schedule; gcomcot_fault_pending_ counts faults still
+ * waiting. */
+int gcomcot_fault_schedule_(const float *params, int *fswitch, float *t0);
+int gcomcot_fault_pending_(void);
 int gcomcot_get_deform_(int *lid, float *deform);
 int gcomcot_get_depth_(int *lid, float *h);
 

@@ -21,6 +21,7 @@ from gcomcot import scenarios as S
 from gcomcot.batch import Batch
 from helpers import BEACH, diff_report
 from oracle import okada as OK
+from oracle import outfiles as OF
 from oracle.bind import Oracle
 
 pytestmark = pytest.mark.gpu
@@ -143,6 +144,54 @@ def test_batched_scenario_with_a_source_equals_the_single_run():
             assert nb == 0, f"layer {L.id} {k}: {nb} cells differ (max {mx})"
     assert b.progress(other) == (n, n)
     b.close()
+
+
+def _firing_call(dt, t0, kmax):
+    """The GET_FLOOR_DEFORM call at which a rupture at t0 applies: 0 before
+    the loop (TIME = 0), k before root step k (TIME after step k, REAL);
+    deform.f90:29, comcot.f90:323, 341."""
+    times = OF.model_times(dt, kmax)
+    t0 = np.float32(t0)
+    for k, t in enumerate(times):
+        if t0 >= t and t0 < np.float32(t + np.float32(dt)):
+            return k
+    return None
+
+
+def test_scheduled_ruptures_apply_at_the_reference_steps():
+    sc = _nest()
+    x, y = S.layer_coords(sc.layers[0])
+    dt = np.float32(sc.layers[0].dt)
+    faults = _faults(float(x[45]), float(y[38]))
+    t0s = [np.float32(0.5) * dt, np.float32(7.5) * dt, np.float32(12.0) * dt]
+    n = 20
+    calls = [_firing_call(dt, t, n) for t in t0s]
+    assert calls[0] == 0 and all(c is not None for c in calls)
+    print(f"[gpu] scheduled ruptures at calls {calls}", flush=True)
+    dev = DeviceRun(sc)
+    dev.set_coords()
+    for f, t in zip(faults, t0s):
+        dev.schedule_fault(f.params(), f.switch, float(t))
+    dev.run(8)
+    dev.run(n - 8, graph=False)
+    assert dev.pending_faults() == 0
+    sched = {L.id: dev.state(L.id) for L in sc.layers}
+    dev.close()
+    dev = DeviceRun(sc)   # the same by hand: each fault before its step
+    dev.set_coords()
+    done = 0
+    for f, k in sorted(zip(faults, calls), key=lambda p: p[1]):
+        if k - 1 > done:
+            dev.run(k - 1 - done)
+            done = k - 1
+        dev.fault(f.params(), f.switch)
+    dev.run(n - done)
+    for L in sc.layers:
+        st = dev.state(L.id)
+        for key in ("z1", "m1", "n1"):
+            nb, mx = diff_report(sched[L.id][key], st[key])
+            assert nb == 0, f"layer {L.id} {key}: {nb} cells differ (max {mx})"
+    dev.close()
 
 
 def test_fault_errors_are_reported():
