@@ -398,6 +398,26 @@ int gcomcot_output_set_(const char *dir, int *dirlen, int *iprt, int *out_opt, f
   return out_configure(g_tree, dir, *dirlen, *iprt, *out_opt, *tend, *tide);
 }
 
+int gcomcot_sponge_set_(int *on) {
+  LayerRec *R = layer(1);
+  if (!R) return fail(8, "no root layer");
+  if (int e = ensure_init()) return e;
+  if (sync()) return 1;
+  if (R->sponge) HIPCHK(hipFree(R->sponge));
+  R->sponge = nullptr;
+  if (!*on) return 0;
+  if ((int)g_tree.xs[1].size() != R->nx || (int)g_tree.ys[1].size() != R->ny)
+    return fail(42, "sponge: coordinates of layer 1 not set");
+  std::vector<float> h(R->nxy()), c((size_t)R->nx + R->ny);
+  HIPCHK(hipMemcpy(h.data(), R->h, h.size() * sizeof(float), hipMemcpyDeviceToHost));
+  int laycord = R->laycord;
+  gcomcot_sponge_coef_(&laycord, &R->nx, &R->ny, h.data(), g_tree.xs[1].data(),
+                       g_tree.ys[1].data(), c.data(), c.data() + R->nx);
+  HIPCHK(hipMalloc(&R->sponge, c.size() * sizeof(float)));
+  HIPCHK(hipMemcpy(R->sponge, c.data(), c.size() * sizeof(float), hipMemcpyHostToDevice));
+  return 0;
+}
+
 int gcomcot_fault_deform_(const float *params, int *fswitch) {
   if (!layer(1)) return fail(8, "no root layer");
   if (int e = ensure_init()) return e;

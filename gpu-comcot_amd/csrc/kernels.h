@@ -32,6 +32,9 @@ struct DevLayer {
   // MAX_AMP's maximum (GPUOutput.cu:46-53: no wet test, NaN replaces),
   // fused into the step of the outermost layer when != null
   float *zraw;
+  // SPONGE_LAYER's coefficients COEFX per column I and COEFY per row J
+  // (boundaries.f90:206-246), applied after the momentum pass when != null
+  const float *sponge_cx, *sponge_cy;
 };
 
 enum StepMode { kFused = 0, kMassOnly = 1, kMomentOnly = 2 };

@@ -596,6 +596,7 @@ __device__ __forceinline__ void tile_body(const DevLayer &L, int t, int ti0, int
     }
     if (MODE != kMassOnly) {
       const bool r = L.is_root;
+      float m2v, n2v;
       // Cells outside a routine's range are never written by the Fortran,
       // so its level 2 holds 0 there (ALLOC) and the level copy carries that
       // 0 into level 1; writing 0 keeps the swapped buffers identical to it.
@@ -606,16 +607,25 @@ __device__ __forceinline__ void tile_body(const DevLayer &L, int t, int ti0, int
         const bool my =
             IN || (r ? (i <= nx - 1 && j <= ny - 1) : (i >= 2 && j >= 2 && j <= ny - 1));
         const float brk = L.linchk == 1 ? breaking_coef(L, s, i, j) : 1.0f;
-        L.m2[g] = mx ? nl_x<SPH, IN>(L, s, i, j, brk) : 0.0f;
-        L.n2[g] = my ? nl_y<SPH, IN>(L, s, i, j, brk) : 0.0f;
+        m2v = mx ? nl_x<SPH, IN>(L, s, i, j, brk) : 0.0f;
+        n2v = my ? nl_y<SPH, IN>(L, s, i, j, brk) : 0.0f;
       } else {
         // MOMT_S / MOMT_C ranges (moment.f90:78-88, 117-119, 174-192, 233-235)
         const int lo = r ? 1 : 2;
         const bool mx = IN || (i >= lo && i <= nx - 1 && j >= lo);
         const bool my = IN || (j >= lo && j <= ny - 1 && i >= lo);
-        L.m2[g] = mx ? lin_x<SPH, IN>(L, s, i, j) : 0.0f;
-        L.n2[g] = my ? lin_y<SPH, IN>(L, s, i, j) : 0.0f;
+        m2v = mx ? lin_x<SPH, IN>(L, s, i, j) : 0.0f;
+        n2v = my ? lin_y<SPH, IN>(L, s, i, j) : 0.0f;
       }
+      if (L.sponge_cx) {  // SPONGE_LAYER (boundaries.f90:228-239)
+        const float cx = L.sponge_cx[i - 1], cy = L.sponge_cy[j - 1];
+        if (cx > 0.0f || cy > 0.0f) {
+          m2v = m2v + cx * L.dt * s.M(i, j);
+          n2v = n2v + cy * L.dt * s.N(i, j);
+        }
+      }
+      L.m2[g] = m2v;
+      L.n2[g] = n2v;
       if (L.zmax && (IN || r || (i >= 2 && j >= 2))) {
         const float z2 = s.Z2(i, j);
         if (s.H(i, j) + z2 > kGx) {
@@ -913,19 +923,29 @@ __global__ void __launch_bounds__(kThreads)
       else zo = r.Z2(i, 2);
       L.z2[g] = zo;
       r.z1own = zq0;
+      float m2v, n2v;
       if (NL) {
         const bool mx = rt ? (i <= nx - 1 && j <= ny - 1) : (i >= 2 && i <= nx - 1 && j >= 2);
         const bool my = rt ? (i <= nx - 1 && j <= ny - 1) : (i >= 2 && j >= 2 && j <= ny - 1);
         const float brk = L.linchk == 1 ? breaking_coef(L, r, i, j) : 1.0f;
-        L.m2[g] = mx ? nl_x<SPH, false>(L, r, i, j, brk) : 0.0f;
-        L.n2[g] = my ? nl_y<SPH, false>(L, r, i, j, brk) : 0.0f;
+        m2v = mx ? nl_x<SPH, false>(L, r, i, j, brk) : 0.0f;
+        n2v = my ? nl_y<SPH, false>(L, r, i, j, brk) : 0.0f;
       } else {
         const int lo = rt ? 1 : 2;
         const bool mx = i >= lo && i <= nx - 1 && j >= lo;
         const bool my = j >= lo && j <= ny - 1 && i >= lo;
-        L.m2[g] = mx ? lin_x<SPH, false>(L, r, i, j) : 0.0f;
-        L.n2[g] = my ? lin_y<SPH, false>(L, r, i, j) : 0.0f;
+        m2v = mx ? lin_x<SPH, false>(L, r, i, j) : 0.0f;
+        n2v = my ? lin_y<SPH, false>(L, r, i, j) : 0.0f;
       }
+      if (L.sponge_cx) {
+        const float cx = L.sponge_cx[i - 1], cy = L.sponge_cy[j - 1];
+        if (cx > 0.0f || cy > 0.0f) {
+          m2v = m2v + cx * L.dt * r.M(i, j);
+          n2v = n2v + cy * L.dt * r.N(i, j);
+        }
+      }
+      L.m2[g] = m2v;
+      L.n2[g] = n2v;
       if (L.zmax && (rt || (i >= 2 && j >= 2))) {
         const float z2 = r.Z2(i, j);
         if (r.H(i, j) + z2 > kGx) {

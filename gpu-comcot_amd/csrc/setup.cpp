@@ -98,4 +98,61 @@ void gcomcot_child_dt_(const int *laycord, const int *laygov, const int *nx, con
   }
 }
 
+// SPONGE_COEF (boundaries.f90:250-330) with H_CALC (:334-365), in REAL:
+// the damping coefficients of the sponge layer.  COEFX depends on X(I)
+// only and COEFY on Y(J) only, so they come out as cx[NX] and cy[NY].
+// The mean wet depth is summed in the Fortran loop order (I outer, J
+// inner).  For spherical layers the Fortran compares a width in degrees
+// with R_MAX in metres, which leaves every coefficient at 0; that is kept.
+void gcomcot_sponge_coef_(const int *laycord, const int *nx, const int *ny, const float *h,
+                          const float *x, const float *y, float *cx, float *cy) {
+  const int NX = *nx, NY = *ny;
+  float h_sum = 0.0f;
+  int k = 0;
+  for (int i = 0; i < NX; ++i)
+    for (int j = 0; j < NY; ++j) {
+      const float hv = h[(size_t)j * NX + i];
+      if (hv > kGx) {
+        h_sum = h_sum + hv;
+        ++k;
+      }
+    }
+  float depth = k > 0 ? h_sum / (float)k : 0.0f;
+  if (depth <= kGx) depth = kGx;
+  const float wavelength = 20.0f * depth;
+  const float width = 2.0f * wavelength;
+  const float alpha_c = 10.0f;
+  const float pi = 3.14159265358979f;
+  const float t = wavelength / sqrtf(kGrav * depth);
+  const float omega = 2.0f * pi / t;
+  const float sponge_width = *laycord == 0 ? width / kREarth * 180.0f / pi : width;
+  const float r_max = width;
+  const float xs = x[0] + sponge_width, ys = y[0] + sponge_width;
+  const float xe = x[NX - 1] - sponge_width, ye = y[NY - 1] - sponge_width;
+  const float e1 = expf(1.0f) - 1.0f;
+  auto coef = [&](float v, float lo, float hi) {
+    float r = 0.0f;
+    if (v <= lo) r = -(v - lo);
+    else if (v >= hi) r = v - hi;
+    float rel = r;
+    if (rel <= 0.0f) rel = 0.0f;
+    if (rel >= r_max) rel = r_max;
+    const float q = rel / r_max;
+    float c = alpha_c * omega * (expf(q * q) - 1.0f) / e1;
+    if (c <= kEps) c = 0.0f;
+    return c;
+  };
+  for (int i = 0; i < NX; ++i) cx[i] = coef(x[i], xs, xe);
+  for (int j = 0; j < NY; ++j) cy[j] = coef(y[j], ys, ye);
+}
+
+// BC_WALL (boundaries.f90:369-401, the branch without a wave maker): the
+// two outermost rows and columns become land (H = -999).
+void gcomcot_bc_wall_(float *h, const int *nx, const int *ny) {
+  const int NX = *nx, NY = *ny;
+  for (int j = 0; j < NY; ++j)
+    for (int i = 0; i < NX; ++i)
+      if (i < 2 || j < 2 || i >= NX - 2 || j >= NY - 2) h[(size_t)j * NX + i] = -999.0f;
+}
+
 }  // extern "C"

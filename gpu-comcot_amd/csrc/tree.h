@@ -35,6 +35,7 @@ struct LayerRec {
   float *dzp = nullptr, *xflux = nullptr, *yflux = nullptr;
   int *arrival = nullptr;
   float *deform = nullptr;  // last seafloor deformation applied (on first use)
+  float *sponge = nullptr;  // SPONGE_COEF: COEFX[NX] then COEFY[NY] (layer 1, when on)
   int cur = 0;  // buffer index holding time level 1
   std::vector<int> children;
   size_t nxy() const { return (size_t)nx * ny; }

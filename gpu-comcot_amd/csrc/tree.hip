@@ -87,6 +87,7 @@ static void free_layer(LayerRec &L) {
   if (L.block) (void)hipFree(L.block);
   if (L.r24) (void)hipFree(L.r24);
   if (L.deform) (void)hipFree(L.deform);
+  if (L.sponge) (void)hipFree(L.sponge);
   L = LayerRec();
 }
 
@@ -273,6 +274,8 @@ DevLayer dev_view(const Tree &T, const LayerRec &L, bool with_max, bool advance)
   d.advance = advance ? 1 : 0;
   d.arr_thr = T.arr_thr;
   d.zraw = (T.raw_max && with_max && advance && L.id == 1) ? L.zmax_raw : nullptr;
+  d.sponge_cx = (L.sponge && L.id == 1) ? L.sponge : nullptr;
+  d.sponge_cy = d.sponge_cx ? L.sponge + L.nx : nullptr;
   return d;
 }
 
@@ -362,8 +365,10 @@ int tree_enqueue_root_step(Tree &T, hipStream_t st) {
 
 bool tree_step2_eligible(const Tree &T) {
   const LayerRec &R = T.layers[1];
-  // gauges sample every step, so they keep one step per launch
-  return T.blocking > 0 && R.used && R.children.empty() && !R.nl && T.ngauges == 0;
+  // gauges sample every step and the sponge acts between steps, so both
+  // keep one step per launch
+  return T.blocking > 0 && R.used && R.children.empty() && !R.nl && T.ngauges == 0 &&
+         !R.sponge;
 }
 
 // GET_TS's stencil search over X(IS..IE) (output.f90:1062-1069, 1103-1110)

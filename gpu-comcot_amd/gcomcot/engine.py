@@ -116,6 +116,11 @@ class DeviceRun:
         check(lib().gcomcot_fault_deform_(pa.ctypes.data_as(ctypes.c_void_p), ip(switch)),
               "fault deform")
 
+    def sponge(self, on=True):
+        """SPONGE_LAYER on layer 1 after every momentum pass (coefficients from
+        its depth and coordinates; call set_coords first)."""
+        check(lib().gcomcot_sponge_set_(ip(1 if on else 0)), "sponge")
+
     def schedule_fault(self, params, switch=0, t0=0.0):
         """A fault that ruptures at model time t0 during run(): applied where
         the reference's loop calls GET_FLOOR_DEFORM with TIME <= t0 < TIME + DT."""

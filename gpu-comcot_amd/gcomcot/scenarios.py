@@ -422,3 +422,15 @@ def custom(nx=64, ny=56, dx=2.0, laycord=0, laygov=0, fric_coef=0.013, child=Non
                           corners=(IS, IE, JS, JE))
         layers.append(C)
     return Scenario(f"custom-{laycord}{laygov}-{nx}x{ny}", layers, steps, {"seed": seed})
+
+
+def bc_wall(h):
+    """BC_WALL (boundaries.f90:369-401) through the library: a copy of the
+    depth h (ny, nx) with its two outermost rows and columns turned to land."""
+    import ctypes
+    from ._lib import lib
+    out = np.ascontiguousarray(h, np.float32).copy()
+    ny, nx = out.shape
+    lib().gcomcot_bc_wall_(out.ctypes.data_as(ctypes.c_void_p), ctypes.byref(ctypes.c_int(nx)),
+                           ctypes.byref(ctypes.c_int(ny)))
+    return out

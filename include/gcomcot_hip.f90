@@ -182,6 +182,25 @@ module gcomcot_hip
       integer(c_int) :: mode
     end function
 
+    ! Boundary treatments of the outermost layer (SPONGE_COEF/SPONGE_LAYER, BC_WALL)
+    subroutine gcomcot_sponge_coef(laycord, nx, ny, h, x, y, cx, cy) &
+        bind(C, name="gcomcot_sponge_coef_")
+      import :: c_int, c_float
+      integer(c_int) :: laycord, nx, ny
+      real(c_float) :: h(*), x(*), y(*), cx(*), cy(*)
+    end subroutine
+
+    integer(c_int) function gcomcot_sponge_set(on) bind(C, name="gcomcot_sponge_set_")
+      import :: c_int
+      integer(c_int) :: on
+    end function
+
+    subroutine gcomcot_bc_wall(h, nx, ny) bind(C, name="gcomcot_bc_wall_")
+      import :: c_int, c_float
+      real(c_float) :: h(*)
+      integer(c_int) :: nx, ny
+    end subroutine
+
     ! Fault sources on the device (GET_FLOOR_DEFORM's Okada branch)
     integer(c_int) function gcomcot_fault_deform(params, fswitch) &
         bind(C, name="gcomcot_fault_deform_")

@@ -178,6 +178,25 @@ int gcomcot_set_layer_coords_(int *lid, const float *x, const float *y);
 int gcomcot_gauges_set_(int *n, const float *x0, const float *y0, const int *lid, int *capacity);
 int gcomcot_gauges_read_(float *series, int *max_steps, int *nout);
 
+/* Boundary treatments of the outermost layer.
+ * gcomcot_sponge_coef_ is SPONGE_COEF (boundaries.f90:250-330, with H_CALC
+ * :334-365) on the host, in REAL: cx[NX] = COEFX(I, .), cy[NY] = COEFY(., J)
+ * (each depends on one coordinate only) from the layer's H (NX*NY,
+ * column-major) and coordinates X(NX), Y(NY).
+ * gcomcot_sponge_set_(*on) makes the device loop damp the fluxes of layer 1
+ * after every momentum pass as SPONGE_LAYER does (boundaries.f90:206-246:
+ * M2 += COEFX*DT*M1, N2 += COEFY*DT*N1 where either coefficient is > 0),
+ * with coefficients from its current depth and its coordinates
+ * (gcomcot_set_layer_coords_); the reference's GPU build does not run it
+ * (comcot.f90:403-406).  Linear layers then take one step per launch.
+ * gcomcot_bc_wall_ is BC_WALL (boundaries.f90:369-401) on a host depth
+ * array: the two outermost rows and columns become land (H = -999), as the
+ * reference applies it before the layers are created (comcot.f90:221). */
+void gcomcot_sponge_coef_(const int *laycord, const int *nx, const int *ny, const float *h,
+                          const float *x, const float *y, float *cx, float *cy);
+int gcomcot_sponge_set_(int *on);
+void gcomcot_bc_wall_(float *h, const int *nx, const int *ny);
+
 /* Seafloor deformation of one fault plane, computed on the device
  * (reference: GET_FLOOR_DEFORM's fault-model branch, deform.f90:2-71, with
  * DEFORM_OKADA :74-191, STRIKE_SLIP/DIP_SLIP :194-238, STEREO_PROJECTION

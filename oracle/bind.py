@@ -95,6 +95,7 @@ class Oracle:
             L.orc_flux_io.argtypes = [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 2
             L.orc_max_io.argtypes = [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 3
             L.orc_set_arrival_threshold.argtypes = [ctypes.c_void_p, ctypes.c_double]
+            L.orc_set_sponge.argtypes = [ctypes.c_void_p] * 3
             L.orc_cr_dt.restype = ctypes.c_double
             L.orc_cr_dt.argtypes = [ctypes.c_int] * 4 + [ctypes.c_void_p] + [ctypes.c_double] * 5
             cls._lib = L
@@ -126,6 +127,13 @@ class Oracle:
 
     def step(self, n=1):
         self.lib().orc_step(self.h, n)
+
+    def set_sponge(self, cx, cy):
+        """SPONGE_LAYER after every momentum pass of layer 1, with COEFX per
+        column (nx) and COEFY per row (ny)."""
+        a = np.ascontiguousarray(cx, np.float64)
+        b = np.ascontiguousarray(cy, np.float64)
+        self.lib().orc_set_sponge(self.h, _p(a), _p(b))
 
     def call(self, op, lid=1, t=0):
         self.lib().orc_call(self.h, OP[op] if isinstance(op, str) else op, lid, t)
@@ -291,6 +299,19 @@ class Reference:
         """The reference's GET_FLOOR_DEFORM for one fault rupturing at T = 0."""
         pa = np.ascontiguousarray(params, np.float32)
         big_stack(self.lib().ref_floor_deform, _p(pa), _i(fswitch))
+
+    def sponge(self, on=1):
+        """SPONGE_COEF on layer 1 (returns COEFX, COEFY as (ny, nx)); on != 0
+        also applies SPONGE_LAYER after every MOMENT of step()."""
+        s = self.specs[1]
+        cx = np.empty(s.nx * s.ny, np.float32)
+        cy = np.empty(s.nx * s.ny, np.float32)
+        self.lib().ref_sponge(_i(on), _p(cx), _p(cy))
+        return cx.reshape(s.ny, s.nx), cy.reshape(s.ny, s.nx)
+
+    def bc_wall(self):
+        """BC_WALL on layer 1 (H of the two outermost rows/columns = -999)."""
+        self.lib().ref_bc_wall()
 
     def deform(self, lid):
         """(DEFORM, H) of layer lid as (ny, nx) REAL arrays."""

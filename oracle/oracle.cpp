@@ -58,6 +58,7 @@ struct Layer {
   std::vector<T> r0, r1, r11, r2, r21, r22, r3, r4, r5, r6;
   std::vector<T> xflux, yflux;            // (nx,2), (ny,2)
   std::vector<T> zmax_wet, zmax_raw;
+  std::vector<T> sponge_cx, sponge_cy;    // COEFX per column, COEFY per row (layer 1)
   std::vector<int> arrival;
   std::vector<int> children;              // indices into State::layers
 
@@ -910,6 +911,19 @@ struct State {
     }
   }
 
+  // SPONGE_LAYER (boundaries.f90:206-246): damping of the new fluxes where
+  // either coefficient is positive
+  void sponge(Layer<T> &L) {
+    for (int i = 1; i <= L.nx; ++i)
+      for (int j = 1; j <= L.ny; ++j) {
+        const T cx = L.sponge_cx[i - 1], cy = L.sponge_cy[j - 1];
+        if (cx > T(0) || cy > T(0)) {
+          L.M(i, j, 2) = L.M(i, j, 2) + cx * L.dt * L.M(i, j, 1);
+          L.N(i, j, 2) = L.N(i, j, 2) + cy * L.dt * L.N(i, j, 1);
+        }
+      }
+  }
+
   // one root step: comcot.f90:359-439 with CHANGE's copies (:487-489)
   void step() {
     Layer<T> &R = layers[0];
@@ -918,6 +932,7 @@ struct State {
     open_bc(R);
     if (!R.children.empty()) all_grid_children(R);
     moment(R);
+    if (!R.sponge_cx.empty()) sponge(R);  // BC_TYPE = 1 (comcot.f90:403-406)
     max_arrival(R);
     const size_t n = R.nxy();
     std::memcpy(&R.z[0], &R.z[n], n * sizeof(T));
@@ -1138,6 +1153,16 @@ void orc_max_io(void *s, int id, double *zmax_wet, double *zmax_raw, int *arriva
   Handle *H = (Handle *)s;
   if (H->precision == 32) max_io_t(*H->f, id, zmax_wet, zmax_raw, arrival);
   else max_io_t(*H->d, id, zmax_wet, zmax_raw, arrival);
+}
+
+void orc_set_sponge(void *s, const double *cx, const double *cy) {
+  Handle *H = (Handle *)s;
+  auto set = [&](auto &st) {
+    auto &L = st.layers[0];
+    L.sponge_cx.assign(cx, cx + L.nx);
+    L.sponge_cy.assign(cy, cy + L.ny);
+  };
+  if (H->precision == 32) set(*H->f); else set(*H->d);
 }
 
 void orc_set_arrival_threshold(void *s, double thr) {

@@ -22,6 +22,7 @@
 module ref_state
   use layer_params
   use fault_params
+  use wave_params
   implicit none
   type(layer), save :: lo
   type(layer), allocatable, save :: la(:)
@@ -30,6 +31,8 @@ module ref_state
   ! MAX_AMP reads back with CUDA_GETZMAX: starts at the initial elevation
   ! (GPUcomcot.cu:79), raised after each root step by MAXAMP_LAUNCH
   real, allocatable, save :: zmaxr(:,:)
+  ! BC_TYPE = 1: SPONGE_LAYER on the outermost layer after MOMENT
+  logical, save :: sponge_on = .false.
 contains
 
 
@@ -109,6 +112,7 @@ subroutine ref_init_root(nx, ny, laycord, laygov, dx, dy, dt, y, dely,  &
   real :: dx, dy, dt, fric_coef
   real :: y(ny), dely(ny), h(nx*ny), z0(nx*ny)
   call ref_reset()
+  sponge_on = .false.
   lo%id = 1
   lo%parent = -1
   lo%level = 1
@@ -169,6 +173,7 @@ subroutine ref_step(nsteps) bind(C, name="ref_step")
      call open(lo)
      if (lo%num_child .ge. 1) call all_grid(lo, la)
      call moment(lo)
+     if (sponge_on) call sponge_layer(lo)
      lo%z(:,:,1) = lo%z(:,:,2)
      lo%m(:,:,1) = lo%m(:,:,2)
      lo%n(:,:,1) = lo%n(:,:,2)
@@ -453,6 +458,32 @@ subroutine ref_floor_deform(params, fswitch) bind(C, name="ref_floor_deform")
   call get_floor_deform(lo, la, flt, time)
   deallocate(flt)
 end subroutine ref_floor_deform
+
+!.....the reference's SPONGE_COEF (boundaries.f90:250-330) on layer 1;
+!     on /= 0 also runs SPONGE_LAYER after every MOMENT of ref_step
+!     (comcot.f90:403-406).  cx, cy receive COEFX, COEFY (NX*NY each).
+subroutine ref_sponge(on, cx, cy) bind(C, name="ref_sponge")
+  implicit none
+  integer :: on
+  real :: cx(*), cy(*)
+  integer :: n
+  n = lo%nx * lo%ny
+  if (associated(lo%sponge_coefx)) deallocate(lo%sponge_coefx)
+  if (associated(lo%sponge_coefy)) deallocate(lo%sponge_coefy)
+  allocate(lo%sponge_coefx(lo%nx, lo%ny), lo%sponge_coefy(lo%nx, lo%ny))
+  call sponge_coef(lo)
+  cx(1:n) = reshape(lo%sponge_coefx, (/n/))
+  cy(1:n) = reshape(lo%sponge_coefy, (/n/))
+  sponge_on = on .ne. 0
+end subroutine ref_sponge
+
+!.....the reference's BC_WALL (boundaries.f90:369-401) on layer 1, without a
+!     wave maker (INI_SWITCH /= 2)
+subroutine ref_bc_wall() bind(C, name="ref_bc_wall")
+  implicit none
+  type(wave) :: w
+  call bc_wall(lo, w)
+end subroutine ref_bc_wall
 
 !.....read DEFORM and H of layer id
 subroutine ref_deform_io(id, deform, h) bind(C, name="ref_deform_io")
