@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "comcot_consts.h"
@@ -396,6 +397,27 @@ int gcomcot_output_set_(const char *dir, int *dirlen, int *iprt, int *out_opt, f
   if (int e = ensure_init()) return e;
   if (sync()) return 1;
   return out_configure(g_tree, dir, *dirlen, *iprt, *out_opt, *tend, *tide);
+}
+
+int gcomcot_checkpoint_save_(const char *path, int *pathlen) {
+  if (!layer(1)) return fail(8, "no root layer");
+  if (!path || *pathlen <= 0) return fail(70, "checkpoint: empty path");
+  if (int e = ensure_init()) return e;
+  if (sync()) return 1;
+  int done = 0;
+  HIPCHK(hipMemcpy(&done, g_tree.ctl + g_tree.ctl_in, sizeof(int), hipMemcpyDeviceToHost));
+  const std::string p(path, (size_t)*pathlen);
+  return tree_checkpoint_save(g_tree, g_stream, p.c_str(), done);
+}
+
+int gcomcot_checkpoint_load_(const char *path, int *pathlen) {
+  if (!layer(1)) return fail(8, "no root layer");
+  if (!path || *pathlen <= 0) return fail(72, "checkpoint: empty path");
+  if (int e = ensure_init()) return e;
+  if (sync()) return 1;
+  const std::string p(path, (size_t)*pathlen);
+  long long done = 0;
+  return tree_checkpoint_load(g_tree, g_stream, p.c_str(), &done);
 }
 
 int gcomcot_sponge_set_(int *on) {

@@ -36,6 +36,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "comcot_consts.h"
@@ -622,6 +623,28 @@ int gcomcot_batch_get_max_(int *sid, int *lid, float *zmax_wet, int *arrival) {
     HIPCHK(hipMemcpy(zmax_wet, L->zmax_wet, L->nxy() * sizeof(float), hipMemcpyDeviceToHost));
   if (arrival)
     HIPCHK(hipMemcpy(arrival, L->arrival, L->nxy() * sizeof(int), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int gcomcot_batch_checkpoint_save_(int *sid, const char *path, int *pathlen) {
+  Scenario *s = scenario(*sid);
+  if (!s) return fail(31, "batch: no scenario %d", *sid);
+  if (!path || *pathlen <= 0) return fail(70, "checkpoint: empty path");
+  if (int e = eng_init()) return e;
+  const std::string p(path, (size_t)*pathlen);
+  return tree_checkpoint_save(s->tree, g_eng.st, p.c_str(), s->done);
+}
+
+int gcomcot_batch_checkpoint_load_(int *sid, const char *path, int *pathlen) {
+  Scenario *s = scenario(*sid);
+  if (!s) return fail(31, "batch: no scenario %d", *sid);
+  if (!path || *pathlen <= 0) return fail(72, "checkpoint: empty path");
+  if (int e = eng_init()) return e;
+  invalidate();
+  const std::string p(path, (size_t)*pathlen);
+  long long done = 0;
+  if (int e = tree_checkpoint_load(s->tree, g_eng.st, p.c_str(), &done)) return e;
+  s->done = done;
   return 0;
 }
 

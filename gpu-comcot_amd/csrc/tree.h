@@ -114,6 +114,12 @@ int tree_set_gauges(Tree &T, hipStream_t st, int n, const float *x0, const float
                     const int *lid, int capacity);
 int tree_enqueue_gauges(const Tree &T, hipStream_t st);
 // Kernel launches one root step of the tree enqueues.
+// Exact checkpoint of a tree's device state (every plane of every layer,
+// the time-level parity and the root-step count) to a file, and its
+// restore into a tree built with the same layers.
+int tree_checkpoint_save(const Tree &T, hipStream_t st, const char *path, long long done);
+int tree_checkpoint_load(Tree &T, hipStream_t st, const char *path, long long *done);
+
 long long tree_launches_per_step(Tree &T);
 
 // R2/R4 of the reference ABI as a row factor of max(HP,0) (PARA,

@@ -13,6 +13,7 @@
 
 #include <cmath>
 #include <cstdarg>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -494,6 +495,127 @@ bool row_factor(const float *R, const float *H, int nx, int ny, int j, bool is_x
     if (check(c)) { *F = c; return true; }
   }
   return false;
+}
+
+// ------------------------------------------------------------ checkpoints
+// File: "GCOMCOTCKPT1", int32 version, int32 layer count, int64 completed
+// root steps; per layer an int32 record (id, parent, nx, ny, nl, laycord,
+// laygov, rel_size, rel_time, corners[4], cur), the uint64 size of its
+// device block and the block's bytes (every plane of the layer: depth,
+// both time levels, maxima, arrival, flux caches, row coefficients).
+namespace {
+
+constexpr char kCkptMagic[12] = {'G', 'C', 'O', 'M', 'C', 'O', 'T', 'C', 'K', 'P', 'T', '1'};
+
+struct CkptLayer {
+  int32_t id, parent, nx, ny, nl, laycord, laygov, rel_size, rel_time, corners[4], cur;
+  uint64_t bytes;
+};
+
+CkptLayer ckpt_record(const LayerRec &L) {
+  CkptLayer c{};
+  c.id = L.id;
+  c.parent = L.parent;
+  c.nx = L.nx;
+  c.ny = L.ny;
+  c.nl = L.nl;
+  c.laycord = L.laycord;
+  c.laygov = L.laygov;
+  c.rel_size = L.rel_size;
+  c.rel_time = L.rel_time;
+  for (int k = 0; k < 4; ++k) c.corners[k] = L.corners[k];
+  c.cur = L.cur;
+  c.bytes = L.block_bytes;
+  return c;
+}
+
+bool same_layer(const CkptLayer &a, const CkptLayer &b) {
+  bool ok = a.id == b.id && a.parent == b.parent && a.nx == b.nx && a.ny == b.ny &&
+            a.nl == b.nl && a.laycord == b.laycord && a.laygov == b.laygov &&
+            a.rel_size == b.rel_size && a.rel_time == b.rel_time && a.bytes == b.bytes;
+  for (int k = 0; k < 4; ++k) ok = ok && a.corners[k] == b.corners[k];
+  return ok;
+}
+
+struct File {
+  FILE *f = nullptr;
+  ~File() {
+    if (f) std::fclose(f);
+  }
+};
+
+}  // namespace
+
+int tree_checkpoint_save(const Tree &T, hipStream_t st, const char *path, long long done) {
+  HIPCHK(hipStreamSynchronize(st));
+  File F;
+  F.f = std::fopen(path, "wb");
+  if (!F.f) return fail(70, "checkpoint: cannot create %s", path);
+  int32_t head[2] = {1, 0};
+  for (const LayerRec &L : T.layers) head[1] += L.used ? 1 : 0;
+  const int64_t steps = done;
+  bool ok = std::fwrite(kCkptMagic, 1, sizeof kCkptMagic, F.f) == sizeof kCkptMagic &&
+            std::fwrite(head, sizeof head, 1, F.f) == 1 && std::fwrite(&steps, 8, 1, F.f) == 1;
+  std::vector<char> buf;
+  for (const LayerRec &L : T.layers) {
+    if (!ok) break;
+    if (!L.used) continue;
+    const CkptLayer c = ckpt_record(L);
+    buf.resize(L.block_bytes);
+    HIPCHK(hipMemcpy(buf.data(), L.block, L.block_bytes, hipMemcpyDeviceToHost));
+    ok = std::fwrite(&c, sizeof c, 1, F.f) == 1 &&
+         std::fwrite(buf.data(), 1, buf.size(), F.f) == buf.size();
+  }
+  const bool closed = std::fclose(F.f) == 0;
+  F.f = nullptr;
+  if (!ok || !closed) return fail(71, "checkpoint: write to %s failed", path);
+  return 0;
+}
+
+int tree_checkpoint_load(Tree &T, hipStream_t st, const char *path, long long *done) {
+  File F;
+  F.f = std::fopen(path, "rb");
+  if (!F.f) return fail(72, "checkpoint: cannot open %s", path);
+  char magic[sizeof kCkptMagic];
+  int32_t head[2];
+  int64_t steps = 0;
+  if (std::fread(magic, 1, sizeof magic, F.f) != sizeof magic ||
+      std::memcmp(magic, kCkptMagic, sizeof magic) != 0 || std::fread(head, sizeof head, 1, F.f) != 1 ||
+      head[0] != 1 || std::fread(&steps, 8, 1, F.f) != 1 || steps < 0)
+    return fail(73, "checkpoint: %s is not a checkpoint of this library", path);
+  int used = 0;
+  for (const LayerRec &L : T.layers) used += L.used ? 1 : 0;
+  if (head[1] != used) return fail(74, "checkpoint: %d layers in the file, %d here", head[1], used);
+  // every record must match a layer here before anything is overwritten
+  const long start = std::ftell(F.f);
+  std::vector<CkptLayer> recs;
+  for (int k = 0; k < head[1]; ++k) {
+    CkptLayer c;
+    if (std::fread(&c, sizeof c, 1, F.f) != 1) return fail(73, "checkpoint: %s is truncated", path);
+    const LayerRec *L = T.layer(c.id);
+    if (!L || !same_layer(c, ckpt_record(*L)) || (c.cur != 0 && c.cur != 1))
+      return fail(75, "checkpoint: layer %d of %s does not match this layer tree", c.id, path);
+    if (std::fseek(F.f, (long)c.bytes, SEEK_CUR) != 0)
+      return fail(73, "checkpoint: %s is truncated", path);
+    recs.push_back(c);
+  }
+  if (std::fseek(F.f, start, SEEK_SET) != 0) return fail(73, "checkpoint: cannot reread %s", path);
+  HIPCHK(hipStreamSynchronize(st));
+  std::vector<char> buf;
+  for (const CkptLayer &r : recs) {
+    CkptLayer c;
+    buf.resize(r.bytes);
+    if (std::fread(&c, sizeof c, 1, F.f) != 1 || std::fread(buf.data(), 1, buf.size(), F.f) != buf.size())
+      return fail(73, "checkpoint: %s is truncated", path);
+    LayerRec &L = T.layers[c.id];
+    HIPCHK(hipMemcpy(L.block, buf.data(), buf.size(), hipMemcpyHostToDevice));
+    L.cur = c.cur;
+  }
+  const int n32 = (int)steps;
+  const int ctl[2] = {n32, n32};
+  HIPCHK(hipMemcpy(T.ctl, ctl, sizeof ctl, hipMemcpyHostToDevice));
+  if (done) *done = steps;
+  return 0;
 }
 
 }  // namespace gc

@@ -8,6 +8,7 @@ tile-shape autotuner) is native code in csrc/batch.hip; this class only
 marshals arrays.
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -85,6 +86,18 @@ class Batch:
             check(rc, f"batch add layer {s.id} of {sc.name}")
         self.specs[sid.value] = sc
         return sid.value
+
+    def save(self, sid, path):
+        """Exact checkpoint of scenario sid (its planes and step count)."""
+        b = os.fsencode(os.fspath(path))
+        check(lib().gcomcot_batch_checkpoint_save_(ip(sid), ctypes.create_string_buffer(b),
+                                                   ip(len(b))), "batch checkpoint save")
+
+    def load(self, sid, path):
+        """Resume scenario sid (registered with the same layers) from a checkpoint."""
+        b = os.fsencode(os.fspath(path))
+        check(lib().gcomcot_batch_checkpoint_load_(ip(sid), ctypes.create_string_buffer(b),
+                                                   ip(len(b))), "batch checkpoint load")
 
     def fault(self, sid, params, switch=0):
         """Seafloor deformation of one fault plane for scenario sid, applied on

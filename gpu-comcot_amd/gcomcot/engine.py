@@ -116,6 +116,18 @@ class DeviceRun:
         check(lib().gcomcot_fault_deform_(pa.ctypes.data_as(ctypes.c_void_p), ip(switch)),
               "fault deform")
 
+    def save(self, path):
+        """Exact checkpoint of the device state to a file."""
+        b = os.fsencode(os.fspath(path))
+        check(lib().gcomcot_checkpoint_save_(ctypes.create_string_buffer(b), ip(len(b))),
+              "checkpoint save")
+
+    def load(self, path):
+        """Restore a checkpoint made from a run with the same layers."""
+        b = os.fsencode(os.fspath(path))
+        check(lib().gcomcot_checkpoint_load_(ctypes.create_string_buffer(b), ip(len(b))),
+              "checkpoint load")
+
     def sponge(self, on=True):
         """SPONGE_LAYER on layer 1 after every momentum pass (coefficients from
         its depth and coordinates; call set_coords first)."""

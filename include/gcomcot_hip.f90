@@ -182,6 +182,21 @@ module gcomcot_hip
       integer(c_int) :: mode
     end function
 
+    ! Exact checkpoints of the device state
+    integer(c_int) function gcomcot_checkpoint_save(path, pathlen) &
+        bind(C, name="gcomcot_checkpoint_save_")
+      import :: c_int, c_char
+      character(kind=c_char) :: path(*)
+      integer(c_int) :: pathlen
+    end function
+
+    integer(c_int) function gcomcot_checkpoint_load(path, pathlen) &
+        bind(C, name="gcomcot_checkpoint_load_")
+      import :: c_int, c_char
+      character(kind=c_char) :: path(*)
+      integer(c_int) :: pathlen
+    end function
+
     ! Boundary treatments of the outermost layer (SPONGE_COEF/SPONGE_LAYER, BC_WALL)
     subroutine gcomcot_sponge_coef(laycord, nx, ny, h, x, y, cx, cy) &
         bind(C, name="gcomcot_sponge_coef_")

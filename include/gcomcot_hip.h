@@ -178,6 +178,21 @@ int gcomcot_set_layer_coords_(int *lid, const float *x, const float *y);
 int gcomcot_gauges_set_(int *n, const float *x0, const float *y0, const int *lid, int *capacity);
 int gcomcot_gauges_read_(float *series, int *max_steps, int *nout);
 
+/* Exact checkpoints of the device state (the reference's HOT_START,
+ * hotstart.f90:2-166, saves Z, M, N and the maxima as 10F12.5 text, which
+ * rounds them; the GPU build does not support it, comcot.f90:328-332).
+ * gcomcot_checkpoint_save_ writes every plane of every layer, the time-level
+ * parity and the root-step count to the file path[0..*pathlen);
+ * gcomcot_checkpoint_load_ restores it into a tree built with the same
+ * layers (same ids, parents, sizes, kinds and corners; refused otherwise
+ * before anything changes).  Continuing after a load gives the same bits as
+ * continuing without it.  The batch forms do the same for one scenario and
+ * restore its step count, so a served request can be preempted and resumed. */
+int gcomcot_checkpoint_save_(const char *path, int *pathlen);
+int gcomcot_checkpoint_load_(const char *path, int *pathlen);
+int gcomcot_batch_checkpoint_save_(int *sid, const char *path, int *pathlen);
+int gcomcot_batch_checkpoint_load_(int *sid, const char *path, int *pathlen);
+
 /* Boundary treatments of the outermost layer.
  * gcomcot_sponge_coef_ is SPONGE_COEF (boundaries.f90:250-330, with H_CALC
  * :334-365) on the host, in REAL: cx[NX] = COEFX(I, .), cy[NY] = COEFY(., J)
