@@ -15,9 +15,10 @@
 // layer), the staging area goes to pinned host memory on a copy stream of
 // its own, a host function on that stream hands the files to a pool of
 // writer threads as soon as the copy ends, and the writers put the records
-// on disk in parallel, so the step loop only pays for the pack pass.  Two
-// staging slots are used in turn; a third output waits for the older one
-// (the loop then runs at the speed of the disk).
+// on disk in parallel, so the step loop only pays for the pack pass.  Three
+// staging slots are used in turn; a fourth output waits for the oldest (the
+// loop then runs at the speed of the disk).  A queued write that a later
+// output of the same file supersedes is dropped.
 #pragma once
 
 #include <hip/hip_runtime.h>

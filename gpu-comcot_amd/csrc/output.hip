@@ -35,7 +35,7 @@ struct Slot {
   std::vector<FileRec> files;
 };
 
-constexpr int kSlots = 2;
+constexpr int kSlots = 3;
 // Writer threads; the files of one output go down in parallel.  A file
 // name always goes to the same writer, so rewrites of one name (the final
 // maximum file) land in step order.
@@ -160,8 +160,21 @@ void on_copied(void *arg) {
     Slot &S = O.slots[s];
     S.state = kWriting;
     S.remaining = S.files.size();
-    for (size_t f = 0; f < S.files.size(); ++f)
-      O.queue[writer_of(S.files[f].name)].push_back({s, (int)f});
+    for (size_t f = 0; f < S.files.size(); ++f) {
+      // a queued, unstarted write of the same file is superseded: the file
+      // would be overwritten by this one (the final maximum file is
+      // rewritten on the last steps of a run)
+      auto &q = O.queue[writer_of(S.files[f].name)];
+      for (auto it = q.begin(); it != q.end(); ++it) {
+        Slot &P = O.slots[it->slot];
+        if (P.files[(size_t)it->file].name != S.files[f].name) continue;
+        q.erase(it);
+        ++O.nfiles;  // counted as written: its content is on its way
+        if (--P.remaining == 0) P.state = kFree;
+        break;
+      }
+      q.push_back({s, (int)f});
+    }
     if (S.files.empty()) S.state = kFree;
   }
   O.cv.notify_all();
