@@ -397,15 +397,24 @@ int out_write(Tree &T, long long k, hipStream_t st) {
     hipError_t e = launch_out_pack(q, st);
     if (e != hipSuccess) return fail(205, "output pack launch failed: %s", hipGetErrorString(e));
   }
+  HIPCHK(hipEventRecord(S.packed, st));
+  HIPCHK(hipStreamWaitEvent(O.copy_st, S.packed, 0));
+  HIPCHK(hipMemcpyAsync(S.host, S.dev, total, hipMemcpyDeviceToHost, O.copy_st));
   {
     std::lock_guard<std::mutex> lk(O.mu);
     S.files = std::move(files);
     S.state = kCopying;
   }
-  HIPCHK(hipEventRecord(S.packed, st));
-  HIPCHK(hipStreamWaitEvent(O.copy_st, S.packed, 0));
-  HIPCHK(hipMemcpyAsync(S.host, S.dev, total, hipMemcpyDeviceToHost, O.copy_st));
-  HIPCHK(hipLaunchHostFunc(O.copy_st, on_copied, (void *)(intptr_t)s));
+  const hipError_t e = hipLaunchHostFunc(O.copy_st, on_copied, (void *)(intptr_t)s);
+  if (e != hipSuccess) {  // nothing will hand the slot on: give it back
+    {
+      std::lock_guard<std::mutex> lk(O.mu);
+      S.files.clear();
+      S.state = kFree;
+    }
+    O.cv.notify_all();
+    return fail(100 + (int)e, "output: host function launch failed: %s", hipGetErrorString(e));
+  }
   return 0;
 }
 
