@@ -130,6 +130,30 @@ int gcomcot_init_layer_ex_(int *layerid, int *parent, int *level, int *laycord, 
   return tree_upload_coef(L, c);
 }
 
+int gcomcot_init_layer_sc_(int *layerid, int *parent, int *level, int *laygov, int *rel_size,
+                           int *rel_time, int *corners, int *fric_switch, float *fric_coef,
+                           float *dt, float *dx, float *dy, float *H, float *Z, int *nx, int *ny,
+                           int *pos, float *cxy) {
+  if (!(*laygov == 0 || *laygov == 1 || *laygov == 5 || *laygov == 9))
+    return fail(11, "LAYGOV %d not supported (dispersive forms 2/3 are out of scope)", *laygov);
+  if (*fric_switch != 0 && *fric_switch != 1)
+    return fail(12, "FRIC_SWITCH %d not supported (0 constant Manning, 1 none)", *fric_switch);
+  if (*dt <= 0.0f || *dx <= 0.0f || *dy <= 0.0f) return fail(13, "DT/DX/DY must be positive");
+  if (!pos || !cxy) return fail(84, "SC: layer %d needs POS and CXY", *layerid);
+  if (int e = tree_validate_layer(g_tree, *layerid, *parent, *rel_size, *rel_time, corners, *nx,
+                                  *ny, true))
+    return e;
+  if (int e = ensure_init()) return e;
+  if (int e = tree_create_layer(g_tree, g_stream, *layerid, *parent, *level, 1, *laygov,
+                                *rel_size, *rel_time, corners, *fric_switch, *fric_coef, *dt,
+                                *dx, *dy, *nx, *ny, H, Z, true))
+    return e;
+  LayerRec &L = g_tree.layers[*layerid];
+  std::vector<float> c((size_t)kNumCoef * *ny, 0.0f);
+  if (int e = tree_upload_coef(L, c)) return e;
+  return tree_set_sc(g_tree, g_stream, *layerid, pos, cxy);
+}
+
 void gcomcot_show_grids_(void) {
   for (int id = 1; id < kMaxLayers; ++id) {
     const LayerRec &L = g_tree.layers[id];

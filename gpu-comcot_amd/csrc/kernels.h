@@ -108,6 +108,35 @@ struct DevNest {
 hipError_t launch_jnq(const DevNest &N, int t, hipStream_t st);
 hipError_t launch_jnz(const DevNest &N, hipStream_t st);
 
+// SC_OPTION = 1 coupling of a Cartesian child to its spherical parent
+// (all_grids.f90:996-1425): boundary fluxes bilinear in the parent's fluxes
+// at the positions POS/CXY that SUBGRID_MATCHING's reverse UTM projection
+// gives every child cell, and the parent's elevation averaged from the wet
+// child cells that fall in each parent cell.
+struct DevSc {
+  // parent (spherical)
+  int pnx, pny, p_nl, p_modscm, p_r24;
+  const float *ph, *pz2, *pm1, *pn1, *pcoef, *pr24;
+  float *pm2, *pn2;    // parent level-2 fluxes: NEWQ_SC's band forecasts
+  float *pz2_out;      // parent level-2 elevation, averaged by JNZ_SC
+  // child (Cartesian)
+  int cnx, cny, rel_time;
+  float half;          // REAL(REL_SIZE**2)/2
+  const int *pos;      // POS(NX,NY,2), component-major
+  const float *cxy;    // CXY(NX,NY,4), component-major
+  const float *ch, *cz_now, *cz2;  // H, newest Z (Fortran level 2 after UPDATE), level-2 buffer
+  float *cm1, *cn1;    // child level-1 fluxes, boundary written
+  const int *steps_done;  // root steps completed: the child's first sub-step ever sees Z2 = 0
+  int band[4][4];      // M bands (left, right edge), N bands (top, bottom): IS, IE, JS, JE
+  int corners[4];      // the POS extent
+  const int *jnz_off, *jnz_idx;  // per parent cell of the extent: its child cells in Fortran order
+};
+hipError_t launch_sc_edges(const DevSc &S, int t, hipStream_t st);
+hipError_t launch_sc_jnz(const DevSc &S, hipStream_t st);
+// CELL_INTERP_SC (all_grids.f90:938-992): a child's deformation from its parent's
+hipError_t launch_sc_cell_interp(const float *parent, int pnx, int pny, float *child, int cnx,
+                                 int cny, const int *pos, const float *cxy, hipStream_t st);
+
 // ---- batch engine: one persistent launch over the tiles of many layers
 struct BatchDesc {
   DevLayer L;

@@ -1203,6 +1203,168 @@ __global__ void __launch_bounds__(256) k_out_pack(OutPack p) {
   }
 }
 
+// ------------------------------------------------- SC_OPTION = 1 coupling
+__device__ __forceinline__ float scp(const DevSc &S, const float *a, int i, int j) {
+  return a[(size_t)(j - 1) * S.pnx + (i - 1)];
+}
+__device__ __forceinline__ bool in_rect(const int *r, int i, int j) {
+  return i >= r[0] && i <= r[1] && j >= r[2] && j <= r[3];
+}
+
+// NEWQ_SC's forecast of the parent's linear flux at (i, j) (all_grids.f90:
+// 1131-1271): MOMT_S with the SCM term where MODSCM = 0; x = M, else N
+__device__ float sc_band_flux(const DevSc &S, bool x, int i, int j) {
+  if (x) {
+    const int ip1 = i + 1;
+    if (!(scp(S, S.ph, i, j) > kGx && scp(S, S.ph, ip1, j) > kGx)) return 0.0f;
+    int jm1 = j - 1, jp1 = j + 1;
+    if (jm1 <= 1) jm1 = 1;
+    if (jp1 >= S.pny) jp1 = S.pny;
+    const float tot_n = scp(S, S.pn1, i, j) + scp(S, S.pn1, ip1, j) + scp(S, S.pn1, i, jm1) +
+                        scp(S, S.pn1, ip1, jm1);
+    const float hp = 0.5f * (scp(S, S.ph, i, j) + scp(S, S.ph, ip1 >= S.pnx ? S.pnx : ip1, j));
+    const float f2 = S.pcoef[kF2 * S.pny + (j - 1)];
+    const float r2 = S.p_r24 ? scp(S, S.pr24, i, j) : (S.p_nl ? f2 : f2 * fmaxf(hp, 0.0f));
+    float xm = scp(S, S.pm1, i, j) - r2 * (scp(S, S.pz2, ip1, j) - scp(S, S.pz2, i, j)) +
+               S.pcoef[kR3 * S.pny + (j - 1)] * tot_n;
+    if (S.p_modscm == 0) {
+      const float scm = r2 * kTwlvth *
+                        ((scp(S, S.pz2, ip1, jp1) - 2.0f * scp(S, S.pz2, ip1, j) + scp(S, S.pz2, ip1, jm1)) -
+                         (scp(S, S.pz2, i, jp1) - 2.0f * scp(S, S.pz2, i, j) + scp(S, S.pz2, i, jm1)));
+      xm = xm - scm;
+    }
+    if (fabsf(xm) < kEps) xm = 0.0f;
+    return xm;
+  }
+  const int jp1 = j + 1;
+  if (!(scp(S, S.ph, i, j) > kGx && scp(S, S.ph, i, jp1) > kGx)) return 0.0f;
+  int im1 = i - 1, ip1 = i + 1;
+  if (im1 <= 1) im1 = 1;
+  if (ip1 >= S.pnx) ip1 = S.pnx;
+  const float tot_m = scp(S, S.pm1, im1, j) + scp(S, S.pm1, im1, jp1) + scp(S, S.pm1, i, j) +
+                      scp(S, S.pm1, i, jp1);
+  const float hq = 0.5f * (scp(S, S.ph, i, j) + scp(S, S.ph, i, jp1 >= S.pny ? S.pny : jp1));
+  const float f4 = S.pcoef[kF4 * S.pny + (j - 1)];
+  const float r4 = S.p_r24 ? S.pr24[(size_t)S.pnx * S.pny + (size_t)(j - 1) * S.pnx + (i - 1)]
+                           : (S.p_nl ? f4 : f4 * fmaxf(hq, 0.0f));
+  float xn = scp(S, S.pn1, i, j) - r4 * (scp(S, S.pz2, i, jp1) - scp(S, S.pz2, i, j)) -
+             S.pcoef[kR5 * S.pny + (j - 1)] * tot_m;
+  if (S.p_modscm == 0) {
+    const float scm = r4 * kTwlvth *
+                      ((scp(S, S.pz2, ip1, jp1) - 2.0f * scp(S, S.pz2, i, jp1) + scp(S, S.pz2, im1, jp1)) -
+                       (scp(S, S.pz2, ip1, j) - 2.0f * scp(S, S.pz2, i, j) + scp(S, S.pz2, im1, j)));
+    xn = xn - scm;
+  }
+  if (fabsf(xn) < kEps) xn = 0.0f;
+  return xn;
+}
+
+// one thread per cell of the four bands (M bands first, then N bands)
+__global__ void __launch_bounds__(256) k_sc_band(DevSc S) {
+  int g = blockIdx.x * blockDim.x + threadIdx.x;
+  for (int b = 0; b < 4; ++b) {
+    const int *r = S.band[b];
+    const int w = r[1] - r[0] + 1, n = w * (r[3] - r[2] + 1);
+    if (g < n) {
+      const int i = r[0] + g % w, j = r[2] + g / w;
+      const size_t q = (size_t)(j - 1) * S.pnx + (i - 1);
+      if (b < 2) S.pm2[q] = sc_band_flux(S, true, i, j);
+      else S.pn2[q] = sc_band_flux(S, false, i, j);
+      return;
+    }
+    g -= n;
+  }
+}
+
+// EDGE_INTERP_SC (t = 1, all_grids.f90:996-1091) and the edge part of
+// NEWQ_SC (t > 1, :1274-1372): one thread per child boundary cell, left
+// (I = 1), right (I = NX), bottom (J = 1), top (J = NY).  The parent's
+// level 2 is the band forecast inside a band and level 1 elsewhere (where
+// the Fortran's level 2 holds the copy CHANGE made).
+__global__ void __launch_bounds__(256) k_sc_edges(DevSc S, int t) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ny = S.cny, nx = S.cnx;
+  if (g >= 2 * ny + 2 * nx) return;
+  int side, ci, cj;
+  if (g < ny) { side = 0; ci = 1; cj = g + 1; }
+  else if (g < 2 * ny) { side = 1; ci = nx; cj = g - ny + 1; }
+  else if (g < 2 * ny + nx) { side = 2; ci = g - 2 * ny + 1; cj = 1; }
+  else { side = 3; ci = g - 2 * ny - nx + 1; cj = ny; }
+  const bool x = side < 2;
+  const size_t c = (size_t)(cj - 1) * nx + (ci - 1), cn = (size_t)nx * ny;
+  float *out = x ? S.cm1 : S.cn1;
+  // the child's level-2 elevation: 0 before its first sub-step ever (ALLOC)
+  const float z2 = (t == 1 && *S.steps_done == 0) ? 0.0f : S.cz_now[c];
+  if (!(S.ch[c] + z2 > kGx)) {
+    out[c] = 0.0f;
+    return;
+  }
+  const int ki = S.pos[c], kj = S.pos[cn + c];
+  const float w1 = S.cxy[c], w2 = S.cxy[cn + c], w3 = S.cxy[2 * cn + c], w4 = S.cxy[3 * cn + c];
+  const float *p1 = x ? S.pm1 : S.pn1;
+  const float f1 = scp(S, p1, ki, kj) * w1 + scp(S, p1, ki + 1, kj) * w2 +
+                   scp(S, p1, ki, kj + 1) * w3 + scp(S, p1, ki + 1, kj + 1) * w4;
+  if (t == 1) {
+    out[c] = f1;
+    return;
+  }
+  const float *p2 = x ? S.pm2 : S.pn2;
+  auto lev2 = [&](int i, int j) {
+    const bool band = x ? (in_rect(S.band[0], i, j) || in_rect(S.band[1], i, j))
+                        : (in_rect(S.band[2], i, j) || in_rect(S.band[3], i, j));
+    return band ? scp(S, p2, i, j) : scp(S, p1, i, j);
+  };
+  const float f2 = lev2(ki, kj) * w1 + lev2(ki + 1, kj) * w2 + lev2(ki, kj + 1) * w3 +
+                   lev2(ki + 1, kj + 1) * w4;
+  const float c1 = ((float)t - 1.0f) / (float)S.rel_time;
+  const float c2 = 1.0f - c1;
+  // the top edge takes C1 for both terms (all_grids.f90:1365)
+  out[c] = side == 3 ? c1 * f2 + c1 * f1 : c1 * f2 + c2 * f1;
+}
+
+// JNZ_SC (all_grids.f90:1376-1425): one thread per parent cell of the
+// extent sums its wet child cells in the Fortran's order.  It runs before
+// UPDATE, so the child's ghost column I = 1 and row J = 1, which continuity
+// never computes (mass.f90:70-71), still hold the previous UPDATE's copies
+// (all_grids.f90:323-324): the level-1 values here, whose level-2 slots the
+// step has already refreshed; 0 before the first UPDATE ever (ALLOC).
+__global__ void __launch_bounds__(256) k_sc_jnz(DevSc S) {
+  const int w = S.corners[1] - S.corners[0] + 1;
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= w * (S.corners[3] - S.corners[2] + 1)) return;
+  const int i = S.corners[0] + g % w, j = S.corners[2] + g / w;
+  // JNZ runs in sub-step REL_TIME/2 + 1: the first ever only when REL_TIME = 1
+  const bool first = S.rel_time == 1 && *S.steps_done == 0;
+  float z = 0.0f, cnt = 0.0f;
+  for (int q = S.jnz_off[g]; q < S.jnz_off[g + 1]; ++q) {
+    const int c = S.jnz_idx[q];
+    const bool ghost = c < S.cnx || c % S.cnx == 0;
+    const float zc = ghost ? (first ? 0.0f : S.cz_now[c]) : S.cz2[c];
+    if (S.ch[c] + zc > kGx) {
+      z = z + zc;
+      cnt = cnt + 1.0f;
+    }
+  }
+  if (cnt > S.half) S.pz2_out[(size_t)(j - 1) * S.pnx + (i - 1)] = z / cnt;
+}
+
+__global__ void __launch_bounds__(256) k_sc_cell_interp(const float *__restrict__ pd, int pnx,
+                                                        int pny, float *cd, int cnx, int cny,
+                                                        const int *__restrict__ pos,
+                                                        const float *__restrict__ cxy) {
+  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long n = (long long)cnx * cny;
+  if (g >= n) return;
+  const int ki = pos[g], kj = pos[n + g];
+  float v = 0.0f;
+  if (ki >= 1 && ki < pnx && kj >= 1 && kj < pny) {
+    auto P = [&](int a, int b) { return pd[(size_t)(b - 1) * pnx + (a - 1)]; };
+    v = P(ki, kj) * cxy[g] + P(ki + 1, kj) * cxy[n + g] + P(ki, kj + 1) * cxy[2 * n + g] +
+        P(ki + 1, kj + 1) * cxy[3 * n + g];
+  }
+  cd[g] = v;
+}
+
 // ---------------------------------------------------------------- source
 // STRIKE_SLIP (deform.f90:194-217) and DIP_SLIP (:221-238) at one corner of
 // the fault plane, FP64; sn, cs = sin/cos of the dip, dd = the depth.
@@ -1622,6 +1784,34 @@ hipError_t launch_out_pack(const OutPack &p, hipStream_t st) {
   if (p.count <= 0) return hipSuccess;
   const long long blocks = std::min<long long>((p.count + 255) / 256, 8192);
   hipLaunchKernelGGL(k_out_pack, dim3((unsigned)blocks), dim3(256), 0, st, p);
+  return hipGetLastError();
+}
+
+hipError_t launch_sc_edges(const DevSc &S, int t, hipStream_t st) {
+  if (t > 1) {  // NEWQ_SC: the band forecasts first
+    int n = 0;
+    for (int b = 0; b < 4; ++b)
+      n += (S.band[b][1] - S.band[b][0] + 1) * (S.band[b][3] - S.band[b][2] + 1);
+    hipLaunchKernelGGL(k_sc_band, dim3((n + 255) / 256), dim3(256), 0, st, S);
+    if (hipError_t e = hipGetLastError()) return e;
+  }
+  const int m = 2 * S.cnx + 2 * S.cny;
+  hipLaunchKernelGGL(k_sc_edges, dim3((m + 255) / 256), dim3(256), 0, st, S, t);
+  return hipGetLastError();
+}
+
+hipError_t launch_sc_jnz(const DevSc &S, hipStream_t st) {
+  const int n = (S.corners[1] - S.corners[0] + 1) * (S.corners[3] - S.corners[2] + 1);
+  hipLaunchKernelGGL(k_sc_jnz, dim3((n + 255) / 256), dim3(256), 0, st, S);
+  return hipGetLastError();
+}
+
+hipError_t launch_sc_cell_interp(const float *parent, int pnx, int pny, float *child, int cnx,
+                                 int cny, const int *pos, const float *cxy, hipStream_t st) {
+  const long long n = (long long)cnx * cny;
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sc_cell_interp, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                     parent, pnx, pny, child, cnx, cny, pos, cxy);
   return hipGetLastError();
 }
 

@@ -5,7 +5,8 @@
 //     the stereographic projection (deform.f90:494-588) is split into
 //     per-row and per-column factors computed here, on the host, in FP64
 //   ININTERP / CELL_INTERP (all_grids.f90:805-934)  k_cell_interp, parent
-//     to child, level by level, in REAL
+//     to child, level by level, in REAL (CELL_INTERP_SC, :938-992, for
+//     SC_OPTION = 1 children)
 //   UPDATE_BATH (initialization.f90:2410-2460) and Z(:,:,1) += DEFORM
 //     (deform.f90:55-64)  k_apply_deform on every layer
 //
@@ -111,6 +112,7 @@ int tree_apply_fault(Tree &T, hipStream_t st, const float *params, int fswitch) 
     const LayerRec *C = T.layer(id);
     if (!C) continue;
     const LayerRec &P = T.layers[C->parent];
+    if (C->sc) continue;  // CELL_INTERP_SC reads the parent through POS
     if (C->corners[0] < 2 || C->corners[1] + 1 > P.nx || C->corners[2] < 2 ||
         C->corners[3] + 1 > P.ny)
       return fail(62, "fault: layer %d touches its parent's edge; deformation cannot be spread", id);
@@ -179,9 +181,11 @@ int tree_apply_fault(Tree &T, hipStream_t st, const float *params, int fswitch) 
   for (size_t k = 1; !rc && e == hipSuccess && k < order.size(); ++k) {
     LayerRec &C = T.layers[order[k]];
     rc = ensure_deform(C);
-    if (!rc)
-      e = launch_cell_interp(T.layers[C.parent].deform, T.layers[C.parent].nx, C.deform, C.nx,
-                             C.ny, C.corners, C.rel_size, st);
+    const LayerRec &P = T.layers[C.parent];
+    if (!rc && C.sc)  // CELL_INTERP_SC (all_grids.f90:938-992)
+      e = launch_sc_cell_interp(P.deform, P.nx, P.ny, C.deform, C.nx, C.ny, C.sc_pos, C.sc_cxy, st);
+    else if (!rc)
+      e = launch_cell_interp(P.deform, P.nx, C.deform, C.nx, C.ny, C.corners, C.rel_size, st);
   }
   // before the first step the maximum starts from the deformed surface, as
   // when the source is part of the initial state

@@ -36,6 +36,14 @@ struct LayerRec {
   int *arrival = nullptr;
   float *deform = nullptr;  // last seafloor deformation applied (on first use)
   float *sponge = nullptr;  // SPONGE_COEF: COEFX[NX] then COEFY[NY] (layer 1, when on)
+  // SC_OPTION = 1 (Cartesian child of a spherical layer 1): POS, CXY, the
+  // NEWQ_SC bands and JNZ_SC's child lists, on the device; corners[] is
+  // then the POS extent
+  int sc = 0;
+  int *sc_pos = nullptr, *sc_jnz = nullptr;  // POS (2 planes); CSR offsets then indices
+  float *sc_cxy = nullptr;
+  int sc_band[4][4] = {};
+  size_t sc_jnz_cells = 0;
   int cur = 0;  // buffer index holding time level 1
   std::vector<int> children;
   size_t nxy() const { return (size_t)nx * ny; }
@@ -85,11 +93,15 @@ size_t layer_bytes(int nx, int ny, bool nl, bool r24);
 // device planes, uploads H and Z.  A new root frees the old tree.
 // The argument checks of tree_create_layer alone (no device work).
 int tree_validate_layer(Tree &T, int id, int parent, int rel_size, int rel_time,
-                        const int *corners, int nx, int ny);
+                        const int *corners, int nx, int ny, bool sc = false);
 int tree_create_layer(Tree &T, hipStream_t st, int id, int parent, int level, int laycord,
                       int laygov, int rel_size, int rel_time, const int *corners,
                       int fric_switch, float fric_coef, float dt, float dx, float dy, int nx,
-                      int ny, const float *H, const float *Z);
+                      int ny, const float *H, const float *Z, bool sc = false);
+// Couple child layer id to its spherical parent with SC_OPTION = 1, from
+// SUBGRID_MATCHING's POS (NX*NY*2) and CXY (NX*NY*4), column-major and
+// component-major as the Fortran holds them.
+int tree_set_sc(Tree &T, hipStream_t st, int id, const int *pos, const float *cxy);
 int tree_upload_coef(LayerRec &L, const std::vector<float> &coef);
 int tree_alloc_r24(LayerRec &L, const float *R2, const float *R4);
 void tree_free(Tree &T);

@@ -11,6 +11,7 @@
 // flux cache.  Planes start on 256-byte boundaries.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdint>
@@ -89,6 +90,9 @@ static void free_layer(LayerRec &L) {
   if (L.r24) (void)hipFree(L.r24);
   if (L.deform) (void)hipFree(L.deform);
   if (L.sponge) (void)hipFree(L.sponge);
+  if (L.sc_pos) (void)hipFree(L.sc_pos);
+  if (L.sc_cxy) (void)hipFree(L.sc_cxy);
+  if (L.sc_jnz) (void)hipFree(L.sc_jnz);
   L = LayerRec();
 }
 
@@ -108,7 +112,7 @@ void tree_free(Tree &T) {
 }
 
 int tree_validate_layer(Tree &T, int id, int parent, int rel_size, int rel_time,
-                        const int *corners, int nx, int ny) {
+                        const int *corners, int nx, int ny, bool sc) {
   if (id < 1 || id >= kMaxLayers) return fail(1, "invalid layer id %d (1..%d)", id, kMaxLayers - 1);
   if (nx < 3 || ny < 3) return fail(2, "layer %d: grid %dx%d too small", id, nx, ny);
   if (id == 1 && parent != -1) return fail(3, "layer 1 must be the root (parent -1)");
@@ -116,6 +120,16 @@ int tree_validate_layer(Tree &T, int id, int parent, int rel_size, int rel_time,
     LayerRec *P = T.layer(parent);
     if (!P) return fail(4, "layer %d: parent %d not initialised", id, parent);
     const int is = corners[0], ie = corners[1], js = corners[2], je = corners[3];
+    if (sc) {  // the POS extent: NEWQ_SC's bands reach three parent cells beyond it
+      if (P->id != 1 || P->laycord != 0)
+        return fail(80, "layer %d: SC_OPTION 1 needs the spherical layer 1 as parent", id);
+      if (is < 4 || js < 4 || ie > P->nx - 4 || je > P->ny - 4 || ie < is || je < js)
+        return fail(5, "layer %d: SC extent I %d..%d J %d..%d must lie in 4..NX-4, 4..NY-4", id,
+                    is, ie, js, je);
+      if (rel_size < 1 || rel_time < 1)
+        return fail(7, "layer %d: REL_SIZE %d / REL_TIME %d < 1", id, rel_size, rel_time);
+      return 0;
+    }
     if (is < 4 || js < 4 || ie > P->nx - 2 || je > P->ny - 2 || ie < is || je < js)
       return fail(5, "layer %d: footprint I %d..%d J %d..%d must lie in 4..NX-2, 4..NY-2 of its parent",
                   id, is, ie, js, je);
@@ -130,8 +144,9 @@ int tree_validate_layer(Tree &T, int id, int parent, int rel_size, int rel_time,
 int tree_create_layer(Tree &T, hipStream_t st, int id, int parent, int level, int laycord,
                       int laygov, int rel_size, int rel_time, const int *corners,
                       int fric_switch, float fric_coef, float dt, float dx, float dy, int nx,
-                      int ny, const float *H, const float *Z) {
-  if (int e = tree_validate_layer(T, id, parent, rel_size, rel_time, corners, nx, ny)) return e;
+                      int ny, const float *H, const float *Z, bool sc) {
+  if (int e = tree_validate_layer(T, id, parent, rel_size, rel_time, corners, nx, ny, sc))
+    return e;
   if (!T.ctl) {
     HIPCHK(hipMalloc(&T.ctl, 2 * sizeof(int)));
     HIPCHK(hipMemsetAsync(T.ctl, 0, 2 * sizeof(int), st));
@@ -323,12 +338,50 @@ int tree_launch(const Tree &T, const LayerRec &L, int mode, bool with_max, bool 
 }
 
 // ALL_GRID (all_grids.f90:34-308) / subgrid_solver_rcsv (GPUall_grids.cu:655-699)
+// The SC_OPTION = 1 coupling views of parent P and child C.
+static DevSc sc_view(const Tree &T, const LayerRec &P, const LayerRec &C) {
+  DevSc S{};
+  S.pnx = P.nx;
+  S.pny = P.ny;
+  S.p_nl = P.nl;
+  S.p_modscm = P.modscm;
+  S.p_r24 = P.r24 != nullptr;
+  S.ph = P.h;
+  S.pz2 = P.z[1 - P.cur];
+  S.pm1 = P.m[P.cur];
+  S.pn1 = P.n[P.cur];
+  S.pcoef = P.coef;
+  S.pr24 = P.r24;
+  S.pm2 = P.m[1 - P.cur];
+  S.pn2 = P.n[1 - P.cur];
+  S.pz2_out = P.z[1 - P.cur];
+  S.cnx = C.nx;
+  S.cny = C.ny;
+  S.rel_time = C.rel_time;
+  S.half = (float)(C.rel_size * C.rel_size) / 2.0f;
+  S.pos = C.sc_pos;
+  S.cxy = C.sc_cxy;
+  S.ch = C.h;
+  S.cz_now = C.z[C.cur];
+  S.cz2 = C.z[1 - C.cur];
+  S.cm1 = C.m[C.cur];
+  S.cn1 = C.n[C.cur];
+  S.steps_done = T.ctl + T.ctl_in;
+  for (int b = 0; b < 4; ++b)
+    for (int q = 0; q < 4; ++q) S.band[b][q] = C.sc_band[b][q];
+  for (int q = 0; q < 4; ++q) S.corners[q] = C.corners[q];
+  S.jnz_off = C.sc_jnz;
+  S.jnz_idx = C.sc_jnz + C.sc_jnz_cells + 1;
+  return S;
+}
+
 int tree_enqueue_children(Tree &T, LayerRec &P, bool with_max, hipStream_t st) {
   for (int cid : P.children) {
     LayerRec &C = T.layers[cid];
     const int nhalf = (int)std::floor(C.rel_time / 2.0f) + 1;
     for (int k = 1; k <= C.rel_time; ++k) {
-      hipError_t e = launch_jnq(nest_view(P, C), k, st);
+      hipError_t e = C.sc ? launch_sc_edges(sc_view(T, P, C), k, st)
+                          : launch_jnq(nest_view(P, C), k, st);
       if (e != hipSuccess) return fail(201, "jnq launch failed: %s", hipGetErrorString(e));
       if (C.children.empty()) {
         if (int r = tree_launch(T, C, kFused, with_max, false, st)) return r;
@@ -338,7 +391,7 @@ int tree_enqueue_children(Tree &T, LayerRec &P, bool with_max, hipStream_t st) {
         if (int r = tree_launch(T, C, kMomentOnly, with_max, false, st)) return r;
       }
       if (k == nhalf) {
-        e = launch_jnz(nest_view(P, C), st);
+        e = C.sc ? launch_sc_jnz(sc_view(T, P, C), st) : launch_jnz(nest_view(P, C), st);
         if (e != hipSuccess) return fail(202, "jnz launch failed: %s", hipGetErrorString(e));
       }
       C.cur = 1 - C.cur;  // UPDATE
@@ -451,7 +504,8 @@ static long long launches_below(Tree &T, const LayerRec &P) {
   for (int cid : P.children) {
     const LayerRec &C = T.layers[cid];
     const long long inner = C.children.empty() ? 1 : 2 + launches_below(T, C);
-    n += (long long)C.rel_time * (1 + inner) + 1;  // JNQ + step(s) per sub-step, one JNZ
+    // JNQ (SC: bands and edges) + step(s) per sub-step, one JNZ
+    n += (long long)C.rel_time * ((C.sc ? 2 : 1) + inner) + 1;
   }
   return n;
 }
@@ -615,6 +669,72 @@ int tree_checkpoint_load(Tree &T, hipStream_t st, const char *path, long long *d
   const int ctl[2] = {n32, n32};
   HIPCHK(hipMemcpy(T.ctl, ctl, sizeof ctl, hipMemcpyHostToDevice));
   if (done) *done = steps;
+  return 0;
+}
+
+// ------------------------------------------------------------ SC_OPTION 1
+int tree_set_sc(Tree &T, hipStream_t st, int id, const int *pos, const float *cxy) {
+  LayerRec *C = T.layer(id);
+  if (!C || id == 1) return fail(81, "SC: layer %d is not a nested layer", id);
+  const LayerRec &P = T.layers[C->parent];
+  if (P.id != 1 || P.laycord != 0 || C->laycord != 1)
+    return fail(80, "SC: layer %d must be a Cartesian child of the spherical layer 1", id);
+  const int nx = C->nx, ny = C->ny;
+  const size_t n = C->nxy();
+  auto ki = [&](int i, int j) { return pos[(size_t)(j - 1) * nx + (i - 1)]; };
+  auto kj = [&](int i, int j) { return pos[n + (size_t)(j - 1) * nx + (i - 1)]; };
+  // every cell's parent cell and its +1 neighbours exist, and lie in the extent
+  for (int j = 1; j <= ny; ++j)
+    for (int i = 1; i <= nx; ++i) {
+      const int a = ki(i, j), b = kj(i, j);
+      if (a < C->corners[0] || a > C->corners[1] || b < C->corners[2] || b > C->corners[3] ||
+          a + 1 > P.nx || b + 1 > P.ny)
+        return fail(82, "SC: layer %d cell (%d,%d) has POS (%d,%d) outside its extent", id, i, j,
+                    a, b);
+    }
+  // NEWQ_SC's bands (all_grids.f90:1131-1271): M around the left and right
+  // edges, N around the top and bottom edges
+  const int c1 = C->corners[0], c2 = C->corners[1], c3 = C->corners[2], c4 = C->corners[3];
+  int band[4][4] = {
+      {std::min(ki(1, 1), ki(1, ny)) - 2, std::max(ki(1, 1), ki(1, ny)) + 2, c3 - 2, c4 + 2},
+      {std::min(ki(nx, 1), ki(nx, ny)) - 2, std::max(ki(nx, 1), ki(nx, ny)) + 2, c3 - 2, c4 + 2},
+      {c1 - 2, c2 + 2, std::min(kj(1, ny), kj(nx, ny)) - 2, std::max(kj(1, ny), kj(nx, ny)) + 2},
+      {c1 - 2, c2 + 2, std::min(kj(1, 1), kj(nx, 1)) - 2, std::max(kj(1, 1), kj(nx, 1)) + 2}};
+  for (int b = 0; b < 4; ++b)  // the formulas read (I+1, J) for M and (I, J+1) for N unclamped
+    if (band[b][0] < 1 || band[b][2] < 1 || band[b][1] + 1 > P.nx || band[b][3] + 1 > P.ny)
+      return fail(83, "SC: layer %d: NEWQ_SC band %d leaves the parent", id, b);
+  // JNZ_SC's child lists per parent cell of the extent, in the Fortran loop
+  // order (I outer, J inner)
+  const int w = c2 - c1 + 1;
+  const size_t cells = (size_t)w * (c4 - c3 + 1);
+  std::vector<int> count(cells, 0), off(cells + 1, 0), idx(n);
+  for (int i = 1; i <= nx; ++i)
+    for (int j = 1; j <= ny; ++j) ++count[(size_t)(kj(i, j) - c3) * w + (ki(i, j) - c1)];
+  for (size_t q = 0; q < cells; ++q) off[q + 1] = off[q] + count[q];
+  std::vector<int> fill(off.begin(), off.end() - 1);
+  for (int i = 1; i <= nx; ++i)
+    for (int j = 1; j <= ny; ++j)
+      idx[(size_t)fill[(size_t)(kj(i, j) - c3) * w + (ki(i, j) - c1)]++] = (j - 1) * nx + (i - 1);
+  std::vector<int> csr(off);
+  csr.insert(csr.end(), idx.begin(), idx.end());
+  if (C->sc_pos) (void)hipFree(C->sc_pos);
+  if (C->sc_cxy) (void)hipFree(C->sc_cxy);
+  if (C->sc_jnz) (void)hipFree(C->sc_jnz);
+  C->sc_pos = nullptr;
+  C->sc_cxy = nullptr;
+  C->sc_jnz = nullptr;
+  C->sc = 0;
+  HIPCHK(hipMalloc(&C->sc_pos, 2 * n * sizeof(int)));
+  HIPCHK(hipMalloc(&C->sc_cxy, 4 * n * sizeof(float)));
+  HIPCHK(hipMalloc(&C->sc_jnz, csr.size() * sizeof(int)));
+  HIPCHK(hipMemcpyAsync(C->sc_pos, pos, 2 * n * sizeof(int), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(C->sc_cxy, cxy, 4 * n * sizeof(float), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(C->sc_jnz, csr.data(), csr.size() * sizeof(int), hipMemcpyHostToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));
+  for (int b = 0; b < 4; ++b)
+    for (int q = 0; q < 4; ++q) C->sc_band[b][q] = band[b][q];
+  C->sc_jnz_cells = cells;
+  C->sc = 1;
   return 0;
 }
 

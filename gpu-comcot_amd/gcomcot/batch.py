@@ -62,6 +62,9 @@ class Batch:
 
     def add(self, sc, steps=None):
         """Register scenario `sc`; returns its id.  steps defaults to sc.steps."""
+        if any(getattr(s, "sc_pos", None) is not None for s in sc.layers):
+            raise GcomcotError("batched scenarios take SC_OPTION = 0 nests only; "
+                               "run an SC_OPTION = 1 nest with DeviceRun")
         L = lib()
         sid = ctypes.c_int(0)
         check(L.gcomcot_batch_new_scenario_(_ll(sc.steps if steps is None else steps),

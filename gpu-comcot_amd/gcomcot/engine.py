@@ -76,6 +76,18 @@ class DeviceRun:
                                       keep["R11"][1], hp, zp, ip(s.nx), ip(s.ny))
                 _ok(f"gcomcot_init_layer_({s.id})")
                 continue
+            if s.sc_pos is not None:  # SC_OPTION = 1 child
+                pa = np.ascontiguousarray(s.sc_pos, np.int32)
+                ca = np.ascontiguousarray(s.sc_cxy, np.float32)
+                rc = L.gcomcot_init_layer_sc_(ip(s.id), ip(s.parent), ip(s.level), ip(s.laygov),
+                                              ip(s.rel_size), ip(s.rel_time),
+                                              corners.ctypes.data_as(ctypes.c_void_p),
+                                              ip(s.fric_switch), fp(s.fric_coef), fp(s.dt),
+                                              fp(s.dx), fp(s.dy), hp, zp, ip(s.nx), ip(s.ny),
+                                              pa.ctypes.data_as(ctypes.c_void_p),
+                                              ca.ctypes.data_as(ctypes.c_void_p))
+                check(rc, f"init SC layer {s.id}")
+                continue
             if s.laycord == 0:
                 coef = scenarios.para_rows(s.laygov, s.dt, s.dx, s.y, s.dely)
                 cptr = coef.ctypes.data_as(ctypes.c_void_p)

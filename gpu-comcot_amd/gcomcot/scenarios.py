@@ -75,6 +75,11 @@ class LayerSpec:
     rel_size: int = 1
     rel_time: int = 1
     corners: tuple = (0, 0, 0, 0)   # IS, IE, JS, JE in the parent (1-based)
+    # SC_OPTION = 1 (Cartesian child of the spherical layer 1): POS (nx*ny*2)
+    # and CXY (nx*ny*4) of SUBGRID_MATCHING, column- and component-major;
+    # corners are then the POS extent
+    sc_pos: np.ndarray = None
+    sc_cxy: np.ndarray = None
 
     @property
     def cells(self):

@@ -133,6 +133,17 @@ module gcomcot_hip
       real(c_float) :: fric_coef, dt, dx, dy, coef(*), h(*), z(*)
     end function
 
+    ! a Cartesian child of the spherical layer 1 with SC_OPTION = 1: POS, CXY
+    ! and CORNERS as SUBGRID_MATCHING leaves them
+    integer(c_int) function gcomcot_init_layer_sc(id, parent, level, laygov, rel_size, &
+        rel_time, corners, fric_switch, fric_coef, dt, dx, dy, h, z, nx, ny, pos, cxy) &
+        bind(C, name="gcomcot_init_layer_sc_")
+      import :: c_int, c_float
+      integer(c_int) :: id, parent, level, laygov, rel_size, rel_time, corners(4)
+      integer(c_int) :: fric_switch, nx, ny, pos(*)
+      real(c_float) :: fric_coef, dt, dx, dy, h(*), z(*), cxy(*)
+    end function
+
     integer(c_int) function gcomcot_run_steps(nsteps, use_graph) &
         bind(C, name="gcomcot_run_steps_")
       import :: c_int

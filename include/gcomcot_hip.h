@@ -178,6 +178,21 @@ int gcomcot_set_layer_coords_(int *lid, const float *x, const float *y);
 int gcomcot_gauges_set_(int *n, const float *x0, const float *y0, const int *lid, int *capacity);
 int gcomcot_gauges_read_(float *series, int *max_steps, int *nout);
 
+/* A Cartesian layer nested in the spherical layer 1 with SC_OPTION = 1, the
+ * coupling the reference selects for such a layer (initialization.f90:
+ * 362-370; all_grids.f90:996-1425: EDGE_INTERP_SC, NEWQ_SC, JNZ_SC,
+ * CELL_INTERP_SC).  Arguments as gcomcot_init_layer_ex_ for a Cartesian
+ * layer, with corners = LA%CORNERS as SUBGRID_MATCHING leaves them (the
+ * extent of POS), plus its POS (NX,NY,2) and CXY (NX,NY,4) arrays
+ * (initialization.f90:1062-1124).  EDGE_INTERP_SC's wet tests index H and
+ * Z with stale loop variables in the reference (undefined values); here
+ * they test the boundary cell itself, as NEWQ_SC does.  Returns 0 or an
+ * error code. */
+int gcomcot_init_layer_sc_(int *layerid, int *parent, int *level, int *laygov, int *rel_size,
+                           int *rel_time, int *corners, int *fric_switch, float *fric_coef,
+                           float *dt, float *dx, float *dy, float *H, float *Z, int *nx, int *ny,
+                           int *pos, float *cxy);
+
 /* Exact checkpoints of the device state (the reference's HOT_START,
  * hotstart.f90:2-166, saves Z, M, N and the maxima as 10F12.5 text, which
  * rounds them; the GPU build does not support it, comcot.f90:328-332).

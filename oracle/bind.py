@@ -96,6 +96,7 @@ class Oracle:
             L.orc_max_io.argtypes = [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 3
             L.orc_set_arrival_threshold.argtypes = [ctypes.c_void_p, ctypes.c_double]
             L.orc_set_sponge.argtypes = [ctypes.c_void_p] * 3
+            L.orc_set_sc.argtypes = [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 3
             L.orc_cr_dt.restype = ctypes.c_double
             L.orc_cr_dt.argtypes = [ctypes.c_int] * 4 + [ctypes.c_void_p] + [ctypes.c_double] * 5
             cls._lib = L
@@ -119,6 +120,8 @@ class Oracle:
                                  s.fric_switch, s.fric_coef, s.rel_size, s.rel_time, _p(corners))
             if rc != 0:
                 raise ValueError(f"orc_add_layer({s.id}) = {rc}")
+            if getattr(s, "sc_pos", None) is not None:
+                self.set_sc(s.id, s.sc_pos, s.sc_cxy, s.corners)
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -127,6 +130,15 @@ class Oracle:
 
     def step(self, n=1):
         self.lib().orc_step(self.h, n)
+
+    def set_sc(self, lid, pos, cxy, corners):
+        """Couple child lid with SC_OPTION = 1: POS (nx*ny*2) and CXY (nx*ny*4)
+        component-major as SUBGRID_MATCHING leaves them, and its corners."""
+        p = np.ascontiguousarray(pos, np.int32).reshape(-1)
+        c = np.ascontiguousarray(cxy, np.float64).reshape(-1)
+        k = np.ascontiguousarray(corners, np.int32)
+        if self.lib().orc_set_sc(self.h, lid, _p(p), _p(c), _p(k)) != 0:
+            raise ValueError(f"orc_set_sc({lid})")
 
     def set_sponge(self, cx, cy):
         """SPONGE_LAYER after every momentum pass of layer 1, with COEFX per
@@ -308,6 +320,35 @@ class Reference:
         cy = np.empty(s.nx * s.ny, np.float32)
         self.lib().ref_sponge(_i(on), _p(cx), _p(cy))
         return cx.reshape(s.ny, s.nx), cy.reshape(s.ny, s.nx)
+
+    def sc_child(self, laygov, rel_size, rel_time, extent, dt, fric_switch=0, fric_coef=0.013):
+        """A Cartesian child of the spherical layer 1 laid out by the reference's
+        SUBGRID_MATCHING with SC_OPTION = 1 over extent (X_START, X_END,
+        Y_START, Y_END) in degrees; returns its geometry, POS and CXY."""
+        ext = np.ascontiguousarray(extent, np.float32)
+        nx, ny = ctypes.c_int(0), ctypes.c_int(0)
+        dx, dy = ctypes.c_float(0), ctypes.c_float(0)
+        corners = np.zeros(4, np.int32)
+        self.lib().ref_sc_child(_i(laygov), _i(rel_size), _i(rel_time), _p(ext), _f(dt),
+                                _i(fric_switch), _f(fric_coef), ctypes.byref(nx), ctypes.byref(ny),
+                                ctypes.byref(dx), ctypes.byref(dy), _p(corners))
+        lid = max(self.specs) + 1
+        n = nx.value * ny.value
+        pos = np.empty(2 * n, np.int32)
+        cxy = np.empty(4 * n, np.float32)
+        x = np.empty(nx.value, np.float32)
+        y = np.empty(ny.value, np.float32)
+        self.lib().ref_sc_io(_i(lid), _p(pos), _p(cxy), _p(x), _p(y))
+        geom = dict(id=lid, nx=nx.value, ny=ny.value, dx=dx.value, dy=dy.value,
+                    corners=tuple(int(c) for c in corners), pos=pos, cxy=cxy, x=x, y=y)
+        self.specs[lid] = type("Spec", (), dict(nx=nx.value, ny=ny.value))()
+        return geom
+
+    def set_hz(self, lid, h, z0):
+        """Depth and initial elevation of a layer (then PQ_DEPTH)."""
+        ha = np.ascontiguousarray(h, np.float32)
+        za = np.ascontiguousarray(z0, np.float32)
+        self.lib().ref_set_hz(_i(lid), _p(ha), _p(za))
 
     def bc_wall(self):
         """BC_WALL on layer 1 (H of the two outermost rows/columns = -999)."""

@@ -11,7 +11,8 @@ numpy:
   rectangular fault (Okada 1985) at every cell of a layer;
 - CELL_INTERP (all_grids.f90:859-934): the bilinear spreading of a
   parent's deformation onto a nested layer, as ININTERP (:805-855) applies
-  it level by level.
+  it level by level; CELL_INTERP_SC (:938-992) for a Cartesian child with
+  SC_OPTION = 1, through its POS/CXY.
 
 The fault model is evaluated in FP64 with the reference's REAL constants
 (each literal rounded to float first, as the Fortran holds it; the
@@ -208,4 +209,25 @@ def cell_interp(parent, child_shape, corners, rel_size):
                     cj = (j - JS) * ir + 1 + m
                     if ci <= cnx and cj <= cny:
                         out[cj - 1, ci - 1] = v
+    return out
+
+
+def cell_interp_sc(parent, pos, cxy, cnx, cny):
+    """CELL_INTERP_SC (all_grids.f90:938-992) in REAL: each child cell takes
+    the CXY-weighted sum of the four parent cells at POS (0 where POS lies
+    outside the parent)."""
+    P = np.asarray(parent, np.float32)
+    pny, pnx = P.shape
+    pos = np.asarray(pos, np.int32).reshape(2, cny, cnx)
+    cxy = np.asarray(cxy, np.float32).reshape(4, cny, cnx)
+    out = np.zeros((cny, cnx), np.float32)
+    for j in range(cny):
+        for i in range(cnx):
+            ki, kj = int(pos[0, j, i]), int(pos[1, j, i])
+            if 1 <= ki < pnx and 1 <= kj < pny:
+                z1 = P[kj - 1, ki - 1] * cxy[0, j, i]
+                z2 = P[kj - 1, ki] * cxy[1, j, i]
+                z3 = P[kj, ki - 1] * cxy[2, j, i]
+                z4 = P[kj, ki] * cxy[3, j, i]
+                out[j, i] = ((z1 + z2) + z3) + z4
     return out

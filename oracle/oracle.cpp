@@ -59,6 +59,11 @@ struct Layer {
   std::vector<T> xflux, yflux;            // (nx,2), (ny,2)
   std::vector<T> zmax_wet, zmax_raw;
   std::vector<T> sponge_cx, sponge_cy;    // COEFX per column, COEFY per row (layer 1)
+  // SC_OPTION = 1 (Cartesian child of a spherical parent): POS(NX,NY,2) and
+  // CXY(NX,NY,4) of SUBGRID_MATCHING, component-major like the Fortran
+  bool sc = false;
+  std::vector<int> pos;
+  std::vector<T> cxy;
   std::vector<int> arrival;
   std::vector<int> children;              // indices into State::layers
 
@@ -899,16 +904,164 @@ struct State {
       Layer<T> &C = layers[ci];
       int nhalf = (int)std::floor(C.rel_time / 2.0f) + 1;
       for (int k = 1; k <= C.rel_time; ++k) {
-        if (k == 1) jnq(P, C);
-        else newq(P, C, k);
+        if (k == 1) {
+          if (C.sc) jnq_sc(P, C); else jnq(P, C);
+        } else {
+          if (C.sc) newq_sc(P, C, k); else newq(P, C, k);
+        }
         mass(C);
         all_grid_children(C);
         moment(C);
         max_arrival(C);
-        if (k == nhalf) jnz(P, C);
+        if (k == nhalf) {
+          if (C.sc) jnz_sc(P, C); else jnz(P, C);
+        }
         update(C);
       }
     }
+  }
+
+  // ------------------------------------------- SC_OPTION = 1 coupling
+  int pos_of(const Layer<T> &C, int i, int j, int c) const {
+    return C.pos[(size_t)c * C.nxy() + C.ix(i, j)];
+  }
+  T cxy_of(const Layer<T> &C, int i, int j, int c) const {
+    return C.cxy[(size_t)c * C.nxy() + C.ix(i, j)];
+  }
+  // The four-corner weighted sum of a parent plane at child cell (i, j),
+  // Z1+Z2+Z3+Z4 in the Fortran's order (all_grids.f90:1029-1034)
+  T sc_sum(Layer<T> &P, const Layer<T> &C, int i, int j, bool n_not_m, int level) {
+    const int ki = pos_of(C, i, j, 0), kj = pos_of(C, i, j, 1);
+    auto F = [&](int a, int b) -> T & { return n_not_m ? P.N(a, b, level) : P.M(a, b, level); };
+    const T z1 = F(ki, kj) * cxy_of(C, i, j, 0);
+    const T z2 = F(ki + 1, kj) * cxy_of(C, i, j, 1);
+    const T z3 = F(ki, kj + 1) * cxy_of(C, i, j, 2);
+    const T z4 = F(ki + 1, kj + 1) * cxy_of(C, i, j, 3);
+    return z1 + z2 + z3 + z4;
+  }
+
+  // EDGE_INTERP_SC (all_grids.f90:996-1091).  Its wet tests index H and Z
+  // with loop variables left over from other loops (undefined values); the
+  // boundary cell itself is tested here, as NEWQ_SC does.
+  void jnq_sc(Layer<T> &P, Layer<T> &C) {
+    const T GX = T(kGx);
+    auto wet = [&](int i, int j) { return C.H(i, j) + C.Z(i, j, 2) > GX; };
+    for (int j = 1; j <= C.ny; ++j) C.M(1, j, 1) = wet(1, j) ? sc_sum(P, C, 1, j, false, 1) : T(0);
+    for (int j = 1; j <= C.ny; ++j)
+      C.M(C.nx, j, 1) = wet(C.nx, j) ? sc_sum(P, C, C.nx, j, false, 1) : T(0);
+    for (int i = 1; i <= C.nx; ++i) C.N(i, 1, 1) = wet(i, 1) ? sc_sum(P, C, i, 1, true, 1) : T(0);
+    for (int i = 1; i <= C.nx; ++i)
+      C.N(i, C.ny, 1) = wet(i, C.ny) ? sc_sum(P, C, i, C.ny, true, 1) : T(0);
+  }
+
+  // NEWQ_SC's forecast of the parent's linear fluxes on the bands around the
+  // child's edges (all_grids.f90:1131-1271): MOMT_S with the SCM term, into
+  // the parent's level 2
+  void sc_band_x(Layer<T> &P, int is, int ie, int js, int je) {
+    const T GX = T(kGx), EPS = T(kEps), TW = T(0.08333333333333f);  // TWLVTH (all_grids.f90:1105)
+    for (int i = is; i <= ie; ++i) {
+      const int ip1 = i + 1;
+      for (int j = js; j <= je; ++j) {
+        if (P.H(i, j) > GX && P.H(ip1, j) > GX) {
+          int jm1 = j - 1, jp1 = j + 1;
+          if (jm1 <= 1) jm1 = 1;
+          if (jp1 >= P.ny) jp1 = P.ny;
+          const T tot_n = P.N(i, j, 1) + P.N(ip1, j, 1) + P.N(i, jm1, 1) + P.N(ip1, jm1, 1);
+          T xm = P.M(i, j, 1) - P.R(P.r2, i, j) * (P.Z(ip1, j, 2) - P.Z(i, j, 2)) +
+                 P.R(P.r3, i, j) * tot_n;
+          if (P.modscm == 0) {
+            const T scm = P.R(P.r2, i, j) * TW *
+                          ((P.Z(ip1, jp1, 2) - T(2) * P.Z(ip1, j, 2) + P.Z(ip1, jm1, 2)) -
+                           (P.Z(i, jp1, 2) - T(2) * P.Z(i, j, 2) + P.Z(i, jm1, 2)));
+            xm = xm - scm;
+          }
+          if (m_abs(xm) < EPS) xm = T(0);
+          P.M(i, j, 2) = xm;
+        } else {
+          P.M(i, j, 2) = T(0);
+        }
+      }
+    }
+  }
+  void sc_band_y(Layer<T> &P, int is, int ie, int js, int je) {
+    const T GX = T(kGx), EPS = T(kEps), TW = T(0.08333333333333f);  // TWLVTH (all_grids.f90:1105)
+    for (int j = js; j <= je; ++j) {
+      const int jp1 = j + 1;
+      for (int i = is; i <= ie; ++i) {
+        if (P.H(i, j) > GX && P.H(i, jp1) > GX) {
+          int im1 = i - 1, ip1 = i + 1;
+          if (im1 <= 1) im1 = 1;
+          if (ip1 >= P.nx) ip1 = P.nx;
+          const T tot_m = P.M(im1, j, 1) + P.M(im1, jp1, 1) + P.M(i, j, 1) + P.M(i, jp1, 1);
+          T xn = P.N(i, j, 1) - P.R(P.r4, i, j) * (P.Z(i, jp1, 2) - P.Z(i, j, 2)) -
+                 P.R(P.r5, i, j) * tot_m;
+          if (P.modscm == 0) {
+            const T scm = P.R(P.r4, i, j) * TW *
+                          ((P.Z(ip1, jp1, 2) - T(2) * P.Z(i, jp1, 2) + P.Z(im1, jp1, 2)) -
+                           (P.Z(ip1, j, 2) - T(2) * P.Z(i, j, 2) + P.Z(im1, j, 2)));
+            xn = xn - scm;
+          }
+          if (m_abs(xn) < EPS) xn = T(0);
+          P.N(i, j, 2) = xn;
+        } else {
+          P.N(i, j, 2) = T(0);
+        }
+      }
+    }
+  }
+
+  // NEWQ_SC (all_grids.f90:1095-1373), the top edge with its C1*YN1 (App. Q3)
+  void newq_sc(Layer<T> &P, Layer<T> &C, int t) {
+    const T GX = T(kGx);
+    const int nx = C.nx, ny = C.ny;
+    auto p0 = [&](int i, int j) { return pos_of(C, i, j, 0); };
+    auto p1 = [&](int i, int j) { return pos_of(C, i, j, 1); };
+    sc_band_x(P, std::min(p0(1, 1), p0(1, ny)) - 2, std::max(p0(1, 1), p0(1, ny)) + 2,
+              C.corners[2] - 2, C.corners[3] + 2);
+    sc_band_x(P, std::min(p0(nx, 1), p0(nx, ny)) - 2, std::max(p0(nx, 1), p0(nx, ny)) + 2,
+              C.corners[2] - 2, C.corners[3] + 2);
+    sc_band_y(P, C.corners[0] - 2, C.corners[1] + 2, std::min(p1(1, ny), p1(nx, ny)) - 2,
+              std::max(p1(1, ny), p1(nx, ny)) + 2);
+    sc_band_y(P, C.corners[0] - 2, C.corners[1] + 2, std::min(p1(1, 1), p1(nx, 1)) - 2,
+              std::max(p1(1, 1), p1(nx, 1)) + 2);
+    const T c1 = (T(t) - T(1.0f)) / T(C.rel_time);
+    const T c2 = T(1.0f) - c1;
+    auto wet = [&](int i, int j) { return C.H(i, j) + C.Z(i, j, 2) > GX; };
+    for (int j = 1; j <= ny; ++j)
+      C.M(1, j, 1) = wet(1, j) ? c1 * sc_sum(P, C, 1, j, false, 2) + c2 * sc_sum(P, C, 1, j, false, 1)
+                               : T(0);
+    for (int j = 1; j <= ny; ++j)
+      C.M(nx, j, 1) = wet(nx, j) ? c1 * sc_sum(P, C, nx, j, false, 2) +
+                                       c2 * sc_sum(P, C, nx, j, false, 1)
+                                 : T(0);
+    for (int i = 1; i <= nx; ++i)
+      C.N(i, 1, 1) = wet(i, 1) ? c1 * sc_sum(P, C, i, 1, true, 2) + c2 * sc_sum(P, C, i, 1, true, 1)
+                               : T(0);
+    for (int i = 1; i <= nx; ++i)
+      C.N(i, ny, 1) = wet(i, ny) ? c1 * sc_sum(P, C, i, ny, true, 2) +
+                                       c1 * sc_sum(P, C, i, ny, true, 1)
+                                 : T(0);
+  }
+
+  // JNZ_SC (all_grids.f90:1376-1425): wet child cells averaged into the
+  // parent cell that POS names, over the SC footprint; cells with no more
+  // than half of R*R contributors keep their value
+  void jnz_sc(Layer<T> &P, Layer<T> &C) {
+    const T GX = T(kGx);
+    const T half = T((float)(C.rel_size * C.rel_size) / 2.0f);
+    std::vector<T> z(P.nxy(), T(0)), cnt(P.nxy(), T(0));
+    for (int i = 1; i <= C.nx; ++i)
+      for (int j = 1; j <= C.ny; ++j)
+        if (C.H(i, j) + C.Z(i, j, 2) > GX) {
+          const size_t q = P.ix(pos_of(C, i, j, 0), pos_of(C, i, j, 1));
+          z[q] = z[q] + C.Z(i, j, 2);
+          cnt[q] = cnt[q] + T(1.0f);
+        }
+    for (int i = C.corners[0]; i <= C.corners[1]; ++i)
+      for (int j = C.corners[2]; j <= C.corners[3]; ++j) {
+        const size_t q = P.ix(i, j);
+        if (cnt[q] > half) P.Z(i, j, 2) = z[q] / cnt[q];
+      }
   }
 
   // SPONGE_LAYER (boundaries.f90:206-246): damping of the new fluxes where
@@ -1043,9 +1196,9 @@ void call_op(State<T> &S, int op, int id, int t) {
     case 1: S.mass(L); break;
     case 2: S.open_bc(L); break;
     case 3: S.moment(L); break;
-    case 4: if (P) S.jnq(*P, L); break;
-    case 5: if (P) S.newq(*P, L, t); break;
-    case 6: if (P) S.jnz(*P, L); break;
+    case 4: if (P) { if (L.sc) S.jnq_sc(*P, L); else S.jnq(*P, L); } break;
+    case 5: if (P) { if (L.sc) S.newq_sc(*P, L, t); else S.newq(*P, L, t); } break;
+    case 6: if (P) { if (L.sc) S.jnz_sc(*P, L); else S.jnz(*P, L); } break;
     case 7: S.update(L); break;
     case 8: S.all_grid_children(S.layers[0]); break;
     case 9: {
@@ -1153,6 +1306,25 @@ void orc_max_io(void *s, int id, double *zmax_wet, double *zmax_raw, int *arriva
   Handle *H = (Handle *)s;
   if (H->precision == 32) max_io_t(*H->f, id, zmax_wet, zmax_raw, arrival);
   else max_io_t(*H->d, id, zmax_wet, zmax_raw, arrival);
+}
+
+// Mark child id as SC_OPTION = 1 with SUBGRID_MATCHING's POS (NX*NY*2) and
+// CXY (NX*NY*4), component-major, and its corners (the POS extent)
+int orc_set_sc(void *s, int id, const int *pos, const double *cxy, const int *corners) {
+  Handle *H = (Handle *)s;
+  auto set = [&](auto &st) {
+    const int k = st.find(id);
+    if (k <= 0) return 1;
+    auto &L = st.layers[k];
+    const size_t n = L.nxy();
+    L.sc = true;
+    L.pos.assign(pos, pos + 2 * n);
+    L.cxy.resize(4 * n);
+    for (size_t q = 0; q < 4 * n; ++q) L.cxy[q] = (decltype(L.cxy[0] + 0))cxy[q];
+    for (int c = 0; c < 4; ++c) L.corners[c] = corners[c];
+    return 0;
+  };
+  return H->precision == 32 ? set(*H->f) : set(*H->d);
 }
 
 void orc_set_sponge(void *s, const double *cx, const double *cy) {

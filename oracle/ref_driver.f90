@@ -485,6 +485,84 @@ subroutine ref_bc_wall() bind(C, name="ref_bc_wall")
   call bc_wall(lo, w)
 end subroutine ref_bc_wall
 
+!.....a Cartesian child of the spherical layer 1 laid out by the reference's
+!     own SUBGRID_MATCHING (initialization.f90:882-1131) from its extent in
+!     degrees (X_START, X_END, Y_START, Y_END), with SC_OPTION = 1 (the
+!     coupling the reference selects for such a child, initialization.f90:
+!     362-370): grid size and spacing, POS/CXY and the corners it sets
+subroutine ref_sc_child(laygov, rel_size, rel_time, xyse, dt, fric_switch, fric_coef,  &
+                        nx, ny, dx, dy, corners) bind(C, name="ref_sc_child")
+  implicit none
+  integer :: laygov, rel_size, rel_time, fric_switch, nx, ny, corners(4)
+  real :: xyse(4), dt, fric_coef, dx, dy
+  nla = nla + 1
+  associate (c => la(nla))
+  c%id = nla + 1
+  c%parent = 1
+  c%level = 2
+  lo%num_child = lo%num_child + 1
+  c%rel_size = rel_size
+  c%rel_time = rel_time
+  c%laycord = 1
+  c%laygov = laygov
+  c%layswitch = 0
+  c%dt = dt
+  c%ini_switch = 0
+  c%bc_type = 0
+  c%dim = 2
+  c%fric_switch = fric_switch
+  c%fric_coef = fric_coef
+  c%h_limit = 0.0
+  c%tide_level = 0.0
+  c%num_child = 0
+  c%upz = .true.
+  c%sc_option = 1
+  c%x_start = xyse(1)
+  c%x_end = xyse(2)
+  c%y_start = xyse(3)
+  c%y_end = xyse(4)
+  call subgrid_matching(lo, c)
+  call alloc(c, 2)
+  nx = c%nx
+  ny = c%ny
+  dx = c%dx
+  dy = c%dy
+  corners = c%corners
+  end associate
+end subroutine ref_sc_child
+
+!.....POS (NX*NY*2), CXY (NX*NY*4), X (NX), Y (NY) of an SC_OPTION = 1 child
+subroutine ref_sc_io(id, pos, cxy, x, y) bind(C, name="ref_sc_io")
+  implicit none
+  integer :: id, pos(*)
+  real :: cxy(*), x(*), y(*)
+  integer :: k, n
+  call ref_pick(id, k)
+  n = la(k)%nx * la(k)%ny
+  pos(1:2*n) = reshape(la(k)%pos, (/2*n/))
+  cxy(1:4*n) = reshape(la(k)%cxy, (/4*n/))
+  x(1:la(k)%nx) = la(k)%x
+  y(1:la(k)%ny) = la(k)%y
+end subroutine ref_sc_io
+
+!.....depth and initial elevation of a layer (then PQ_DEPTH)
+subroutine ref_set_hz(id, h, z0) bind(C, name="ref_set_hz")
+  implicit none
+  integer :: id
+  real :: h(*), z0(*)
+  integer :: k
+  call ref_pick(id, k)
+  if (k .eq. 0) then
+     lo%h = reshape(h(1:lo%nx*lo%ny), (/lo%nx, lo%ny/))
+     lo%z(:,:,1) = reshape(z0(1:lo%nx*lo%ny), (/lo%nx, lo%ny/))
+     call pq_depth(lo)
+  else
+     la(k)%h = reshape(h(1:la(k)%nx*la(k)%ny), (/la(k)%nx, la(k)%ny/))
+     la(k)%z(:,:,1) = reshape(z0(1:la(k)%nx*la(k)%ny), (/la(k)%nx, la(k)%ny/))
+     call pq_depth(la(k))
+  end if
+end subroutine ref_set_hz
+
 !.....read DEFORM and H of layer id
 subroutine ref_deform_io(id, deform, h) bind(C, name="ref_deform_io")
   implicit none

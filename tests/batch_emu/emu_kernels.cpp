@@ -185,6 +185,10 @@ hipError_t launch_gauges(const GaugeDev *, int, const ZNow &, const int *, float
                          hipStream_t) {
   return hipSuccess;  // gauges belong to the single-tree device loop, not the batch planner
 }
+hipError_t launch_sc_edges(const DevSc &, int, hipStream_t) {
+  return hipSuccess;  // SC_OPTION 1 children belong to the single-tree device loop
+}
+hipError_t launch_sc_jnz(const DevSc &, hipStream_t) { return hipSuccess; }
 int tree_apply_fault(Tree &, hipStream_t, const float *, int) {
   return 1;  // fault sources run on the device only; the planner tests do not use them
 }

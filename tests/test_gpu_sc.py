@@ -1,0 +1,74 @@
+"""GPU: SC_OPTION = 1 coupling in the device loop, bit for bit.
+
+A Cartesian child of the spherical layer 1, laid out by the reference's
+SUBGRID_MATCHING (POS, CXY), runs on the device with EDGE_INTERP_SC,
+NEWQ_SC (band forecasts written into the parent's level 2, the edges
+blended from them) and JNZ_SC (parent elevation averaged over the wet child
+cells of each parent cell); the oracle, pinned against the reference's own
+NEWQ_SC and JNZ_SC in tests/test_sc_oracle.py, must give the same bits for
+every plane of both layers, the maxima and the arrival field.  A fault
+spread to the child goes through CELL_INTERP_SC.  Batched scenarios
+refuse SC_OPTION = 1 nests.
+"""
+import numpy as np
+import pytest
+
+from gcomcot import DeviceRun
+from helpers import diff_report, sc_nest
+from oracle import okada as OK
+from oracle.bind import Oracle, Reference
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not Reference.available(), reason="needs the flang-built reference")]
+
+
+# REL_TIME 1 puts JNZ_SC in the first sub-step, before any UPDATE has
+# filled the child's ghost row and column
+@pytest.mark.parametrize("child_gov,rel_size,rel_time", [(1, 3, 3), (0, 3, 2), (1, 1, 1)])
+def test_sc_nest_matches_the_oracle(child_gov, rel_size, rel_time):
+    sc, _ = sc_nest(laygov_child=child_gov, rel_size=rel_size, rel_time=rel_time)
+    n = 40
+    print(f"[gpu] SC nest child laygov {child_gov} REL_SIZE {rel_size} REL_TIME {rel_time}, "
+          f"{n} steps", flush=True)
+    dev = DeviceRun(sc)
+    dev.run(n // 2 + 1)
+    dev.run(n - n // 2 - 1, graph=False)
+    orc = Oracle(sc, 32, pow_mode=1)
+    orc.step(n)
+    for L in sc.layers:
+        g, o = dev.state(L.id), orc.state(L.id)
+        for k in ("z1", "m1", "n1"):
+            nb, mx = diff_report(g[k], o[k])
+            assert nb == 0, f"layer {L.id} {k}: {nb} cells differ (max {mx})"
+        gm, om = dev.maxima(L.id), orc.maxima(L.id)
+        assert diff_report(gm["zmax_wet"], om["zmax_wet"])[0] == 0
+        assert np.array_equal(gm["arrival"], om["arrival"])
+    dev.close()
+
+
+def test_fault_spreads_to_an_sc_child():
+    sc, _ = sc_nest()
+    R, C = sc.layers
+    from gcomcot import scenarios as S
+    x, y = S.layer_coords(R)
+    f = OK.Fault(10000, 100000, 50000, 5, 30, 15, 90, x0=float(x[40]), y0=float(y[36]))
+    print("[gpu] fault on an SC nest", flush=True)
+    dev = DeviceRun(sc)
+    dev.set_coords([1])
+    dev.fault(f.params(), f.switch)
+    d_root, d_child = dev.deform(1), dev.deform(2)
+    dev.close()
+    want = OK.cell_interp_sc(d_root, C.sc_pos, C.sc_cxy, C.nx, C.ny)
+    nb, mx = diff_report(d_child, want)
+    assert nb == 0, f"CELL_INTERP_SC: {nb} cells differ (max {mx})"
+    assert np.abs(d_child).max() > 0.01
+
+
+def test_batch_refuses_an_sc_nest():
+    from gcomcot._lib import GcomcotError
+    from gcomcot.batch import Batch
+    sc, _ = sc_nest()
+    b = Batch()
+    with pytest.raises(GcomcotError, match="SC_OPTION"):
+        b.add(sc, 4)
+    b.close()
