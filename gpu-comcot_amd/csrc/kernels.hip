@@ -23,6 +23,7 @@
 
 #include "comcot_consts.h"
 #include "detmath.h"
+#include "fastdiv.h"
 #include "kernels.h"
 
 namespace gc {
@@ -339,13 +340,13 @@ __device__ float nl_x(const DevLayer &L, const A &s, int i, int j, float brk) {
   const float p = s.M(i, j);
   const float xqq = 0.25f * (s.N(i, j) + s.N(ip1, j) + s.N(i, jm1) + s.N(ip1, jm1));
   float FF = 0.0f;
-  if (L.fric) FF = L.kfric * sqrtf(p * p + xqq * xqq) / gcdet::powf_manning(DF, kManExp, *s.tab);
+  if (L.fric) FF = div_inrange(L.kfric * sqrtf(p * p + xqq * xqq), gcdet::powf_manning(DF, kManExp, *s.tab));
   float XP;
   if (SPH) {
     const float r2 = s.CF(L, kF2, j);
     if (L.modscm == 0 && (IN || j < JY)) {
       XP = p - r2 * DD * (s.Z2(ip1, j) - s.Z2(i, j)) -
-           r2 * DD / 12.0f *
+           div12(r2 * DD) *
                ((s.Z2(ip1, jp1) - 2.0f * s.Z2(ip1, j) + s.Z2(ip1, jm1)) -
                 (s.Z2(i, jp1) - 2.0f * s.Z2(i, j) + s.Z2(i, jm1)));
     } else {
@@ -356,7 +357,7 @@ __device__ float nl_x(const DevLayer &L, const A &s, int i, int j, float brk) {
   } else {
     XP = p - L.grx * DD * (s.Z2(ip1, j) - s.Z2(i, j));
     if (L.modscm == 0 && (IN || j < JY)) {
-      float scm = L.grx * DD / 12.0f *
+      float scm = div12(L.grx * DD) *
                   ((s.Z2(ip1, jp1) - 2.0f * s.Z2(ip1, j) + s.Z2(ip1, jm1)) -
                    (s.Z2(i, jp1) - 2.0f * s.Z2(i, j) + s.Z2(i, jm1)));
       XP = XP - scm;
@@ -374,21 +375,21 @@ __device__ float nl_x(const DevLayer &L, const A &s, int i, int j, float brk) {
     const int ia = pn ? ip1 : im1;
     const float da = dp1_at<IN>(L, s, ia, j);
     const bool degx = pn ? (da < kGx || s.DZ(ip1, j) < kGx) : (da < kGx || s.DZ(i, j) < kGx);
-    const float ax = p * p / d;
+    const float ax = div_inrange(p * p, d);
     const float pa = s.M(ia, j);
-    const float bx = pa * pa / da;
+    const float bx = div_inrange(pa * pa, da);
     const float advx = fx * (degx ? (pn ? -ax : ax) : (pn ? bx - ax : ax - bx));
     const bool qn = xqq < 0.0f;
     const int jn = qn ? jp1 : jm1, jf = qn ? j : j - 2;
     const float dn = dp1_at<IN>(L, s, i, jn);
     const bool degy = s.DZ(i, jn) < kGx || s.DZ(ip1, jn) < kGx || dn < kGx;
-    const float ay = p * xqq / d;
+    const float ay = div_inrange(p * xqq, d);
     const float xqe = 0.25f * (s.N(i, jn) + s.N(ip1, jn) + s.N(i, jf) + s.N(ip1, jf));
-    const float by = s.M(i, jn) * xqe / dn;
+    const float by = div_inrange(s.M(i, jn) * xqe, dn);
     float advy = fy * (degy ? (qn ? -ay : ay) : (qn ? by - ay : ay - by));
     XP = XP - brk * advx - brk * advy;
   }
-  if (L.fric) XP = XP / (1.0f + FF);
+  if (L.fric) XP = div_inrange(XP, 1.0f + FF);
   if (fabsf(XP) < kEps) XP = 0.0f;
   const float lim = kVLimit * DD;
   if (XP > lim) XP = lim;
@@ -428,13 +429,13 @@ __device__ float nl_y(const DevLayer &L, const A &s, int i, int j, float brk) {
   const float q = s.N(i, j);
   const float xpp = 0.25f * (s.M(i, j) + s.M(i, jp1) + s.M(im1, j) + s.M(im1, jp1));
   float FF = 0.0f;
-  if (L.fric) FF = L.kfric * sqrtf(q * q + xpp * xpp) / gcdet::powf_manning(DF, kManExp, *s.tab);
+  if (L.fric) FF = div_inrange(L.kfric * sqrtf(q * q + xpp * xpp), gcdet::powf_manning(DF, kManExp, *s.tab));
   float XQ;
   if (SPH) {
     const float r4 = s.CF(L, kF4, j);
     if (L.modscm == 0 && (IN || i < IX)) {
       XQ = q - r4 * DD * (s.Z2(i, jp1) - s.Z2(i, j)) -
-           r4 * DD / 12.0f *
+           div12(r4 * DD) *
                ((s.Z2(ip1, jp1) - 2.0f * s.Z2(i, jp1) + s.Z2(im1, jp1)) -
                 (s.Z2(ip1, j) - 2.0f * s.Z2(i, j) + s.Z2(im1, j)));
     } else {
@@ -445,7 +446,7 @@ __device__ float nl_y(const DevLayer &L, const A &s, int i, int j, float brk) {
   } else {
     XQ = q - L.gry * DD * (s.Z2(i, jp1) - s.Z2(i, j));
     if (L.modscm == 0 && (IN || i < IX)) {
-      float scm = L.gry * DD / 12.0f *
+      float scm = div12(L.gry * DD) *
                   ((s.Z2(ip1, jp1) - 2.0f * s.Z2(i, jp1) + s.Z2(im1, jp1)) -
                    (s.Z2(ip1, j) - 2.0f * s.Z2(i, j) + s.Z2(im1, j)));
       XQ = XQ - scm;
@@ -460,22 +461,22 @@ __device__ float nl_y(const DevLayer &L, const A &s, int i, int j, float brk) {
     const int ja = qn ? jp1 : jm1;
     const float da = dq1_at<IN>(L, s, i, ja);
     const bool degy = qn ? (da < kGx || s.DZ(i, jp1) < kGx) : (da < kGx || s.DZ(i, j) < kGx);
-    const float ay = q * q / d;
+    const float ay = div_inrange(q * q, d);
     const float qa = s.N(i, ja);
-    const float by = qa * qa / da;
+    const float by = div_inrange(qa * qa, da);
     float advy = fy * (degy ? (qn ? -ay : ay) : (qn ? by - ay : ay - by));
     const bool pn = xpp < 0.0f;
     const int in = pn ? ip1 : im1, ifar = pn ? i : i - 2;
     const float dn = dq1_at<IN>(L, s, pn ? ip1 : i - 1, j);
     const bool degx = pn ? (s.DZ(ip1, j) < kGx || s.DZ(ip1, jp1) < kGx || dn < kGx)
                          : (s.DZ(im1, j) < kGx || s.DZ(im1, jp1) < kGx || dn < kGx);
-    const float ax = q * xpp / d;
+    const float ax = div_inrange(q * xpp, d);
     const float xpe = 0.25f * (s.M(in, j) + s.M(in, jp1) + s.M(ifar, j) + s.M(ifar, jp1));
-    const float bx = s.N(in, j) * xpe / dq1_at<IN>(L, s, in, j);
+    const float bx = div_inrange(s.N(in, j) * xpe, dq1_at<IN>(L, s, in, j));
     float advx = fx * (degx ? (pn ? -ax : ax) : (pn ? bx - ax : ax - bx));
     XQ = XQ - brk * advx - brk * advy;
   }
-  if (L.fric) XQ = XQ / (1.0f + FF);
+  if (L.fric) XQ = div_inrange(XQ, 1.0f + FF);
   if (fabsf(XQ) < kEps) XQ = 0.0f;
   const float lim = kVLimit * DD;
   if (XQ > lim) XQ = lim;
@@ -652,8 +653,17 @@ __device__ __forceinline__ void tile_step(const DevLayer &L, int t, int tiles_x,
   else tile_body<MODE, NL, SPH, TX, TY, false>(L, t, ti0, tj0, sm, stab);
 }
 
+// Waves per SIMD the register allocation of the step kernels aims for
+// (at most 72 VGPRs).  The nonlinear tile needs 87 unconstrained, i.e. 5
+// waves; with 7 a few values spill and the step is ~14% faster, since its
+// long dependent chains (divisions, square roots, FP64 powers) want more
+// waves to hide latency.  Register allocation never changes a result.
+#ifndef GC_STEP_WAVES
+#define GC_STEP_WAVES 7
+#endif
+
 template <int MODE, bool NL, bool SPH, int TX, int TY>
-__global__ void __launch_bounds__(kThreads)
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(GC_STEP_WAVES)))
     k_step(DevLayer L, int tiles_x, int ntiles, int per_xcd) {
   __shared__ float sm[6 * (TX + 4) * (TY + 4) + kNumCoef * (TY + 4)];
   __shared__ gcdet::Tables stab;
@@ -841,7 +851,7 @@ struct Ring {
 };
 
 template <bool NL, bool SPH>
-__global__ void __launch_bounds__(kThreads)
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(GC_STEP_WAVES)))
     k_stream(DevLayer L, int nstrip, int nseg, int seg, int per_xcd) {
   __shared__ float sm[(6 * 3 + 5 * 2) * kStreamW];
   const int t = tile_of_block(per_xcd);
@@ -1184,6 +1194,59 @@ __global__ void __launch_bounds__(256) k_copy(const float4 *__restrict__ src,
     dst[i] = src[i];
 }
 
+// Self-check of the short divisions against IEEE division on the device.
+// Lane g owns a slice of the 2^32 floats for div12 (mismatches counted
+// where |x| >= 2^-124 or x = 0) and `per_lane` pseudo-random pairs (a, b)
+// for div_inrange, drawn with exponents spread over the range in which it
+// claims equality (pairs outside it are skipped).  Lane g writes out[4g..4g+3]:
+// the two counts, the bits of its first mismatching x and of its first
+// mismatching pair (a << 32 | b); the host adds the counts up.
+__device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+__device__ __forceinline__ float f_of(unsigned u) { return __uint_as_float(u); }
+
+__global__ void __launch_bounds__(256) k_division_check(unsigned long long seed, int per_lane,
+                                                        unsigned long long *out) {
+  const unsigned long long g = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const unsigned long long lanes = (unsigned long long)gridDim.x * blockDim.x;
+  unsigned long long bad12 = 0, bad = 0, ex12 = 0, ex = 0;
+  const unsigned long long span = ((1ull << 32) + lanes - 1) / lanes;
+  for (unsigned long long k = g * span; k < (g + 1) * span && k < (1ull << 32); ++k) {
+    const float x = f_of((unsigned)k);
+    if (!(fabsf(x) <= 3.4028235e38f)) continue;  // NaN, infinities
+    if (!(fabsf(x) >= 0x1p-124f) && x != 0.0f) continue;
+    const float want = x / 12.0f, got = div12(x);
+    if (__float_as_uint(want) != __float_as_uint(got)) {
+      if (!bad12) ex12 = k;
+      ++bad12;
+    }
+  }
+  for (int k = 0; k < per_lane; ++k) {
+    const unsigned long long r = mix64(seed + g * 0x9e3779b97f4a7c15ull + (unsigned long long)k);
+    // exponents: b in [-kDivRange, kDivRange], a within 100 of it
+    const int eb = (int)((r >> 32) % (2u * kDivRange + 1u)) - kDivRange;
+    const int ea = eb + (int)(((r >> 40) & 0xffu) % 201u) - 100;
+    if (ea < -126 || ea > 127) continue;
+    const unsigned ma = (unsigned)(r & 0x7fffffu), mb = (unsigned)((r >> 8) & 0x7fffffu);
+    const unsigned sa = (unsigned)((r >> 48) & 1u) << 31, sb = (unsigned)((r >> 49) & 1u) << 31;
+    const float a = f_of(sa | ((unsigned)(ea + 127) << 23) | ma);
+    const float b = f_of(sb | ((unsigned)(eb + 127) << 23) | mb);
+    const float want = a / b;
+    if (!div_inrange_applies(a, b, want)) continue;
+    if (__float_as_uint(want) != __float_as_uint(div_inrange(a, b))) {
+      if (!bad) ex = ((unsigned long long)__float_as_uint(a) << 32) | __float_as_uint(b);
+      ++bad;
+    }
+  }
+  out[4 * g] = bad12;
+  out[4 * g + 1] = bad;
+  out[4 * g + 2] = ex12;
+  out[4 * g + 3] = ex;
+}
+
 // Dry land shows 0, else the elevation plus the tide correction when it is
 // not negligible: DATA_PRT (output.f90:781-793) and MAX_AMP (:918-931).
 __device__ __forceinline__ float shown_elevation(float z, float h, float tide) {
@@ -1510,7 +1573,7 @@ __global__ void k_fill_i32(int *p, int n, int v) {
 // claim on it.  Every workgroup leaves once all 8 heads have passed their
 // ends.
 template <int MODE, bool NL, bool SPH, int TX, int TY>
-__global__ void __launch_bounds__(kThreads)
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(GC_STEP_WAVES)))
     k_batch(const BatchDesc *__restrict__ descs, const BatchItem *__restrict__ items,
             const int *__restrict__ qstart, int *heads) {
   __shared__ float sm[6 * (TX + 4) * (TY + 4) + kNumCoef * (TY + 4)];
@@ -1550,7 +1613,7 @@ __global__ void __launch_bounds__(kThreads)
 // the balancing): item (b % 8) * per_xcd + b / 8, so each XCD takes one
 // contiguous eighth of the items, as the queues split them.
 template <int MODE, bool NL, bool SPH, int TX, int TY>
-__global__ void __launch_bounds__(kThreads)
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(GC_STEP_WAVES)))
     k_batch_grid(const BatchDesc *__restrict__ descs, const BatchItem *__restrict__ items,
                  int nitems, int per_xcd) {
   __shared__ float sm[6 * (TX + 4) * (TY + 4) + kNumCoef * (TY + 4)];
@@ -1770,6 +1833,12 @@ hipError_t launch_jnz_multi(const DevNest *nests, const int *start, int nnest, i
                             hipStream_t st) {
   if (nnest <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_jnz_multi, dim3(nblocks), dim3(128), 0, st, nests, start, nnest);
+  return hipGetLastError();
+}
+
+hipError_t launch_division_check(unsigned long long seed, int per_lane, unsigned long long *out,
+                                 int blocks, hipStream_t st) {
+  hipLaunchKernelGGL(k_division_check, dim3(blocks), dim3(256), 0, st, seed, per_lane, out);
   return hipGetLastError();
 }
 
