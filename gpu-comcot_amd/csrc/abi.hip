@@ -680,37 +680,41 @@ int gcomcot_hbm_copy_(int *mib, int *reps, float *ms_per_copy) {
 }
 
 int gcomcot_division_check_(long long *pairs, unsigned long long *seed, long long *bad_div12,
-                            long long *bad_inrange, long long *tested, unsigned *example_x,
-                            unsigned *example_a, unsigned *example_b) {
+                            long long *bad_inrange, long long *bad_sqrt, long long *tested,
+                            unsigned *example_x, unsigned *example_a, unsigned *example_b,
+                            unsigned *example_s) {
   if (int e = ensure_init()) return e;
   if (*pairs < 0) return fail(35, "gcomcot_division_check_: pairs must be >= 0");
   const int blocks = 4096, lanes = blocks * 256;
   const long long per = (*pairs + lanes - 1) / lanes;
   if (per > (1 << 20)) return fail(35, "gcomcot_division_check_: at most 2^20 pairs per lane");
   unsigned long long *d = nullptr;
-  HIPCHK(hipMalloc(&d, 4 * (size_t)lanes * sizeof(unsigned long long)));
+  HIPCHK(hipMalloc(&d, 6 * (size_t)lanes * sizeof(unsigned long long)));
   hipError_t e = launch_division_check(*seed, (int)per, d, blocks, g_stream);
-  std::vector<unsigned long long> h(4 * (size_t)lanes);
+  std::vector<unsigned long long> h(6 * (size_t)lanes);
   if (e == hipSuccess)
     e = hipMemcpyAsync(h.data(), d, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                        g_stream);
   if (e == hipSuccess) e = hipStreamSynchronize(g_stream);
   (void)hipFree(d);
   if (e != hipSuccess) return fail(205, "division check failed: %s", hipGetErrorString(e));
-  long long b12 = 0, bin = 0;
-  *example_x = *example_a = *example_b = 0;
+  long long b12 = 0, bin = 0, bsq = 0;
+  *example_x = *example_a = *example_b = *example_s = 0;
   for (int g = 0; g < lanes; ++g) {
-    const unsigned long long *o = h.data() + 4 * (size_t)g;
-    if (o[0] && !b12) *example_x = (unsigned)o[2];
+    const unsigned long long *o = h.data() + 6 * (size_t)g;
+    if (o[0] && !b12) *example_x = (unsigned)o[3];
     if (o[1] && !bin) {
-      *example_a = (unsigned)(o[3] >> 32);
-      *example_b = (unsigned)o[3];
+      *example_a = (unsigned)(o[4] >> 32);
+      *example_b = (unsigned)o[4];
     }
+    if (o[2] && !bsq) *example_s = (unsigned)o[5];
     b12 += (long long)o[0];
     bin += (long long)o[1];
+    bsq += (long long)o[2];
   }
   *bad_div12 = b12;
   *bad_inrange = bin;
+  *bad_sqrt = bsq;
   *tested = per * (long long)lanes;
   return 0;
 }

@@ -1,5 +1,5 @@
-// Shorter division sequences for the nonlinear momentum terms, equal to
-// IEEE division wherever their results can matter.
+// Shorter division and square root sequences for the nonlinear momentum
+// terms, equal to the IEEE operations wherever their results can matter.
 //
 // The IEEE single-precision division on gfx950 is eleven VALU operations:
 // two operand scalings (V_DIV_SCALE), a reciprocal with one Newton step,
@@ -69,6 +69,25 @@ __device__ __forceinline__ float div_inrange(float a, float b) {
   q = fmaf(r0, y, q);
   const float r1 = fmaf(-b, q, a);
   return fmaf(r1, y, q);
+}
+
+// sqrt(x) as the correctly rounded square root sequence computes it for
+// x >= 2^-96: the hardware estimate, then the neighbour one ulp below or
+// above where the residual says so.  That sequence first scales smaller x
+// by 2^32 (and selects x itself for +-0 and +inf, which the steps below
+// already return).  At the call sites x is g*D (D = 0 or >= 1e-5) or
+// P^2 + Q^2 of the friction term; there a smaller x only feeds a friction
+// numerator below 2^-48 into the quotient argument above.  Equal to IEEE
+// sqrt for every float >= 2^-96 and for +-0 and +inf (device self-check
+// over all floats, gcomcot_division_check_).
+__device__ __forceinline__ float sqrt_inrange(float x) {
+  const float s = __builtin_amdgcn_sqrtf(x);
+  const float sm = __uint_as_float(__float_as_uint(s) - 1u);
+  const float rm = fmaf(-sm, s, x);
+  const float sp = __uint_as_float(__float_as_uint(s) + 1u);
+  const float rp = fmaf(-sp, s, x);
+  const float t = rm <= 0.0f ? sm : s;
+  return rp > 0.0f ? sp : t;
 }
 #endif
 

@@ -184,10 +184,11 @@ hipError_t launch_gauges(const GaugeDev *g, int n, const ZNow &z, const int *ste
 
 // Streaming copy of bytes (a multiple of 16) from src to dst.
 hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st);
-// Self-check of the short divisions (fastdiv.h) against IEEE division:
-// out[4g..4g+3] = lane g's mismatches of div12 (over its slice of all floats)
-// and of div_inrange (over per_lane random in-range pairs), and its first
-// mismatching x and pair (a << 32 | b); 256*blocks lanes.
+// Self-check of the short divisions and square root (fastdiv.h) against
+// the IEEE operations: out[6g..6g+5] = lane g's mismatches of div12 and
+// sqrt_inrange (over its slice of all floats) and of div_inrange (over
+// per_lane random in-range pairs), then its first mismatching x, pair
+// (a << 32 | b) and square-root argument; 256*blocks lanes.
 hipError_t launch_division_check(unsigned long long seed, int per_lane, unsigned long long *out,
                                  int blocks, hipStream_t st);
 

@@ -299,7 +299,7 @@ __device__ __forceinline__ float dq2_at(const DevLayer &L, const A &s, int i, in
 // included, unchanged, so one factor per cell serves both directions).
 template <class A>
 __device__ __forceinline__ float breaking_coef(const DevLayer &L, const A &s, int i, int j) {
-  float c = sqrtf(kGrav * s.DZ(i, j));
+  float c = sqrt_inrange(kGrav * s.DZ(i, j));
   float dzdt_f = 0.08f * c;
   float dzdt_i = 0.65f * c;
   float dzdt = fabsf(s.Z2(i, j) - s.Z1(i, j)) / L.dt;
@@ -340,7 +340,9 @@ __device__ float nl_x(const DevLayer &L, const A &s, int i, int j, float brk) {
   const float p = s.M(i, j);
   const float xqq = 0.25f * (s.N(i, j) + s.N(ip1, j) + s.N(i, jm1) + s.N(ip1, jm1));
   float FF = 0.0f;
-  if (L.fric) FF = div_inrange(L.kfric * sqrtf(p * p + xqq * xqq), gcdet::powf_manning(DF, kManExp, *s.tab));
+  if (L.fric)
+    FF = div_inrange(L.kfric * sqrt_inrange(p * p + xqq * xqq),
+                     gcdet::powf_manning(DF, kManExp, *s.tab));
   float XP;
   if (SPH) {
     const float r2 = s.CF(L, kF2, j);
@@ -429,7 +431,9 @@ __device__ float nl_y(const DevLayer &L, const A &s, int i, int j, float brk) {
   const float q = s.N(i, j);
   const float xpp = 0.25f * (s.M(i, j) + s.M(i, jp1) + s.M(im1, j) + s.M(im1, jp1));
   float FF = 0.0f;
-  if (L.fric) FF = div_inrange(L.kfric * sqrtf(q * q + xpp * xpp), gcdet::powf_manning(DF, kManExp, *s.tab));
+  if (L.fric)
+    FF = div_inrange(L.kfric * sqrt_inrange(q * q + xpp * xpp),
+                     gcdet::powf_manning(DF, kManExp, *s.tab));
   float XQ;
   if (SPH) {
     const float r4 = s.CF(L, kF4, j);
@@ -1194,13 +1198,14 @@ __global__ void __launch_bounds__(256) k_copy(const float4 *__restrict__ src,
     dst[i] = src[i];
 }
 
-// Self-check of the short divisions against IEEE division on the device.
-// Lane g owns a slice of the 2^32 floats for div12 (mismatches counted
-// where |x| >= 2^-124 or x = 0) and `per_lane` pseudo-random pairs (a, b)
-// for div_inrange, drawn with exponents spread over the range in which it
-// claims equality (pairs outside it are skipped).  Lane g writes out[4g..4g+3]:
-// the two counts, the bits of its first mismatching x and of its first
-// mismatching pair (a << 32 | b); the host adds the counts up.
+// Self-check of the short divisions and square root against the IEEE
+// operations on the device.  Lane g owns a slice of the 2^32 floats for
+// div12 (mismatches counted where |x| >= 2^-124 or x = 0) and sqrt_inrange
+// (x = +-0 or x >= 2^-96), and `per_lane` pseudo-random pairs (a, b) for
+// div_inrange, drawn with exponents spread over the range in which it
+// claims equality (pairs outside it are skipped).  Lane g writes
+// out[6g..6g+5]: the three counts and the bits of its first mismatching x,
+// pair (a << 32 | b) and square-root argument; the host adds them up.
 __device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
   z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
   z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
@@ -1214,8 +1219,15 @@ __global__ void __launch_bounds__(256) k_division_check(unsigned long long seed,
   const unsigned long long lanes = (unsigned long long)gridDim.x * blockDim.x;
   unsigned long long bad12 = 0, bad = 0, ex12 = 0, ex = 0;
   const unsigned long long span = ((1ull << 32) + lanes - 1) / lanes;
+  unsigned long long badsq = 0, exsq = 0;
   for (unsigned long long k = g * span; k < (g + 1) * span && k < (1ull << 32); ++k) {
     const float x = f_of((unsigned)k);
+    if (x >= 0x1p-96f || x == 0.0f) {  // +-0, [2^-96, +inf]
+      if (__float_as_uint(sqrtf(x)) != __float_as_uint(sqrt_inrange(x))) {
+        if (!badsq) exsq = k;
+        ++badsq;
+      }
+    }
     if (!(fabsf(x) <= 3.4028235e38f)) continue;  // NaN, infinities
     if (!(fabsf(x) >= 0x1p-124f) && x != 0.0f) continue;
     const float want = x / 12.0f, got = div12(x);
@@ -1241,10 +1253,12 @@ __global__ void __launch_bounds__(256) k_division_check(unsigned long long seed,
       ++bad;
     }
   }
-  out[4 * g] = bad12;
-  out[4 * g + 1] = bad;
-  out[4 * g + 2] = ex12;
-  out[4 * g + 3] = ex;
+  out[6 * g] = bad12;
+  out[6 * g + 1] = bad;
+  out[6 * g + 2] = badsq;
+  out[6 * g + 3] = ex12;
+  out[6 * g + 4] = ex;
+  out[6 * g + 5] = exsq;
 }
 
 // Dry land shows 0, else the elevation plus the tide correction when it is

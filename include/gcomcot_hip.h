@@ -303,18 +303,20 @@ int gcomcot_set_temporal_blocking_(int *mode);
  * counters for a transfer of known size. */
 int gcomcot_hbm_copy_(int *mib, int *reps, float *ms_per_copy);
 
-/* Self-check of the short division sequences of the nonlinear momentum
- * kernels against IEEE division, on the device: x / 12 for every float
- * (*bad_div12 = mismatches where |x| >= 2^-124 or x = 0; *example_x = the
- * bits of one) and a / b for about *pairs pseudo-random operand pairs drawn
- * from seed *seed (*bad_inrange = mismatches among those inside the range
- * the sequence is claimed for: both normal, |b| within 2^+-120,
- * |a| >= 2^-103, exponent gap below 96, normal quotient; *example_a/_b =
- * the bits of one; *tested = pairs drawn).  Both counts are 0 on a correct
- * build. */
+/* Self-check of the short division and square root sequences of the
+ * nonlinear momentum kernels against the IEEE operations, on the device:
+ * x / 12 for every float (*bad_div12 = mismatches where |x| >= 2^-124 or
+ * x = 0; *example_x = the bits of one), sqrt(x) for every x = +-0 or
+ * x >= 2^-96 (*bad_sqrt, *example_s), and a / b for about *pairs
+ * pseudo-random operand pairs drawn from seed *seed (*bad_inrange =
+ * mismatches among those inside the range the sequence is claimed for:
+ * both normal, |b| within 2^+-120, |a| >= 2^-103, exponent gap below 96,
+ * normal quotient; *example_a/_b = the bits of one; *tested = pairs
+ * drawn).  All counts are 0 on a correct build. */
 int gcomcot_division_check_(long long *pairs, unsigned long long *seed, long long *bad_div12,
-                            long long *bad_inrange, long long *tested, unsigned *example_x,
-                            unsigned *example_a, unsigned *example_b);
+                            long long *bad_inrange, long long *bad_sqrt, long long *tested,
+                            unsigned *example_x, unsigned *example_a, unsigned *example_b,
+                            unsigned *example_s);
 
 /* ------------------------------------------------------------------ */
 /* Part 3: scenario batch engine (no reference counterpart)            */
