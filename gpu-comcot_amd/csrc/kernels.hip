@@ -657,17 +657,25 @@ __device__ __forceinline__ void tile_step(const DevLayer &L, int t, int tiles_x,
   else tile_body<MODE, NL, SPH, TX, TY, false>(L, t, ti0, tj0, sm, stab);
 }
 
-// Waves per SIMD the register allocation of the step kernels aims for
-// (at most 72 VGPRs).  The nonlinear tile needs 87 unconstrained, i.e. 5
-// waves; with 7 a few values spill and the step is ~14% faster, since its
-// long dependent chains (divisions, square roots, FP64 powers) want more
-// waves to hide latency.  Register allocation never changes a result.
+// Waves per SIMD the register allocation of the step kernels aims for.
+// The nonlinear spherical tile needs 87 VGPRs unconstrained, i.e. 5 waves;
+// at 7 (72 VGPRs, a few values spilled) it is ~14% faster, since its long
+// dependent chains (divisions, square roots, FP64 powers) want more waves
+// to hide latency; 6 and 8 are slower.  The Cartesian nonlinear tile is
+// fastest at 8 (64 VGPRs).  Register allocation never changes a result.
 #ifndef GC_STEP_WAVES
 #define GC_STEP_WAVES 7
 #endif
+#ifndef GC_STEP_WAVES_CART_NL
+#define GC_STEP_WAVES_CART_NL 8
+#endif
+template <bool NL, bool SPH>
+struct StepWaves {
+  static constexpr int value = (NL && !SPH) ? GC_STEP_WAVES_CART_NL : GC_STEP_WAVES;
+};
 
 template <int MODE, bool NL, bool SPH, int TX, int TY>
-__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(GC_STEP_WAVES)))
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(StepWaves<NL, SPH>::value)))
     k_step(DevLayer L, int tiles_x, int ntiles, int per_xcd) {
   __shared__ float sm[6 * (TX + 4) * (TY + 4) + kNumCoef * (TY + 4)];
   __shared__ gcdet::Tables stab;
@@ -855,7 +863,7 @@ struct Ring {
 };
 
 template <bool NL, bool SPH>
-__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(GC_STEP_WAVES)))
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(StepWaves<NL, SPH>::value)))
     k_stream(DevLayer L, int nstrip, int nseg, int seg, int per_xcd) {
   __shared__ float sm[(6 * 3 + 5 * 2) * kStreamW];
   const int t = tile_of_block(per_xcd);
@@ -1587,7 +1595,7 @@ __global__ void k_fill_i32(int *p, int n, int v) {
 // claim on it.  Every workgroup leaves once all 8 heads have passed their
 // ends.
 template <int MODE, bool NL, bool SPH, int TX, int TY>
-__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(GC_STEP_WAVES)))
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(StepWaves<NL, SPH>::value)))
     k_batch(const BatchDesc *__restrict__ descs, const BatchItem *__restrict__ items,
             const int *__restrict__ qstart, int *heads) {
   __shared__ float sm[6 * (TX + 4) * (TY + 4) + kNumCoef * (TY + 4)];
@@ -1627,7 +1635,7 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(G
 // the balancing): item (b % 8) * per_xcd + b / 8, so each XCD takes one
 // contiguous eighth of the items, as the queues split them.
 template <int MODE, bool NL, bool SPH, int TX, int TY>
-__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(GC_STEP_WAVES)))
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(StepWaves<NL, SPH>::value)))
     k_batch_grid(const BatchDesc *__restrict__ descs, const BatchItem *__restrict__ items,
                  int nitems, int per_xcd) {
   __shared__ float sm[6 * (TX + 4) * (TY + 4) + kNumCoef * (TY + 4)];
