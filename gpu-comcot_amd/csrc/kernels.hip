@@ -669,13 +669,25 @@ __device__ __forceinline__ void tile_step(const DevLayer &L, int t, int tiles_x,
 #ifndef GC_STEP_WAVES_CART_NL
 #define GC_STEP_WAVES_CART_NL 8
 #endif
-template <bool NL, bool SPH>
+// Tiles whose LDS leaves room for fewer workgroups per CU (160 KiB each;
+// one 256-thread workgroup is one wave per SIMD) aim at what fits.
+constexpr int kLdsPerCu = 160 * 1024;
+template <bool NL, bool SPH, int LDS_BYTES>
 struct StepWaves {
-  static constexpr int value = (NL && !SPH) ? GC_STEP_WAVES_CART_NL : GC_STEP_WAVES;
+  static constexpr int want = (NL && !SPH) ? GC_STEP_WAVES_CART_NL : GC_STEP_WAVES;
+  static constexpr int fit = kLdsPerCu / LDS_BYTES;
+  static constexpr int value = want < fit ? want : (fit > 0 ? fit : 1);
 };
+// LDS of the tile kernels (k_step, k_batch, k_batch_grid) and of k_stream
+template <int TX, int TY>
+constexpr int tile_lds_bytes() {
+  return (int)((6 * (TX + 4) * (TY + 4) + kNumCoef * (TY + 4)) * sizeof(float) +
+               sizeof(gcdet::Tables) + 2 * sizeof(int));
+}
 
 template <int MODE, bool NL, bool SPH, int TX, int TY>
-__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(StepWaves<NL, SPH>::value)))
+__global__ void __launch_bounds__(kThreads)
+    __attribute__((amdgpu_waves_per_eu(StepWaves<NL, SPH, tile_lds_bytes<TX, TY>()>::value)))
     k_step(DevLayer L, int tiles_x, int ntiles, int per_xcd) {
   __shared__ float sm[6 * (TX + 4) * (TY + 4) + kNumCoef * (TY + 4)];
   __shared__ gcdet::Tables stab;
@@ -863,7 +875,9 @@ struct Ring {
 };
 
 template <bool NL, bool SPH>
-__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(StepWaves<NL, SPH>::value)))
+__global__ void __launch_bounds__(kThreads)
+    __attribute__((amdgpu_waves_per_eu(
+        StepWaves<NL, SPH, (int)((6 * 3 + 5 * 2) * kStreamW * sizeof(float) + sizeof(gcdet::Tables))>::value)))
     k_stream(DevLayer L, int nstrip, int nseg, int seg, int per_xcd) {
   __shared__ float sm[(6 * 3 + 5 * 2) * kStreamW];
   const int t = tile_of_block(per_xcd);
@@ -1595,7 +1609,8 @@ __global__ void k_fill_i32(int *p, int n, int v) {
 // claim on it.  Every workgroup leaves once all 8 heads have passed their
 // ends.
 template <int MODE, bool NL, bool SPH, int TX, int TY>
-__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(StepWaves<NL, SPH>::value)))
+__global__ void __launch_bounds__(kThreads)
+    __attribute__((amdgpu_waves_per_eu(StepWaves<NL, SPH, tile_lds_bytes<TX, TY>()>::value)))
     k_batch(const BatchDesc *__restrict__ descs, const BatchItem *__restrict__ items,
             const int *__restrict__ qstart, int *heads) {
   __shared__ float sm[6 * (TX + 4) * (TY + 4) + kNumCoef * (TY + 4)];
@@ -1635,7 +1650,8 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(S
 // the balancing): item (b % 8) * per_xcd + b / 8, so each XCD takes one
 // contiguous eighth of the items, as the queues split them.
 template <int MODE, bool NL, bool SPH, int TX, int TY>
-__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(StepWaves<NL, SPH>::value)))
+__global__ void __launch_bounds__(kThreads)
+    __attribute__((amdgpu_waves_per_eu(StepWaves<NL, SPH, tile_lds_bytes<TX, TY>()>::value)))
     k_batch_grid(const BatchDesc *__restrict__ descs, const BatchItem *__restrict__ items,
                  int nitems, int per_xcd) {
   __shared__ float sm[6 * (TX + 4) * (TY + 4) + kNumCoef * (TY + 4)];
