@@ -72,3 +72,48 @@ def test_batch_refuses_an_sc_nest():
     with pytest.raises(GcomcotError, match="SC_OPTION"):
         b.add(sc, 4)
     b.close()
+
+
+def _planes(dev, sc):
+    return {L.id: dev.state(L.id) for L in sc.layers}
+
+
+def _same_planes(a, b, tag, keys=("z1", "m1", "n1")):
+    for lid in a:
+        for k in keys:
+            nb, mx = diff_report(a[lid][k], b[lid][k])
+            assert nb == 0, f"{tag} layer {lid} {k}: {nb} cells differ (max {mx})"
+
+
+def test_reference_call_sequence_on_an_sc_nest_equals_the_device_loop():
+    """MASS_LAUNCH, ALL_GRID_LAUNCH, MOMT_LAUNCH, MAX_AMP and the level copy
+    per root step (comcot.f90:362-439) drive the SC kernels as the loop does."""
+    sc, _ = sc_nest(rel_time=2)
+    n = 12
+    print(f"[gpu] SC nest through the drop-in calls, {n} steps", flush=True)
+    dev = DeviceRun(sc)
+    for _ in range(n):
+        dev.dropin_step()
+    got = _planes(dev, sc)
+    dev.close()
+    dev = DeviceRun(sc)
+    dev.run(n)
+    _same_planes(got, _planes(dev, sc), "drop-in vs loop")
+    dev.close()
+
+
+def test_sc_nest_checkpoint_continues_to_the_same_bits(tmp_path):
+    sc, _ = sc_nest()
+    ck = tmp_path / "sc.ckpt"
+    print("[gpu] SC nest checkpoint after 13 steps, 10 more", flush=True)
+    dev = DeviceRun(sc)
+    dev.run(13)
+    dev.save(ck)
+    dev.run(10)
+    want = _planes(dev, sc)
+    dev.close()
+    dev = DeviceRun(sc)
+    dev.load(ck)
+    dev.run(10)
+    _same_planes(_planes(dev, sc), want, "restored", keys=("z1", "z2", "m1", "m2", "n1", "n2"))
+    dev.close()
