@@ -98,6 +98,16 @@ GC_HD double dfma(double a, double b, double c) {
 #endif
 }
 
+// x * 2^e for a result in the normal range, where it is exact: one
+// instruction on the device (the same value as multiplying by the power).
+GC_HD double scale2(double x, int e) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_ldexp(x, e);
+#else
+  return x * bits_to_double((uint64_t)(uint32_t)(e + 1023) << 52);
+#endif
+}
+
 GC_HD float bits_to_float(uint32_t u) {
   union { uint32_t u; float f; } v; v.u = u; return v.f;
 }
@@ -140,7 +150,7 @@ GC_HD double exp2_core(double t, const TAB &T) {
   q = dfma(g, q, kE2);
   q = dfma(g, q, kE1);
   q = dfma(g, q, 1.0);
-  return (T.exp2j[j] * q) * bits_to_double((uint64_t)(uint32_t)(e2 + 1023) << 52);
+  return scale2(T.exp2j[j] * q, e2);
 }
 
 // x^y for the depths the momentum routines raise to 7/3: x > 0 finite
@@ -176,7 +186,7 @@ GC_HD float powf_manning(float x, float y, const TAB &T) {
   q = dfma(g, q, kE2);
   q = dfma(g, q, kE1);
   q = dfma(g, q, 1.0);
-  const float r = (float)((T.exp2j[j] * q) * bits_to_double((uint64_t)(uint32_t)(e2 + 1023) << 52));
+  const float r = (float)scale2(T.exp2j[j] * q, e2);
   return x != x ? x : r;
 }
 
