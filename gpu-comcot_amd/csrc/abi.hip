@@ -150,8 +150,12 @@ int gcomcot_init_layer_sc_(int *layerid, int *parent, int *level, int *laygov, i
     return e;
   LayerRec &L = g_tree.layers[*layerid];
   std::vector<float> c((size_t)kNumCoef * *ny, 0.0f);
-  if (int e = tree_upload_coef(L, c)) return e;
-  return tree_set_sc(g_tree, g_stream, *layerid, pos, cxy);
+  int e = tree_upload_coef(L, c);
+  if (!e) e = tree_set_sc(g_tree, g_stream, *layerid, pos, cxy);
+  // a layer without its SC geometry must not stay in the tree: its sizes
+  // are not those of a traditional nest on its corners
+  if (e) tree_drop_layer(g_tree, *layerid);
+  return e;
 }
 
 void gcomcot_show_grids_(void) {

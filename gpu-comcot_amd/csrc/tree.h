@@ -87,6 +87,10 @@ int take_last_code();
 // Bytes of device memory a layer of nx x ny cells takes (planes + tables).
 size_t layer_bytes(int nx, int ny, bool nl, bool r24);
 
+// Unlink nested layer id from its parent and free it (no-op for the root
+// or an unused id).
+void tree_drop_layer(Tree &T, int id);
+
 // Register layer `id` of tree T: validates the nest geometry as the
 // reference's SUBGRID_MATCHING does (initialization.f90:882-1131), derives
 // the ALLOC scalars (comcot.f90:537-552, 602), allocates and clears the

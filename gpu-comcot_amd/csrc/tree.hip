@@ -141,6 +141,16 @@ int tree_validate_layer(Tree &T, int id, int parent, int rel_size, int rel_time,
   return 0;
 }
 
+void tree_drop_layer(Tree &T, int id) {
+  if (id < 2 || id >= kMaxLayers || !T.layers[id].used) return;
+  if (LayerRec *op = T.layer(T.layers[id].parent)) {
+    auto &ch = op->children;
+    for (size_t q = 0; q < ch.size(); ++q)
+      if (ch[q] == id) { ch.erase(ch.begin() + q); break; }
+  }
+  free_layer(T.layers[id]);
+}
+
 int tree_create_layer(Tree &T, hipStream_t st, int id, int parent, int level, int laycord,
                       int laygov, int rel_size, int rel_time, const int *corners,
                       int fric_switch, float fric_coef, float dt, float dx, float dy, int nx,
@@ -163,12 +173,7 @@ int tree_create_layer(Tree &T, hipStream_t st, int id, int parent, int level, in
       for (auto &v : T.xs) v.clear();
       for (auto &v : T.ys) v.clear();
     } else {
-      if (LayerRec *op = T.layer(T.layers[id].parent)) {
-        auto &ch = op->children;
-        for (size_t q = 0; q < ch.size(); ++q)
-          if (ch[q] == id) { ch.erase(ch.begin() + q); break; }
-      }
-      free_layer(T.layers[id]);
+      tree_drop_layer(T, id);
     }
   }
   if (id == 1) {

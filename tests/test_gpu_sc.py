@@ -117,3 +117,53 @@ def test_sc_nest_checkpoint_continues_to_the_same_bits(tmp_path):
     dev.run(10)
     _same_planes(_planes(dev, sc), want, "restored", keys=("z1", "z2", "m1", "m2", "n1", "n2"))
     dev.close()
+
+
+def test_sc_layer_errors_are_reported():
+    import ctypes
+    from gcomcot import lib
+    sc, _ = sc_nest()
+    R, C = sc.layers
+    print("[gpu] SC layer errors", flush=True)
+
+    def add(spec, pos=None, corners=None, parent=1):
+        L = lib()
+        pa = np.ascontiguousarray(spec.sc_pos if pos is None else pos, np.int32)
+        ca = np.ascontiguousarray(spec.sc_cxy, np.float32)
+        cr = np.array(spec.corners if corners is None else corners, np.int32)
+        h = np.ascontiguousarray(spec.h, np.float32)
+        z = np.ascontiguousarray(spec.z0, np.float32)
+        I = lambda v: ctypes.byref(ctypes.c_int(v))
+        F = lambda v: ctypes.byref(ctypes.c_float(v))
+        P = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+        return L.gcomcot_init_layer_sc_(I(spec.id), I(parent), I(2), I(spec.laygov),
+                                        I(spec.rel_size), I(spec.rel_time), P(cr),
+                                        I(spec.fric_switch), F(spec.fric_coef), F(spec.dt),
+                                        F(spec.dx), F(spec.dy), P(h), P(z), I(spec.nx),
+                                        I(spec.ny), P(pa), P(ca))
+
+    from gcomcot import scenarios as S
+    from gcomcot._lib import last_error
+
+    def refused(code, **kw):
+        rc = add(C, **kw)
+        got, msg = last_error()          # read (and clear) the recorded error
+        assert rc == code and got == code, (rc, got, msg)
+
+    dev = DeviceRun(S.Scenario("root-only", [R], 10))
+    bad = np.array(C.sc_pos, np.int32).copy()
+    bad[0] = C.corners[0] - 1            # a POS left of the extent
+    refused(82, pos=bad)
+    nx, ny = ctypes.c_int(0), ctypes.c_int(0)   # the refused layer is not left in the tree
+    assert lib().gcomcot_layer_dims_(ctypes.byref(ctypes.c_int(C.id)), ctypes.byref(nx),
+                                     ctypes.byref(ny)) != 0
+    far = list(C.corners)
+    far[1] = R.nx - 2                    # extent reaching the parent's last cells
+    refused(5, corners=far)
+    assert add(C) == 0                   # the good layer still goes in afterwards
+    assert last_error()[0] == 0
+    dev.close()
+    cart = S.custom(laycord=1, laygov=0, nx=96, ny=80)
+    dev = DeviceRun(cart)
+    refused(80)                          # SC needs a spherical layer 1
+    dev.close()
