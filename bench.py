@@ -128,11 +128,11 @@ def traffic_entry(workload_key):
 
 
 def bench_c1(a):
-    from gcomcot import DeviceRun, config1
+    from gcomcot import DeviceRun, nonlinear_layer
     from gcomcot._lib import ip, lib
 
-    log(f"config1 {a.nx}x{a.ny}: generating")
-    sc = config1(nx=a.nx, ny=a.ny, steps=a.steps)
+    log(f"nonlinear layer {a.nx}x{a.ny}: generating")
+    sc = nonlinear_layer(nx=a.nx, ny=a.ny, steps=a.steps)
     cells = sc.cell_updates_per_step()
     run = DeviceRun(sc)
     if a.tile_variant >= 0:
@@ -142,7 +142,7 @@ def bench_c1(a):
         variant = run.autotune()[0]  # state-neutral library autotuner
     run.run(a.warmup, graph=False)
     lib().gcomcot_run_steps_(ip(0), ip(0))  # sync
-    log(f"config1: timing {a.steps} steps (tile variant {variant})")
+    log(f"nonlinear layer: timing {a.steps} steps (tile variant {variant})")
     t0 = time.perf_counter()
     ms_total, each = run.timed(a.steps)
     wall = time.perf_counter() - t0
@@ -159,7 +159,7 @@ def bench_c1(a):
     ty = (ctypes.c_int * 64)()
     n = ctypes.c_int(0)
     lib().gcomcot_tile_variants_(ctypes.byref(n), tx, ty)
-    key = f"config1-{a.nx}x{a.ny}"
+    key = f"nonlinear-{a.nx}x{a.ny}"
     tr = traffic_entry(key)
     traffic = None
     if tr and tr.get("tile") == [tx[variant], ty[variant]]:
@@ -173,10 +173,10 @@ def bench_other_configs(a):
     two-level nest (2048x1536 linear parent + 1801x1801 nonlinear Cartesian
     child, REL_SIZE 4, REL_TIME 4, 3000 steps) through the whole-run device
     loop (HIP graph): GCUPS over all layers' cell updates."""
-    from gcomcot import DeviceRun, config0, config2
+    from gcomcot import DeviceRun, linear_layer, two_level_nest
     out = {}
-    for name, sc, bpc in (("config0", config0(), BYTES_PER_CELL["linear"]),
-                          ("config2", config2(), BYTES_PER_CELL["linear"])):
+    for name, sc, bpc in (("linear_1024x768", linear_layer(), BYTES_PER_CELL["linear"]),
+                          ("nest_2048x1536_1801x1801", two_level_nest(), BYTES_PER_CELL["linear"])):
         log(f"{name}: {sc.name}, {sc.steps} steps")
         run = DeviceRun(sc)
         tuned = run.autotune() if len(sc.layers) == 1 else None
@@ -205,9 +205,9 @@ def bench_outputs(a):
     import shutil
     import tempfile
 
-    from gcomcot import DeviceRun, config1
+    from gcomcot import DeviceRun, nonlinear_layer
     n = a.outputs_steps
-    sc = config1(nx=a.nx, ny=a.ny, steps=n)
+    sc = nonlinear_layer(nx=a.nx, ny=a.ny, steps=n)
     dt = np.float32(sc.layers[0].dt)
     out = {"steps": n, "iprt": a.outputs_iprt, "fluxswitch": 0, "out_opt": 2}
     walls = {}
@@ -243,10 +243,10 @@ def bench_source(a):
     call; the first allocates the deformation planes), against the
     reference's own DEFORM_OKADA (flang, REAL, one core) on a 512x384 sample
     of the same kind of layer."""
-    from gcomcot import DeviceRun, config1
+    from gcomcot import DeviceRun, nonlinear_layer
     from gcomcot import scenarios as S
     from gcomcot.source import Fault
-    sc = config1(nx=a.nx, ny=a.ny, steps=1)
+    sc = nonlinear_layer(nx=a.nx, ny=a.ny, steps=1)
     L = sc.layers[0]
     x, y = S.layer_coords(L)
     f = Fault(10000, 200000, 80000, 6, 30, 15, 90, x0=float(x[L.nx // 2]), y0=float(y[L.ny // 2]))
@@ -287,7 +287,7 @@ def bench_batch(a):
     from gcomcot.batch import Batch
 
     log(f"batch: generating {a.batch_cases} scenarios")
-    cs = S.config3(n=a.batch_cases)
+    cs = S.scenario_mix(n=a.batch_cases)
     upd = [sum(L.nx * L.ny * (L.rel_time if L.id > 1 else 1) for L in c.layers) * c.steps
            for c in cs]
     total = float(sum(upd))
@@ -345,11 +345,11 @@ def bench_serve(a):
     nbig = max(1, round(0.1 * nreq))
     log(f"serve: generating {nbig} large nonlinear cases")
     rng = np.random.default_rng(20261202 + 4000)
-    bigs = [S.config1(case=10 + i, steps=int(rng.integers(1000, 4001))) for i in range(nbig)]
+    bigs = [S.nonlinear_layer(case=10 + i, steps=int(rng.integers(1000, 4001))) for i in range(nbig)]
     out = {}
     for ri, rate in enumerate(rates):
         log(f"serve: {nreq} requests at {rate:g} req/s")
-        small = S.config3(n=nreq - nbig, case=10 + ri)
+        small = S.scenario_mix(n=nreq - nbig, case=10 + ri)
         pool = small + bigs
         order = np.random.default_rng(ri).permutation(len(pool))
         t = poisson_arrivals(rate, len(pool), seed=100 + ri)
@@ -371,7 +371,7 @@ def cpu_accuracy():
     from gcomcot import DeviceRun
     from gcomcot import scenarios as S
     from oracle import bind
-    sc = S.config1(nx=256, ny=192, steps=400)
+    sc = S.nonlinear_layer(nx=256, ny=192, steps=400)
     o64 = bind.Oracle(sc, 64)
     o64.step(sc.steps)
     r64 = o64.state(1)
@@ -395,19 +395,19 @@ def cpu_accuracy():
 def main():
     a = parse()
     if a.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (got --gpus {a.gpus}); scenario-level "
-              "sharding across GPUs is described in DESIGN.md", file=sys.stderr)
+        print(f"bench.py: one GPU only (got --gpus {a.gpus}); scenarios would shard across GPUs "
+              "whole, one process per GPU, but that is not built", file=sys.stderr)
         sys.exit(2)
     if a.copy_probe:
         print(json.dumps({"copy_probe": copy_probe(a.copy_probe)}))
         return
-    from gcomcot import config1
+    from gcomcot import nonlinear_layer
 
     sc, cells, gcups, wall, kern_ms, achieved, traffic, arrived, ms_total, tile = bench_c1(a)
     cpu = None
     if a.cpu_steps > 0:
         log("cpu baseline")
-        cpu = cpu_baseline(config1(nx=a.nx, ny=a.ny, steps=a.cpu_steps), a.cpu_steps)
+        cpu = cpu_baseline(nonlinear_layer(nx=a.nx, ny=a.ny, steps=a.cpu_steps), a.cpu_steps)
         if a.accuracy:
             cpu["accuracy"] = cpu_accuracy()
     others = bench_other_configs(a) if a.others else None

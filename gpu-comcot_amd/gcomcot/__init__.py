@@ -6,8 +6,8 @@ tests and the benchmark: scenario generation, layer registration, the
 step loop and state access.
 """
 from ._lib import GcomcotError, LIB_PATH, lib
-from .scenarios import LayerSpec, Scenario, config0, config1, config2
+from .scenarios import LayerSpec, Scenario, linear_layer, nonlinear_layer, two_level_nest
 from .engine import DeviceRun
 
-__all__ = ["GcomcotError", "LIB_PATH", "lib", "LayerSpec", "Scenario", "config0", "config1",
-           "config2", "DeviceRun"]
+__all__ = ["GcomcotError", "LIB_PATH", "lib", "LayerSpec", "Scenario", "linear_layer",
+           "nonlinear_layer", "two_level_nest", "DeviceRun"]

@@ -6,13 +6,13 @@ order, which is the Fortran (NX, NY) column-major layout of the reference
 (I fastest).  Depths and sources come from the library's generator
 (csrc/synth.cpp), a pure function of (seed, point).
 
-Configurations (seed = 20261202 + 1000*config + case):
-  config0  spherical linear, 1024x768 at 2', 8 seamounts + shelf ramp,
+Workloads (seed = 20261202 + 1000*k + case, k = 0, 1, 2 in this order):
+  linear_layer     spherical linear, 1024x768 at 2', 8 seamounts + shelf ramp,
            elliptical uplift 1 m (60x25 km), CFL 0.5, 2000 steps
-  config1  spherical nonlinear, 4096x3072 at 0.5', + 3 embayments with
+  nonlinear_layer  spherical nonlinear, 4096x3072 at 0.5', + 3 embayments with
            1:50 beaches, dipole +-2 m (sigma 40 km, 80 km apart), Manning
            0.013, CFL 0.35, 5000 steps
-  config2  spherical linear parent 2048x1536 at 2' with a Cartesian
+  two_level_nest   spherical linear parent 2048x1536 at 2' with a Cartesian
            nonlinear child, REL_SIZE 4, REL_TIME 4, footprint 450x450
            parent cells over the shelf ramp, source 300 km offshore of the
            child, 3000 parent steps
@@ -197,8 +197,8 @@ def _sph_root(world, nx, ny, dx, laygov, src, fric_switch, fric_coef, lon0, lat0
                      y=y, dely=dely, fric_switch=fric_switch, fric_coef=fric_coef)
 
 
-def config0(nx=1024, ny=768, dx=2.0, case=0, steps=2000, laygov=0):
-    """Spherical linear single layer."""
+def linear_layer(nx=1024, ny=768, dx=2.0, case=0, steps=2000, laygov=0):
+    """Spherical linear single layer, 1024x768 at 2 arc-min by default."""
     lon0, lat0 = 140.0, 10.0
     lon1 = lon0 + (nx - 1) * dx / 60.0
     lat1 = lat0 + (ny - 1) * dx / 60.0
@@ -206,11 +206,12 @@ def config0(nx=1024, ny=768, dx=2.0, case=0, steps=2000, laygov=0):
     w = make_world(seed, lon0, lat0, lon1, lat1, 8, 0)
     src = source_draw(w, 0, 0, 1.0, 60.0, 25.0)
     L = _sph_root(w, nx, ny, dx, laygov, src, 1, 0.0, lon0, lat0)
-    return Scenario(f"config0-{nx}x{ny}", [L], steps, {"seed": seed})
+    return Scenario(f"linear-{nx}x{ny}", [L], steps, {"seed": seed})
 
 
-def config1(nx=4096, ny=3072, dx=0.5, case=0, steps=5000, embayments=3):
-    """Spherical nonlinear single layer with Manning friction."""
+def nonlinear_layer(nx=4096, ny=3072, dx=0.5, case=0, steps=5000, embayments=3):
+    """Spherical nonlinear single layer with Manning friction, 4096x3072 at 0.5 arc-min by
+    default."""
     lon0, lat0 = 140.0, 10.0
     lon1 = lon0 + (nx - 1) * dx / 60.0
     lat1 = lat0 + (ny - 1) * dx / 60.0
@@ -218,12 +219,13 @@ def config1(nx=4096, ny=3072, dx=0.5, case=0, steps=5000, embayments=3):
     w = make_world(seed, lon0, lat0, lon1, lat1, 8, embayments)
     src = source(w, 1, 2.0, 40.0, 40.0, 80.0, 0.30, 0.5, 0.35)
     L = _sph_root(w, nx, ny, dx, 1, src, 0, 0.013, lon0, lat0)
-    return Scenario(f"config1-{nx}x{ny}", [L], steps, {"seed": seed})
+    return Scenario(f"nonlinear-{nx}x{ny}", [L], steps, {"seed": seed})
 
 
-def config2(pnx=2048, pny=1536, dx=2.0, footprint=450, rel_size=4, rel_time=4, case=0,
+def two_level_nest(pnx=2048, pny=1536, dx=2.0, footprint=450, rel_size=4, rel_time=4, case=0,
             steps=3000, child_laygov=1, corner_i=30):
-    """Spherical linear parent + Cartesian nonlinear child."""
+    """Spherical linear parent + Cartesian nonlinear child (2048x1536 + 1801x1801 by
+    default)."""
     lon0, lat0 = 140.0, 10.0
     lon1 = lon0 + (pnx - 1) * dx / 60.0
     lat1 = lat0 + (pny - 1) * dx / 60.0
@@ -257,7 +259,7 @@ def config2(pnx=2048, pny=1536, dx=2.0, footprint=450, rel_size=4, rel_time=4, c
                   dx=cdx, dy=cdy, dt=cdt, h=ch, z0=cz, fric_switch=0 if child_laygov else 1,
                   fric_coef=0.013 if child_laygov else 0.0, rel_size=rel_size,
                   rel_time=rel_time, corners=(IS, IE, JS, JE))
-    return Scenario(f"config2-{pnx}x{pny}+{cnx}x{cny}", [P, C], steps, {"seed": seed})
+    return Scenario(f"nest-{pnx}x{pny}+{cnx}x{cny}", [P, C], steps, {"seed": seed})
 
 
 def layer_coords(s):
@@ -312,9 +314,9 @@ def _cart_child(world, src, parent, dx, lon0, lat0, IS, JS, footprint_i, footpri
                      rel_time=rel_time, corners=(IS, IE, JS, JE))
 
 
-def config3(n=64, case=0, min_cells=256 * 256, max_cells=2048 * 1536, nest_share=0.25,
+def scenario_mix(n=64, case=0, min_cells=256 * 256, max_cells=2048 * 1536, nest_share=0.25,
             steps=(1000, 4000), child_side=(400, 1200)):
-    """Batch of independent scenarios.
+    """Mix of independent scenarios for the batch engine and the serving loop.
 
     Parent cell counts log-uniform in [min_cells, max_cells], aspect 4:3,
     spacing 1-4 arc minutes, spherical linear; a share `nest_share` of the
@@ -355,7 +357,7 @@ def config3(n=64, case=0, min_cells=256 * 256, max_cells=2048 * 1536, nest_share
             JS = max(4, (ny - fp_) // 2 + 1)
             layers.append(_cart_child(w, src, P, dx, lon0, lat0, IS, JS, fp_, fp_, R))
         st = int(rng.integers(steps[0], steps[1] + 1))
-        name = f"config3-{k}-{nx}x{ny}" + (f"+{layers[1].nx}x{layers[1].ny}" if nested else "")
+        name = f"mix-{k}-{nx}x{ny}" + (f"+{layers[1].nx}x{layers[1].ny}" if nested else "")
         out.append(Scenario(name, layers, st, {"seed": seed}))
     return out
 

@@ -137,17 +137,17 @@ def test_det_exp_matches_libm_on_edges(oracle_built):
 
 
 def test_generator_is_deterministic():
-    a = S.config0(nx=64, ny=48)
-    b = S.config0(nx=64, ny=48)
+    a = S.linear_layer(nx=64, ny=48)
+    b = S.linear_layer(nx=64, ny=48)
     L1, L2 = a.layers[0], b.layers[0]
     assert np.array_equal(bits(L1.h), bits(L2.h)) and np.array_equal(bits(L1.z0), bits(L2.z0))
     assert L1.dt == L2.dt
-    c = S.config0(nx=64, ny=48, case=1)
+    c = S.linear_layer(nx=64, ny=48, case=1)
     assert not np.array_equal(L1.z0, c.layers[0].z0)
 
 
 def test_generator_world_shape():
-    sc = S.config1(nx=512, ny=384, dx=4.0)
+    sc = S.nonlinear_layer(nx=512, ny=384, dx=4.0)
     L = sc.layers[0]
     h = L.h
     assert (h[:, 0] < 0).all(), "west edge is dry land"

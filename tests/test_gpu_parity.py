@@ -90,25 +90,25 @@ def test_reference_entry_points_match_oracle():
     dev.close()
 
 
-def test_config0_reduced_matches_oracle():
-    sc = S.config0(nx=256, ny=192)
-    print("[gpu] config0 256x192 x 400 steps", flush=True)
+def test_linear_layer_reduced_matches_oracle():
+    sc = S.linear_layer(nx=256, ny=192)
+    print("[gpu] linear layer 256x192 x 400 steps", flush=True)
     orc = Oracle(sc, 32, pow_mode=1)
     orc.step(400)
     dev = DeviceRun(sc)
     dev.run(400)
-    _compare(sc, dev, orc, "config0-256")
+    _compare(sc, dev, orc, "linear-256")
     dev.close()
 
 
-def test_config1_reduced_matches_oracle():
-    sc = S.config1(nx=256, ny=192, dx=2.0)
-    print("[gpu] config1 256x192 x 300 steps", flush=True)
+def test_nonlinear_layer_reduced_matches_oracle():
+    sc = S.nonlinear_layer(nx=256, ny=192, dx=2.0)
+    print("[gpu] nonlinear layer 256x192 x 300 steps", flush=True)
     orc = Oracle(sc, 32, pow_mode=1)
     orc.step(300)
     dev = DeviceRun(sc)
     dev.run(300)
-    _compare(sc, dev, orc, "config1-256")
+    _compare(sc, dev, orc, "nonlinear-256")
     dev.close()
 
 
