@@ -1010,7 +1010,7 @@ struct State {
     }
   }
 
-  // NEWQ_SC (all_grids.f90:1095-1373), the top edge with its C1*YN1 (App. Q3)
+  // NEWQ_SC (all_grids.f90:1095-1373); its top edge takes C1*YN2 + C1*YN1 (:1357)
   void newq_sc(Layer<T> &P, Layer<T> &C, int t) {
     const T GX = T(kGx);
     const int nx = C.nx, ny = C.ny;
