@@ -29,6 +29,14 @@ def _case(name):
         sc = S.custom(nx=72, ny=64, laygov=0, child=child, **BEACH)
         return sc, dict(steps=70, iprt=9, out_opt=2, tend_steps=68, tide=0.5,
                         flux={1: 0, 2: 0}, chunks=[(23, True), (20, False), (27, True)])
+    if name == "sc-nest":
+        from helpers import sc_nest
+        from oracle.bind import Reference
+        if not Reference.available():
+            pytest.skip("the SC nest is laid out by the flang-built reference")
+        sc, _ = sc_nest()
+        return sc, dict(steps=40, iprt=7, out_opt=2, tend_steps=38, tide=0.0,
+                        flux={1: 0, 2: 0}, chunks=[(13, True), (27, False)])
     if name == "nonlinear":
         sc = S.custom(laycord=0, laygov=1, nx=72, ny=64, **BEACH)
         return sc, dict(steps=70, iprt=10, out_opt=2, tend_steps=200, tide=0.0,
@@ -65,7 +73,7 @@ def _expected(sc, c):
     return files, writes, tend, orc
 
 
-@pytest.mark.parametrize("name", ["nest", "nonlinear", "linear-two-step"])
+@pytest.mark.parametrize("name", ["nest", "sc-nest", "nonlinear", "linear-two-step"])
 def test_output_files_match_the_oracle(tmp_path, name):
     sc, c = _case(name)
     print(f"[gpu] output files: {name}, {c['steps']} steps", flush=True)
