@@ -56,7 +56,7 @@ struct Scenario {
   bool finished() const { return done >= total; }
 };
 
-enum GroupType { kStepGroup = 0, kJnqGroup = 1, kJnzGroup = 2 };
+enum GroupType { kStepGroup = 0, kJnqGroup = 1, kJnzGroup = 2, kScEdgeGroup = 3, kScJnzGroup = 4 };
 
 struct Group {
   int type = kStepGroup;
@@ -65,6 +65,7 @@ struct Group {
   std::vector<BatchItem> items;
   int qstart[9] = {0};
   std::vector<DevNest> nests;
+  std::vector<DevSc> scs;  // SC_OPTION = 1 children: one launch pair each
   std::vector<int> start;  // nnest + 1
   size_t o_descs = 0, o_items = 0, o_qstart = 0, o_nests = 0, o_start = 0;
   int head = 0;            // index of this launch's 8 heads
@@ -238,6 +239,10 @@ void emit_step(const std::vector<Scenario *> &act, std::vector<Group> &out) {
       jq.t = k;
       Group jz;
       jz.type = kJnzGroup;
+      Group se, sz;  // the same phases of SC_OPTION = 1 children
+      se.type = kScEdgeGroup;
+      se.t = k;
+      sz.type = kScJnzGroup;
       std::vector<LayerRec *> moved;
       for (Scenario *s : act) {
         Tree &T = s->tree;
@@ -245,11 +250,13 @@ void emit_step(const std::vector<Scenario *> &act, std::vector<Group> &out) {
         if (R.children.size() <= sib) continue;
         LayerRec &C = T.layers[R.children[sib]];
         if (k > C.rel_time) continue;
-        jq.nests.push_back(nest_view(R, C));
+        if (C.sc) se.scs.push_back(sc_view(T, R, C));
+        else jq.nests.push_back(nest_view(R, C));
         pend_add(pend, kFused, T, C, dev_view(T, C, true, false));
         moved.push_back(&C);
       }
-      out.push_back(std::move(jq));
+      if (!jq.nests.empty()) out.push_back(std::move(jq));
+      if (!se.scs.empty()) out.push_back(std::move(se));
       flush_steps(pend, out);
       for (Scenario *s : act) {
         Tree &T = s->tree;
@@ -258,9 +265,12 @@ void emit_step(const std::vector<Scenario *> &act, std::vector<Group> &out) {
         LayerRec &C = T.layers[R.children[sib]];
         if (k > C.rel_time) continue;
         const int nhalf = (int)std::floor(C.rel_time / 2.0f) + 1;
-        if (k == nhalf) jz.nests.push_back(nest_view(R, C));
+        if (k != nhalf) continue;
+        if (C.sc) sz.scs.push_back(sc_view(T, R, C));
+        else jz.nests.push_back(nest_view(R, C));
       }
       if (!jz.nests.empty()) out.push_back(std::move(jz));
+      if (!sz.scs.empty()) out.push_back(std::move(sz));
       for (LayerRec *C : moved) C->cur = 1 - C->cur;  // UPDATE
     }
   }
@@ -290,6 +300,8 @@ int upload_plan(Plan &P, bool engine_arena = false) {
         g.o_items = bytes; bytes += al(g.items.size() * sizeof(BatchItem));
         g.o_qstart = bytes; bytes += al(sizeof(g.qstart));
         g.head = heads; heads += 8;
+      } else if (g.type == kScEdgeGroup || g.type == kScJnzGroup) {
+        // the views go to the kernels by value
       } else {
         g.start.assign(1, 0);
         for (auto &n : g.nests)
@@ -323,6 +335,8 @@ int upload_plan(Plan &P, bool engine_arena = false) {
         std::memcpy(&host[g.o_descs], g.descs.data(), g.descs.size() * sizeof(BatchDesc));
         std::memcpy(&host[g.o_items], g.items.data(), g.items.size() * sizeof(BatchItem));
         std::memcpy(&host[g.o_qstart], g.qstart, sizeof(g.qstart));
+      } else if (g.type == kScEdgeGroup || g.type == kScJnzGroup) {
+        continue;
       } else {
         std::memcpy(&host[g.o_nests], g.nests.data(), g.nests.size() * sizeof(DevNest));
         std::memcpy(&host[g.o_start], g.start.data(), g.start.size() * sizeof(int));
@@ -351,6 +365,10 @@ int enqueue_plan_step(const Plan &P, int r) {
   for (const auto &g : P.step[r]) {
     if (g.type == kStepGroup) {
       e = launch_batch(step_launch(P, g), g.mode, g.nl != 0, g.sph != 0, g.shape, st);
+    } else if (g.type == kScEdgeGroup || g.type == kScJnzGroup) {
+      for (size_t q = 0; q < g.scs.size() && e == hipSuccess; ++q)
+        e = g.type == kScEdgeGroup ? launch_sc_edges(g.scs[q], g.t, st)
+                                   : launch_sc_jnz(g.scs[q], st);
     } else {
       const DevNest *n = reinterpret_cast<const DevNest *>(P.arena + g.o_nests);
       const int *s = reinterpret_cast<const int *>(P.arena + g.o_start);
@@ -571,6 +589,34 @@ int gcomcot_batch_add_layer_(int *sid, int *layerid, int *parent, int *laycord, 
   std::vector<float> c((size_t)kNumCoef * *ny, 0.0f);
   if (*laycord == 0) std::memcpy(c.data(), coef, c.size() * sizeof(float));
   return tree_upload_coef(L, c);
+}
+
+int gcomcot_batch_add_layer_sc_(int *sid, int *layerid, int *parent, int *laygov,
+                                int *rel_size, int *rel_time, int *corners, int *fric_switch,
+                                float *fric_coef, float *dt, float *dx, float *dy, float *H,
+                                float *Z, int *nx, int *ny, int *pos, float *cxy) {
+  Scenario *s = scenario(*sid);
+  if (!s) return fail(31, "batch: no scenario %d", *sid);
+  if (!(*laygov == 0 || *laygov == 1 || *laygov == 5 || *laygov == 9))
+    return fail(11, "LAYGOV %d not supported", *laygov);
+  if (*fric_switch != 0 && *fric_switch != 1)
+    return fail(12, "FRIC_SWITCH %d not supported", *fric_switch);
+  if (*dt <= 0.0f || *dx <= 0.0f || *dy <= 0.0f) return fail(13, "DT/DX/DY must be positive");
+  if (!pos || !cxy) return fail(84, "SC: layer %d needs POS and CXY", *layerid);
+  if (int e = tree_validate_layer(s->tree, *layerid, *parent, *rel_size, *rel_time, corners, *nx,
+                                  *ny, true))
+    return e;
+  if (int e = eng_init()) return e;
+  invalidate();
+  if (int e = tree_create_layer(s->tree, g_eng.st, *layerid, *parent, 2, 1, *laygov, *rel_size,
+                                *rel_time, corners, *fric_switch, *fric_coef, *dt, *dx, *dy, *nx,
+                                *ny, H, Z, true))
+    return e;
+  std::vector<float> c((size_t)kNumCoef * *ny, 0.0f);
+  int e = tree_upload_coef(s->tree.layers[*layerid], c);
+  if (!e) e = tree_set_sc(s->tree, g_eng.st, *layerid, pos, cxy);
+  if (e) tree_drop_layer(s->tree, *layerid);  // no half-registered SC layer
+  return e;
 }
 
 int gcomcot_batch_run_(long long *nsteps, int *nfinished) {

@@ -112,6 +112,8 @@ void tree_free(Tree &T);
 
 DevLayer dev_view(const Tree &T, const LayerRec &L, bool with_max, bool advance);
 DevNest nest_view(const LayerRec &P, const LayerRec &C);
+// The SC_OPTION = 1 coupling views of parent P and child C of tree T.
+DevSc sc_view(const Tree &T, const LayerRec &P, const LayerRec &C);
 
 // One layer pass (kFused / kMassOnly / kMomentOnly) on stream st.
 int tree_launch(const Tree &T, const LayerRec &L, int mode, bool with_max, bool advance,

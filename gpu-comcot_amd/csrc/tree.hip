@@ -344,7 +344,7 @@ int tree_launch(const Tree &T, const LayerRec &L, int mode, bool with_max, bool 
 
 // ALL_GRID (all_grids.f90:34-308) / subgrid_solver_rcsv (GPUall_grids.cu:655-699)
 // The SC_OPTION = 1 coupling views of parent P and child C.
-static DevSc sc_view(const Tree &T, const LayerRec &P, const LayerRec &C) {
+DevSc sc_view(const Tree &T, const LayerRec &P, const LayerRec &C) {
   DevSc S{};
   S.pnx = P.nx;
   S.pny = P.ny;

@@ -62,9 +62,6 @@ class Batch:
 
     def add(self, sc, steps=None):
         """Register scenario `sc`; returns its id.  steps defaults to sc.steps."""
-        if any(getattr(s, "sc_pos", None) is not None for s in sc.layers):
-            raise GcomcotError("batched scenarios take SC_OPTION = 0 nests only; "
-                               "run an SC_OPTION = 1 nest with DeviceRun")
         L = lib()
         sid = ctypes.c_int(0)
         check(L.gcomcot_batch_new_scenario_(_ll(sc.steps if steps is None else steps),
@@ -75,6 +72,17 @@ class Batch:
             ha, hp = arr(s.h)
             za, zp = arr(s.z0)
             keep += [corners, ha, za]
+            if getattr(s, "sc_pos", None) is not None:  # SC_OPTION = 1 child
+                pa = np.ascontiguousarray(s.sc_pos, np.int32)
+                ca = np.ascontiguousarray(s.sc_cxy, np.float32)
+                keep += [pa, ca]
+                rc = L.gcomcot_batch_add_layer_sc_(
+                    ctypes.byref(sid), ip(s.id), ip(s.parent), ip(s.laygov), ip(s.rel_size),
+                    ip(s.rel_time), corners.ctypes.data_as(ctypes.c_void_p), ip(s.fric_switch),
+                    fp(s.fric_coef), fp(s.dt), fp(s.dx), fp(s.dy), hp, zp, ip(s.nx), ip(s.ny),
+                    pa.ctypes.data_as(ctypes.c_void_p), ca.ctypes.data_as(ctypes.c_void_p))
+                check(rc, f"batch add SC layer {s.id} of {sc.name}")
+                continue
             if s.laycord == 0:
                 coef = scenarios.para_rows(s.laygov, s.dt, s.dx, s.y, s.dely)
                 keep.append(coef)

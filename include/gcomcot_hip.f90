@@ -317,6 +317,15 @@ module gcomcot_hip
       real(c_float) :: fric_coef, dt, dx, dy, coef(*), h(*), z(*)
     end function
 
+    integer(c_int) function gcomcot_batch_add_layer_sc(sid, id, parent, laygov, rel_size, &
+        rel_time, corners, fric_switch, fric_coef, dt, dx, dy, h, z, nx, ny, pos, cxy) &
+        bind(C, name="gcomcot_batch_add_layer_sc_")
+      import :: c_int, c_float
+      integer(c_int) :: sid, id, parent, laygov, rel_size, rel_time, corners(4)
+      integer(c_int) :: fric_switch, nx, ny, pos(*)
+      real(c_float) :: fric_coef, dt, dx, dy, h(*), z(*), cxy(*)
+    end function
+
     integer(c_int) function gcomcot_batch_run(nsteps, nfinished) &
         bind(C, name="gcomcot_batch_run_")
       import :: c_int, c_long_long

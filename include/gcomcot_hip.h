@@ -341,6 +341,12 @@ int gcomcot_batch_add_layer_(int *sid, int *layerid, int *parent, int *laycord, 
                              int *rel_size, int *rel_time, int *corners, int *fric_switch,
                              float *fric_coef, float *dt, float *dx, float *dy, float *coef,
                              float *H, float *Z, int *nx, int *ny);
+/* a Cartesian child of scenario *sid's spherical layer 1 with SC_OPTION = 1;
+ * arguments as gcomcot_init_layer_sc_ */
+int gcomcot_batch_add_layer_sc_(int *sid, int *layerid, int *parent, int *laygov,
+                                int *rel_size, int *rel_time, int *corners, int *fric_switch,
+                                float *fric_coef, float *dt, float *dx, float *dy, float *H,
+                                float *Z, int *nx, int *ny, int *pos, float *cxy);
 /* advance every unfinished scenario by up to *nsteps root steps (each
  * stops at its own count); *nfinished = scenarios finished and not yet
  * removed.  Returns after the device work is complete. */

@@ -7,14 +7,14 @@ blended from them) and JNZ_SC (parent elevation averaged over the wet child
 cells of each parent cell); the oracle, pinned against the reference's own
 NEWQ_SC and JNZ_SC in tests/test_sc_oracle.py, must give the same bits for
 every plane of both layers, the maxima and the arrival field.  A fault
-spread to the child goes through CELL_INTERP_SC.  Batched scenarios
-refuse SC_OPTION = 1 nests.
+spread to the child goes through CELL_INTERP_SC.  A batched SC nest
+matches its run alone.
 """
 import numpy as np
 import pytest
 
 from gcomcot import DeviceRun
-from helpers import diff_report, sc_nest
+from helpers import BEACH, diff_report, sc_nest
 from oracle import okada as OK
 from oracle.bind import Oracle, Reference
 
@@ -64,13 +64,29 @@ def test_fault_spreads_to_an_sc_child():
     assert np.abs(d_child).max() > 0.01
 
 
-def test_batch_refuses_an_sc_nest():
-    from gcomcot._lib import GcomcotError
+def test_batched_sc_nest_equals_its_single_run():
+    """The batch engine's SC edge and JNZ groups, beside a traditional nest and
+    a single layer, give the SC nest the bits of its run alone."""
+    from gcomcot import scenarios as S
     from gcomcot.batch import Batch
     sc, _ = sc_nest()
+    n = 24
+    print(f"[gpu] batched SC nest, {n} steps", flush=True)
+    dev = DeviceRun(sc)
+    dev.run(n)
+    alone = {L.id: dev.state(L.id) for L in sc.layers}
+    dev.close()
     b = Batch()
-    with pytest.raises(GcomcotError, match="SC_OPTION"):
-        b.add(sc, 4)
+    child = dict(laycord=0, laygov=1, footprint=(20, 24), rel_size=3, rel_time=3, corner=(30, 26))
+    b.add(S.custom(laycord=0, laygov=1, nx=96, ny=80, dx=2.0, child=child, **BEACH), n)
+    sid = b.add(sc, n)
+    b.add(S.custom(laycord=1, laygov=1, nx=64, ny=56, **BEACH), n + 5)
+    b.run(n + 5)
+    for L in sc.layers:
+        got = b.state(sid, L.id)
+        for k in ("z1", "m1", "n1"):
+            nb, mx = diff_report(got[k], alone[L.id][k])
+            assert nb == 0, f"layer {L.id} {k}: {nb} cells differ (max {mx})"
     b.close()
 
 
