@@ -8,9 +8,11 @@
 //   fused step of every single-layer root      } persistent step kernel,
 //   continuity half of every nested root       } one launch per layer kind
 //   for each child slot s, sub-step k = 1..max REL_TIME:
-//     JNQ/NEWQ of every child s with k <= its REL_TIME   (one launch)
+//     JNQ/NEWQ of every child s with k <= its REL_TIME   (one launch; an
+//       SC_OPTION = 1 child takes its band and edge launches instead)
 //     fused step of those children                       (one per kind)
-//     JNZ of the children at their half step             (one launch)
+//     JNZ of the children at their half step             (one launch; JNZ_SC
+//       per SC_OPTION = 1 child)
 //   momentum half of every nested root (+ max / arrival)
 //
 // which is the per-scenario order of the drop-in path (all_grids.f90:34-
