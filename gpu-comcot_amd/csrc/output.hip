@@ -421,10 +421,16 @@ int out_write(Tree &T, long long k, hipStream_t st) {
 int out_flush(long long *nfiles) {
   Output &O = g_out;
   for (int s = 0; s < kSlots; ++s) wait_free(s);
-  std::lock_guard<std::mutex> lk(O.mu);
-  if (nfiles) *nfiles = O.nfiles;
-  if (O.err) return fail(52, "output: %s", O.errmsg.c_str());
-  return 0;
+  std::string why;
+  {
+    std::lock_guard<std::mutex> lk(O.mu);
+    if (nfiles) *nfiles = O.nfiles;
+    if (!O.err) return 0;
+    why = O.errmsg;  // reported once: the next flush reports later failures only
+    O.err = 0;
+    O.errmsg.clear();
+  }
+  return fail(52, "output: %s", why.c_str());
 }
 
 }  // namespace gc

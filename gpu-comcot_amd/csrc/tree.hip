@@ -658,6 +658,10 @@ int tree_checkpoint_load(Tree &T, hipStream_t st, const char *path, long long *d
       return fail(73, "checkpoint: %s is truncated", path);
     recs.push_back(c);
   }
+  // seeking past the end succeeds, so compare the file's size with the records'
+  const long need = std::ftell(F.f);
+  if (need < 0 || std::fseek(F.f, 0, SEEK_END) != 0 || std::ftell(F.f) < need)
+    return fail(73, "checkpoint: %s is truncated", path);
   if (std::fseek(F.f, start, SEEK_SET) != 0) return fail(73, "checkpoint: cannot reread %s", path);
   HIPCHK(hipStreamSynchronize(st));
   std::vector<char> buf;

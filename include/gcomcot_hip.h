@@ -273,8 +273,9 @@ int gcomcot_get_depth_(int *lid, float *h);
  * others: none; default 1).  Files are written by a thread of the library
  * from pinned host copies while the loop continues; gcomcot_output_flush_
  * waits until every file is on disk and reports how many were written
- * (cuda_shutdown_ and a new layer 1 flush too).  Each returns 0 or an
- * error code. */
+ * (cuda_shutdown_ and a new layer 1 flush too).  A file that could not be
+ * written does not stop the loop: the next flush returns its error, once.
+ * Each returns 0 or an error code. */
 int gcomcot_output_set_(const char *dir, int *dirlen, int *iprt, int *out_opt, float *tend,
                         float *tide);
 int gcomcot_output_layer_(int *lid, int *fluxswitch);

@@ -82,6 +82,24 @@ def test_checkpoint_of_other_layers_is_refused(tmp_path):
     dev.close()
 
 
+def test_truncated_checkpoint_is_refused_before_anything_changes(tmp_path):
+    sc = _nest()
+    ck = tmp_path / "run.ckpt"
+    print("[gpu] truncated checkpoint", flush=True)
+    dev = DeviceRun(sc)
+    dev.run(5)
+    dev.save(ck)
+    body = ck.read_bytes()
+    ck.write_bytes(body[:len(body) - 1000])   # the last layer record cut short
+    dev.run(4)
+    before = _snapshot(dev, sc)
+    with pytest.raises(GcomcotError, match="truncated"):
+        dev.load(ck)
+    _same(_snapshot(dev, sc), before, "after a refused load")
+    dev.run(3)                                # and the loop goes on
+    dev.close()
+
+
 def test_batched_scenario_preempted_and_resumed(tmp_path):
     sc = _nest()
     side = S.custom(laycord=1, laygov=1, nx=64, ny=56, **BEACH)

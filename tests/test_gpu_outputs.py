@@ -102,6 +102,24 @@ def test_output_files_match_the_oracle(tmp_path, name):
         assert nb == 0, f"recorder maximum: {nb} cells differ (max {mx})"
 
 
+def test_unwritable_directory_is_reported_at_flush(tmp_path):
+    """A write that fails does not stop the loop; the flush reports it."""
+    from gcomcot._lib import GcomcotError
+    sc, c = _case("nonlinear")
+    print("[gpu] outputs into a missing directory", flush=True)
+    dev = DeviceRun(sc)
+    dev.set_output(tmp_path / "no" / "such" / "dir", 10, 1, 1000.0, 0.0, {1: 1})
+    dev.run(25)
+    z = dev.state(1)["z1"]
+    with pytest.raises(GcomcotError, match="cannot open"):
+        dev.flush_output()
+    dev.close()
+    ref = DeviceRun(sc)                       # the step results are those of a run without files
+    ref.run(25)
+    np.testing.assert_array_equal(ref.state(1)["z1"].view(np.int32), z.view(np.int32))
+    ref.close()
+
+
 def test_outputs_off_leave_the_loop_as_it_was(tmp_path):
     """Without gcomcot_output_set_ nothing is written and the recorder
     maximum stays at the initial elevation (only maxamp_launch_ raises it)."""
