@@ -445,7 +445,14 @@ int gcomcot_checkpoint_load_(const char *path, int *pathlen) {
   if (sync()) return 1;
   const std::string p(path, (size_t)*pathlen);
   long long done = 0;
-  return tree_checkpoint_load(g_tree, g_stream, p.c_str(), &done);
+  if (int e = tree_checkpoint_load(g_tree, g_stream, p.c_str(), &done)) return e;
+  // scheduled ruptures whose call (GET_FLOOR_DEFORM's TIME <= T0 < TIME + DT)
+  // lies at or before the restored step are in the restored state already
+  const float next = loop_time(done) + g_tree.layers[1].dt;
+  for (size_t i = 0; i < g_faults.size();)
+    if (g_faults[i].t0 < next) g_faults.erase(g_faults.begin() + (long)i);
+    else ++i;
+  return 0;
 }
 
 int gcomcot_sponge_set_(int *on) {

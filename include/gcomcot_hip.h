@@ -200,8 +200,11 @@ int gcomcot_init_layer_sc_(int *layerid, int *parent, int *level, int *laygov, i
  * parity and the root-step count to the file path[0..*pathlen);
  * gcomcot_checkpoint_load_ restores it into a tree built with the same
  * layers (same ids, parents, sizes, kinds and corners; refused otherwise
- * before anything changes).  Continuing after a load gives the same bits as
- * continuing without it.  The batch forms do the same for one scenario and
+ * before anything changes, also when the file is cut short).  Continuing
+ * after a load gives the same bits as continuing without it.  Scheduled
+ * ruptures are not part of a checkpoint: a load drops those whose call lies
+ * at or before the restored step (their deformation is in the state) and
+ * keeps later ones.  The batch forms do the same for one scenario and
  * restore its step count, so a served request can be preempted and resumed. */
 int gcomcot_checkpoint_save_(const char *path, int *pathlen);
 int gcomcot_checkpoint_load_(const char *path, int *pathlen);

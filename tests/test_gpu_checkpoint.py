@@ -131,3 +131,39 @@ def test_batched_scenario_preempted_and_resumed(tmp_path):
             assert nb == 0, f"layer {L.id} {k}: {nb} cells differ (max {mx})"
     assert b.progress(other) == (n, n)
     b.close()
+
+
+def test_restore_keeps_only_the_ruptures_still_to_come(tmp_path):
+    """Scheduled faults are not part of a checkpoint: after a load, those whose
+    GET_FLOOR_DEFORM call lies at or before the restored step are dropped
+    (their deformation is in the state), later ones still fire."""
+    from oracle import okada as OK
+    sc = _nest()
+    ck = tmp_path / "run.ckpt"
+    x, y = S.layer_coords(sc.layers[0])
+    dt = np.float32(sc.layers[0].dt)
+    early = OK.Fault(10000, 100000, 50000, 5, 30, 15, 90, x0=float(x[45]), y0=float(y[38]))
+    late = OK.Fault(5000, 60000, 30000, 3, 200, 45, 30, x0=float(x[45]), y0=float(y[38]),
+                    switch=9)
+    times = (float(np.float32(2.5) * dt), float(np.float32(17.5) * dt))
+    print("[gpu] checkpoint with scheduled ruptures", flush=True)
+    dev = DeviceRun(sc)
+    dev.set_coords()
+    for f, t in zip((early, late), times):
+        dev.schedule_fault(f.params(), f.switch, t)
+    dev.run(9)
+    dev.save(ck)
+    dev.run(16)
+    assert dev.pending_faults() == 0
+    want = _snapshot(dev, sc)
+    dev.close()
+    dev = DeviceRun(sc)
+    dev.set_coords()
+    for f, t in zip((early, late), times):
+        dev.schedule_fault(f.params(), f.switch, t)
+    assert dev.pending_faults() == 2
+    dev.load(ck)
+    assert dev.pending_faults() == 1
+    dev.run(16)
+    _same(_snapshot(dev, sc), want, "restored with a rupture to come")
+    dev.close()
