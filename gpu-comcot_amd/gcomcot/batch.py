@@ -66,6 +66,16 @@ class Batch:
         sid = ctypes.c_int(0)
         check(L.gcomcot_batch_new_scenario_(_ll(sc.steps if steps is None else steps),
                                             ctypes.byref(sid)), "batch new scenario")
+        try:
+            self._add_layers(sid, sc)
+        except GcomcotError:
+            lib().gcomcot_batch_remove_(ctypes.byref(sid))  # no half-registered scenario
+            raise
+        self.specs[sid.value] = sc
+        return sid.value
+
+    def _add_layers(self, sid, sc):
+        L = lib()
         keep = []
         for s in sc.layers:
             corners = np.array(s.corners, dtype=np.int32)
@@ -95,8 +105,6 @@ class Batch:
                 ip(s.fric_switch), fp(s.fric_coef), fp(s.dt), fp(s.dx), fp(s.dy), cptr, hp, zp,
                 ip(s.nx), ip(s.ny))
             check(rc, f"batch add layer {s.id} of {sc.name}")
-        self.specs[sid.value] = sc
-        return sid.value
 
     def save(self, sid, path):
         """Exact checkpoint of scenario sid (its planes and step count)."""

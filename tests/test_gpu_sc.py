@@ -183,3 +183,22 @@ def test_sc_layer_errors_are_reported():
     dev = DeviceRun(cart)
     refused(80)                          # SC needs a spherical layer 1
     dev.close()
+
+
+def test_batch_refuses_bad_sc_geometry_and_stays_usable():
+    import copy
+    from gcomcot._lib import GcomcotError
+    from gcomcot.batch import Batch
+    sc, _ = sc_nest()
+    bad = copy.deepcopy(sc)
+    pos = np.array(bad.layers[1].sc_pos, np.int32).copy()
+    pos[0] = bad.layers[1].corners[0] - 1      # a POS left of the extent
+    bad.layers[1].sc_pos = pos
+    print("[gpu] batch with a refused SC layer", flush=True)
+    b = Batch()
+    with pytest.raises(GcomcotError, match="outside its extent"):
+        b.add(bad, 4)
+    sid = b.add(sc, 4)
+    b.run(4)
+    assert b.progress(sid) == (4, 4)
+    b.close()
