@@ -98,6 +98,22 @@ GC_HD double dfma(double a, double b, double c) {
 #endif
 }
 
+// fma(a, b, c) for a constant addend c.  On the device it is one VOP3
+// v_fma_f64 that reads c from a scalar register pair; left to itself the
+// compiler writes the two-address v_fmac_f64 instead, which needs c copied
+// into the accumulator first (an extra vector move per polynomial step) and
+// keeps every coefficient in vector registers for the whole kernel.  The
+// operation, and so the result, is the same IEEE double fma either way.
+GC_HD double dfma_k(double a, double b, double c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  double r;
+  __asm__("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "s"(c));
+  return r;
+#else
+  return __builtin_fma(a, b, c);
+#endif
+}
+
 // x * 2^e for a result in the normal range, where it is exact: one
 // instruction on the device (the same value as multiplying by the power).
 GC_HD double scale2(double x, int e) {
@@ -124,10 +140,10 @@ GC_HD double log2_pos(float x, const TAB &T) {
   const double m = (double)bits_to_float((b & 0x007fffffu) | 0x3f800000u);
   const double r = dfma(m, T.invc[k], -1.0);
   double p = dfma(r, kL6, kL5);
-  p = dfma(r, p, kL4);
-  p = dfma(r, p, kL3);
-  p = dfma(r, p, kL2);
-  p = dfma(r, p, kL1);
+  p = dfma_k(r, p, kL4);
+  p = dfma_k(r, p, kL3);
+  p = dfma_k(r, p, kL2);
+  p = dfma_k(r, p, kL1);
   return (double)e + (T.logc[k] + r * p);
 }
 
@@ -145,10 +161,10 @@ GC_HD double exp2_core(double t, const TAB &T) {
   const int j = k & 31;
   const int e2 = k >> 5;  // floor(k / 32)
   double q = dfma(g, kE6, kE5);
-  q = dfma(g, q, kE4);
-  q = dfma(g, q, kE3);
-  q = dfma(g, q, kE2);
-  q = dfma(g, q, kE1);
+  q = dfma_k(g, q, kE4);
+  q = dfma_k(g, q, kE3);
+  q = dfma_k(g, q, kE2);
+  q = dfma_k(g, q, kE1);
   q = dfma(g, q, 1.0);
   return scale2(T.exp2j[j] * q, e2);
 }
@@ -181,10 +197,10 @@ GC_HD float powf_manning(float x, float y, const TAB &T) {
   const int j = k & 31;
   const int e2 = k >> 5;
   double q = dfma(g, kE6, kE5);
-  q = dfma(g, q, kE4);
-  q = dfma(g, q, kE3);
-  q = dfma(g, q, kE2);
-  q = dfma(g, q, kE1);
+  q = dfma_k(g, q, kE4);
+  q = dfma_k(g, q, kE3);
+  q = dfma_k(g, q, kE2);
+  q = dfma_k(g, q, kE1);
   q = dfma(g, q, 1.0);
   const float r = (float)scale2(T.exp2j[j] * q, e2);
   return x != x ? x : r;

@@ -658,23 +658,24 @@ __device__ __forceinline__ void tile_step(const DevLayer &L, int t, int tiles_x,
 }
 
 // Waves per SIMD the register allocation of the step kernels aims for.
-// The nonlinear spherical tile needs 87 VGPRs unconstrained, i.e. 5 waves;
-// at 7 (72 VGPRs, a few values spilled) it is ~14% faster, since its long
-// dependent chains (divisions, square roots, FP64 powers) want more waves
-// to hide latency; 6 and 8 are slower.  The Cartesian nonlinear tile is
-// fastest at 8 (64 VGPRs).  Register allocation never changes a result.
+// The nonlinear tiles have long dependent chains (divisions, square roots,
+// FP64 powers) that want many waves to hide latency: the spherical one needs
+// 87 VGPRs unconstrained (5 waves) and is ~20% faster at 8 (64 VGPRs, a few
+// values spilled); 7 is 3% and 6 is 11% slower than 8.  The Cartesian one is
+// also fastest at 8.  Linear tiles aim at 7.  Register allocation never
+// changes a result.
 #ifndef GC_STEP_WAVES
 #define GC_STEP_WAVES 7
 #endif
-#ifndef GC_STEP_WAVES_CART_NL
-#define GC_STEP_WAVES_CART_NL 8
+#ifndef GC_STEP_WAVES_NL
+#define GC_STEP_WAVES_NL 8
 #endif
 // Tiles whose LDS leaves room for fewer workgroups per CU (160 KiB each;
 // one 256-thread workgroup is one wave per SIMD) aim at what fits.
 constexpr int kLdsPerCu = 160 * 1024;
 template <bool NL, bool SPH, int LDS_BYTES>
 struct StepWaves {
-  static constexpr int want = (NL && !SPH) ? GC_STEP_WAVES_CART_NL : GC_STEP_WAVES;
+  static constexpr int want = NL ? GC_STEP_WAVES_NL : GC_STEP_WAVES;
   static constexpr int fit = kLdsPerCu / LDS_BYTES;
   static constexpr int value = want < fit ? want : (fit > 0 ? fit : 1);
 };
