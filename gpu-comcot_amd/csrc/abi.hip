@@ -730,6 +730,26 @@ int gcomcot_division_check_(long long *pairs, unsigned long long *seed, long lon
   return 0;
 }
 
+int gcomcot_detmath_eval_(long long *n, const float *x, float *pow_out, float *exp_out) {
+  if (*n < 0 || (*n > 0 && (!x || !pow_out || !exp_out)))
+    return fail(36, "gcomcot_detmath_eval_: n must be >= 0 and the arrays given");
+  if (*n == 0) return 0;
+  if (int e = ensure_init()) return e;
+  const size_t bytes = (size_t)*n * sizeof(float);
+  float *d = nullptr;
+  HIPCHK(hipMalloc(&d, 3 * bytes));
+  hipError_t e = hipMemcpyAsync(d, x, bytes, hipMemcpyHostToDevice, g_stream);
+  if (e == hipSuccess) e = launch_detmath_eval(d, *n, d + *n, d + 2 * *n, g_stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(pow_out, d + *n, bytes, hipMemcpyDeviceToHost, g_stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(exp_out, d + 2 * *n, bytes, hipMemcpyDeviceToHost, g_stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(g_stream);
+  (void)hipFree(d);
+  if (e != hipSuccess) return fail(207, "detmath eval failed: %s", hipGetErrorString(e));
+  return 0;
+}
+
 int gcomcot_set_tile_variant_(int *variant) {
   if (*variant < 0 || *variant >= num_tile_variants()) return 1;
   g_tree.variant = *variant;

@@ -189,6 +189,9 @@ hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st)
 // sqrt_inrange (over its slice of all floats) and of div_inrange (over
 // per_lane random in-range pairs), then its first mismatching x, pair
 // (a << 32 | b) and square-root argument; 256*blocks lanes.
+// powf_manning(x[q], 2.3333333f) -> pw[q] and expf_det(x[q]) -> ex[q], q < n.
+hipError_t launch_detmath_eval(const float *x, long long n, float *pw, float *ex,
+                               hipStream_t st);
 hipError_t launch_division_check(unsigned long long seed, int per_lane, unsigned long long *out,
                                  int blocks, hipStream_t st);
 

@@ -1875,6 +1875,25 @@ hipError_t launch_jnz_multi(const DevNest *nests, const int *start, int nnest, i
   return hipGetLastError();
 }
 
+// The step's Manning power x^2.3333333 and breaking exponential e^x, as the
+// momentum kernels evaluate them, for a list of arguments.
+__global__ void __launch_bounds__(256)
+    k_detmath_eval(const float *x, long long n, float *pw, float *ex) {
+  for (long long q = blockIdx.x * 256LL + threadIdx.x; q < n; q += (long long)gridDim.x * 256) {
+    const float v = x[q];
+    pw[q] = gcdet::powf_manning(v, kManExp, gcdet::kTab);
+    ex[q] = gcdet::expf_det(v, gcdet::kTab);
+  }
+}
+
+hipError_t launch_detmath_eval(const float *x, long long n, float *pw, float *ex,
+                               hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  const long long blocks = std::min<long long>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_detmath_eval, dim3((unsigned)blocks), dim3(256), 0, st, x, n, pw, ex);
+  return hipGetLastError();
+}
+
 hipError_t launch_division_check(unsigned long long seed, int per_lane, unsigned long long *out,
                                  int blocks, hipStream_t st) {
   hipLaunchKernelGGL(k_division_check, dim3(blocks), dim3(256), 0, st, seed, per_lane, out);

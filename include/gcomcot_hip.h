@@ -322,6 +322,13 @@ int gcomcot_division_check_(long long *pairs, unsigned long long *seed, long lon
                             unsigned *example_x, unsigned *example_a, unsigned *example_b,
                             unsigned *example_s);
 
+/* The Manning depth power x^2.3333333 and the breaking exponential e^x as
+ * the nonlinear momentum kernels compute them (moment.f90:459-460 and
+ * :1323 call libm's powf and expf there), evaluated on the device for the
+ * n arguments x[] into pow_out[] and exp_out[] (host arrays).  A check of
+ * the device arithmetic against the same functions compiled for the host. */
+int gcomcot_detmath_eval_(long long *n, const float *x, float *pow_out, float *exp_out);
+
 /* ------------------------------------------------------------------ */
 /* Part 3: scenario batch engine (no reference counterpart)            */
 /* ------------------------------------------------------------------ */

@@ -1377,3 +1377,9 @@ double orc_cr_dt(int laycord, int laygov, int nx, int ny, const double *h, doubl
 extern "C" float orc_powf_det(float x, float y) { return gcdet::powf_det(x, y); }
 extern "C" float orc_expf_det(float x) { return gcdet::expf_det(x); }
 extern "C" float orc_powf_manning(float x, float y) { return gcdet::powf_manning(x, y); }
+extern "C" void orc_detmath_eval(long long n, const float *x, float y, float *pw, float *ex) {
+  for (long long q = 0; q < n; ++q) {
+    pw[q] = gcdet::powf_manning(x[q], y);
+    ex[q] = gcdet::expf_det(x[q]);
+  }
+}
