@@ -104,3 +104,14 @@ def test_bench_refuses_more_than_one_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_detmath_eval_refuses_bad_arguments():
+    n = ctypes.c_longlong(-1)
+    assert lib().gcomcot_detmath_eval_(ctypes.byref(n), None, None, None) == 36
+    assert last_error()[0] == 36
+    n = ctypes.c_longlong(4)                       # a count without arrays
+    assert lib().gcomcot_detmath_eval_(ctypes.byref(n), None, None, None) == 36
+    assert last_error()[0] == 36
+    n = ctypes.c_longlong(0)                       # nothing to do: no device needed
+    assert lib().gcomcot_detmath_eval_(ctypes.byref(n), None, None, None) == 0

@@ -77,9 +77,3 @@ def test_device_pow_and_exp_equal_the_host_build():
     k = slice(depths.size + edges.size, depths.size + edges.size + 4096)
     assert np.abs(dex[k] / np.exp(r) - 1.0).max() < 1.2e-7
 
-
-def test_detmath_eval_refuses_a_negative_count():
-    from gcomcot._lib import last_error
-    n = ctypes.c_longlong(-1)
-    assert lib().gcomcot_detmath_eval_(ctypes.byref(n), None, None, None) == 36
-    assert last_error()[0] == 36
