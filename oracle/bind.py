@@ -24,6 +24,13 @@ REFERENCE_DIR = "/root/reference"
 
 OP = dict(MASS=1, OPEN=2, MOMENT=3, JNQ=4, NEWQ=5, JNZ=6, UPDATE=7, ALL_GRID=8, CHANGE=9, MAX=10)
 
+# Branches of CONMOME(_S) / WAVE_BREAKING counted by the oracle (enum Branch
+# of oracle.cpp, same order)
+BRANCHES = ("noflux", "dd_wet", "dd_edge_a", "dd_edge_b", "df_clamp", "fric_on", "fric_off",
+            "adv", "up_neg_deg", "up_neg_full", "up_pos_deg", "up_pos_full", "cross_neg_dz",
+            "cross_neg_dp", "cross_neg_full", "cross_pos_dz", "cross_pos_dp", "cross_pos_full",
+            "brk_none", "brk_finite", "brk_over", "brk_dry", "vlim_hi", "vlim_lo", "eps_zero")
+
 
 def build(ref=True):
     """Build liboracle.so, and the reference library when its sources exist."""
@@ -97,6 +104,8 @@ class Oracle:
             L.orc_set_arrival_threshold.argtypes = [ctypes.c_void_p, ctypes.c_double]
             L.orc_set_sponge.argtypes = [ctypes.c_void_p] * 3
             L.orc_set_sc.argtypes = [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 3
+            L.orc_branch_counts.restype = ctypes.c_int
+            L.orc_branch_counts.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
             L.orc_cr_dt.restype = ctypes.c_double
             L.orc_cr_dt.argtypes = [ctypes.c_int] * 4 + [ctypes.c_void_p] + [ctypes.c_double] * 5
             cls._lib = L
@@ -130,6 +139,16 @@ class Oracle:
 
     def step(self, n=1):
         self.lib().orc_step(self.h, n)
+
+    def branch_counts(self):
+        """Flux points per CONMOME(_S)/WAVE_BREAKING branch since the last
+        call, {name: (x_count, y_count)}; the counters are cleared."""
+        n = len(BRANCHES)
+        out = np.zeros(2 * n, np.int64)
+        got = self.lib().orc_branch_counts(self.h, _p(out))
+        if got != n:
+            raise RuntimeError(f"oracle has {got} branch counters, bind.py names {n}")
+        return {k: (int(out[i]), int(out[n + i])) for i, k in enumerate(BRANCHES)}
 
     def set_sc(self, lid, pos, cxy, corners):
         """Couple child lid with SC_OPTION = 1: POS (nx*ny*2) and CXY (nx*ny*4)

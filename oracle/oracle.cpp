@@ -80,11 +80,45 @@ struct Layer {
   T &R(std::vector<T> &r, int i, int j) { return r[ix(i, j)]; }
 };
 
+// Branch counters of CONMOME(_S) and WAVE_BREAKING, per direction (0 = X
+// flux, 1 = Y flux): how many flux points took each case of the reference's
+// IF chains.  Instrumentation for the tests, which assert that a case really
+// reaches the branches it is meant to cover; no arithmetic depends on them.
+enum Branch {
+  kBrNoFlux = 0,    // H or HP/HQ <= ELMAX, no wet/dry case, or DD < GX
+  kBrDdWet,         // both cells wet: DD = DP(,,2), DF = DP(,,1)
+  kBrDdEdgeA,       // wet cell, dry neighbour flooded from it
+  kBrDdEdgeB,       // dry cell flooded from the wet neighbour
+  kBrDfClamp,       // DF < 1E-3 raised to 1E-3
+  kBrFricOn,        // Manning term computed
+  kBrFricOff,       // FRIC_SWITCH = 1
+  kBrAdv,           // convection terms computed (LIN_CHK, DP1 >= GX, SPAN)
+  kBrUpNegDeg,      // along-flux upwind, flux < 0, one-sided
+  kBrUpNegFull,     // flux < 0, two-sided
+  kBrUpPosDeg,      // flux >= 0, one-sided
+  kBrUpPosFull,     // flux >= 0, two-sided
+  kBrCrNegDz,       // cross term, mean flux < 0, dry total depth
+  kBrCrNegDp,       // mean flux < 0, dry flux depth
+  kBrCrNegFull,     // mean flux < 0, two-sided
+  kBrCrPosDz,       // mean flux >= 0, dry total depth
+  kBrCrPosDp,       // mean flux >= 0, dry flux depth
+  kBrCrPosFull,     // mean flux >= 0, two-sided
+  kBrBrkNone,       // WAVE_BREAKING: DZDT <= DZDT_F
+  kBrBrkFinite,     // DZDT > DZDT_F at a wet cell: COEF = EXP(-REF), 0 < REF
+  kBrBrkOver,       // ... of those, DZDT > DZDT_I (REF > 1)
+  kBrBrkDry,        // DZDT > DZDT_F = 0 at a cell with DZ = 0 (COEF = 0)
+  kBrVlimHi,        // flux clamped to +V_LIMIT*DD
+  kBrVlimLo,        // flux clamped to -V_LIMIT*DD
+  kBrEpsZero,       // |flux| < EPS set to 0 (after the convection terms)
+  kNumBranch
+};
+
 template <class T>
 struct State {
   int pow_mode = 0;
   T arrival_thr = T(0.01f);
   int nstep = 0;
+  long long br[2][kNumBranch] = {};
   std::vector<Layer<T>> layers;
   int find(int id) {
     for (size_t k = 0; k < layers.size(); ++k)
@@ -423,18 +457,26 @@ struct State {
   }
 
   // WAVE_BREAKING (moment.f90:1272-1384): damping of the convection terms
-  void wave_breaking(Layer<T> &L, T &advx, T &advy, int i, int j) {
+  void wave_breaking(Layer<T> &L, T &advx, T &advy, int i, int j, int dir) {
     const T G = T(kGrav);
     T c = m_sqrt(G * L.DZ(i, j, 2));
     T dzdt_f = T(0.08f) * c;
     T dzdt_i = T(0.65f) * c;
     T dzdt = m_abs(L.Z(i, j, 2) - L.Z(i, j, 1)) / L.dt;
     if (dzdt > dzdt_f) {
+      if (c > T(0)) {
+        ++br[dir][kBrBrkFinite];
+        if (dzdt > dzdt_i) ++br[dir][kBrBrkOver];
+      } else {
+        ++br[dir][kBrBrkDry];
+      }
       T ref = (dzdt - dzdt_f) / (dzdt_i - dzdt_f);
       T a = T(1);
       T coef = ex(-(a * ref));
       advx = coef * advx;
       advy = coef * advy;
+    } else {
+      ++br[dir][kBrBrkNone];
     }
   }
 
@@ -505,23 +547,30 @@ struct State {
             if (dz2(i, j) > GX && dz2(ip1, j) > GX) {
               DD = DP[nxy + L.ix(i, j)];
               DF = dp1(i, j);
+              ++br[0][kBrDdWet];
             } else if (dz2(i, j) > GX && dz2(ip1, j) <= GX && L.H(ip1, j) + z2(i, j) > GX) {
               DD = sph ? L.HP(i, j) + T(0.5f) * (z2(i, j) + z2(ip1, j)) : T(0.5f) * dz2(i, j);
               DF = DD;
+              ++br[0][kBrDdEdgeA];
             } else if (dz2(i, j) <= GX && dz2(ip1, j) > GX && L.H(i, j) + z2(ip1, j) > GX) {
               DD = sph ? L.HP(i, j) + T(0.5f) * (z2(i, j) + z2(ip1, j)) : T(0.5f) * dz2(ip1, j);
               DF = DD;
+              ++br[0][kBrDdEdgeB];
             } else {
               noflux = 1;
             }
             if (DD < GX) noflux = 1;
           }
-          if (noflux == 1) { L.M(i, j, 2) = T(0); continue; }
-          if (DF < T(1.0e-3f)) DF = T(1.0e-3f);
+          if (noflux == 1) { L.M(i, j, 2) = T(0); ++br[0][kBrNoFlux]; continue; }
+          if (DF < T(1.0e-3f)) { DF = T(1.0e-3f); ++br[0][kBrDfClamp]; }
           T xqq = T(0.25f) * (q1(i, j) + q1(ip1, j) + q1(i, jm1) + q1(ip1, jm1));
-          if (IFRIC != 1)
+          if (IFRIC != 1) {
             FF = CONST * FM * FM * m_sqrt(p1(i, j) * p1(i, j) + xqq * xqq) / pw(DF, T(2.3333333f));
-          else FF = T(0);
+            ++br[0][kBrFricOn];
+          } else {
+            FF = T(0);
+            ++br[0][kBrFricOff];
+          }
           if (sph) {
             if (MOD_SCM == 0 && j < JY) {
               T r2 = L.R(L.r2, i, j);
@@ -549,44 +598,57 @@ struct State {
             T fx = sph ? L.R(L.r21, i, j) : RX;
             T fy = sph ? L.R(L.r0, i, j) : RY;
             T advx = T(0), advy = T(0);
+            ++br[0][kBrAdv];
             if (p1(i, j) < T(0)) {
-              if (dp1(ip1, j) < GX || dz2(ip1, j) < GX)
+              if (dp1(ip1, j) < GX || dz2(ip1, j) < GX) {
                 advx = fx * (-(p1(i, j) * p1(i, j) / dp1(i, j)));
-              else
+                ++br[0][kBrUpNegDeg];
+              } else {
                 advx = fx * (p1(ip1, j) * p1(ip1, j) / dp1(ip1, j) - p1(i, j) * p1(i, j) / dp1(i, j));
+                ++br[0][kBrUpNegFull];
+              }
             } else {
-              if (dp1(im1, j) < GX || dz2(i, j) < GX)
+              if (dp1(im1, j) < GX || dz2(i, j) < GX) {
                 advx = fx * (p1(i, j) * p1(i, j) / dp1(i, j));
-              else
+                ++br[0][kBrUpPosDeg];
+              } else {
                 advx = fx * (p1(i, j) * p1(i, j) / dp1(i, j) - p1(im1, j) * p1(im1, j) / dp1(im1, j));
+                ++br[0][kBrUpPosFull];
+              }
             }
             if (xqq < T(0)) {
               if (dz2(i, jp1) < GX || dz2(ip1, jp1) < GX) {
                 advy = fy * (-(p1(i, j) * xqq / dp1(i, j)));
+                ++br[0][kBrCrNegDz];
               } else if (dp1(i, jp1) < GX) {
                 advy = fy * (-(p1(i, j) * xqq / dp1(i, j)));
+                ++br[0][kBrCrNegDp];
               } else {
                 T xqe = T(0.25f) * (q1(i, jp1) + q1(ip1, jp1) + q1(i, j) + q1(ip1, j));
                 advy = fy * (p1(i, jp1) * xqe / dp1(i, jp1) - p1(i, j) * xqq / dp1(i, j));
+                ++br[0][kBrCrNegFull];
               }
             } else {
               if (dz2(i, jm1) < GX || dz2(ip1, jm1) < GX) {
                 advy = fy * (p1(i, j) * xqq / dp1(i, j));
+                ++br[0][kBrCrPosDz];
               } else if (dp1(i, jm1) < GX) {
                 advy = fy * (p1(i, j) * xqq / dp1(i, j));
+                ++br[0][kBrCrPosDp];
               } else {
                 T xqe = T(0.25f) * (q1(i, jm1) + q1(ip1, jm1) + q1(i, j - 2) + q1(ip1, j - 2));
                 advy = fy * (p1(i, j) * xqq / dp1(i, j) - p1(i, jm1) * xqe / dp1(i, jm1));
+                ++br[0][kBrCrPosFull];
               }
             }
-            wave_breaking(L, advx, advy, i, j);
+            wave_breaking(L, advx, advy, i, j, 0);
             XP = XP - advx - advy;
           }
           if (IFRIC != 1) XP = XP / (T(1) + FF);
-          if (m_abs(XP) < EPS) XP = T(0);
+          if (m_abs(XP) < EPS) { XP = T(0); ++br[0][kBrEpsZero]; }
           T lim = VL * DD;
-          if (XP > lim) XP = lim;
-          if (XP < -lim) XP = -lim;
+          if (XP > lim) { XP = lim; ++br[0][kBrVlimHi]; }
+          if (XP < -lim) { XP = -lim; ++br[0][kBrVlimLo]; }
           L.M(i, j, 2) = XP;
         }
       }
@@ -610,23 +672,30 @@ struct State {
             if (dz2(i, j) > GX && dz2(i, jp1) > GX) {
               DD = DQ[nxy + L.ix(i, j)];
               DF = dq1(i, j);
+              ++br[1][kBrDdWet];
             } else if (dz2(i, j) > GX && dz2(i, jp1) <= GX && L.H(i, jp1) + z2(i, j) > GX) {
               DD = sph ? L.HQ(i, j) + T(0.5f) * (z2(i, j) + z2(i, jp1)) : T(0.5f) * dz2(i, j);
               DF = DD;
+              ++br[1][kBrDdEdgeA];
             } else if (dz2(i, j) <= GX && dz2(i, jp1) > GX && L.H(i, j) + z2(i, jp1) > GX) {
               DD = sph ? L.HQ(i, j) + T(0.5f) * (z2(i, j) + z2(i, jp1)) : T(0.5f) * dz2(i, jp1);
               DF = DD;
+              ++br[1][kBrDdEdgeB];
             } else {
               noflux = 1;
             }
             if (DD < GX) noflux = 1;
           }
-          if (noflux == 1) { L.N(i, j, 2) = T(0); continue; }
-          if (DF < T(1.0e-3f)) DF = T(1.0e-3f);
+          if (noflux == 1) { L.N(i, j, 2) = T(0); ++br[1][kBrNoFlux]; continue; }
+          if (DF < T(1.0e-3f)) { DF = T(1.0e-3f); ++br[1][kBrDfClamp]; }
           T xpp = T(0.25f) * (p1(i, j) + p1(i, jp1) + p1(im1, j) + p1(im1, jp1));
-          if (IFRIC != 1)
+          if (IFRIC != 1) {
             FF = CONST * FM * FM * m_sqrt(q1(i, j) * q1(i, j) + xpp * xpp) / pw(DF, T(2.3333333f));
-          else FF = T(0);
+            ++br[1][kBrFricOn];
+          } else {
+            FF = T(0);
+            ++br[1][kBrFricOff];
+          }
           if (sph) {
             if (MOD_SCM == 0 && i < IX) {
               T r4 = L.R(L.r4, i, j);
@@ -655,44 +724,57 @@ struct State {
             T fy = sph ? L.R(L.r0, i, j) : RY;
             T fx = sph ? L.R(L.r22, i, j) : RX;
             T advx = T(0), advy = T(0);
+            ++br[1][kBrAdv];
             if (q1(i, j) < T(0)) {
-              if (dq1(i, jp1) < GX || dz2(i, jp1) < GX)
+              if (dq1(i, jp1) < GX || dz2(i, jp1) < GX) {
                 advy = fy * (-(q1(i, j) * q1(i, j) / dq1(i, j)));
-              else
+                ++br[1][kBrUpNegDeg];
+              } else {
                 advy = fy * (q1(i, jp1) * q1(i, jp1) / dq1(i, jp1) - q1(i, j) * q1(i, j) / dq1(i, j));
+                ++br[1][kBrUpNegFull];
+              }
             } else {
-              if (dq1(i, jm1) < GX || dz2(i, j) < GX)
+              if (dq1(i, jm1) < GX || dz2(i, j) < GX) {
                 advy = fy * (q1(i, j) * q1(i, j) / dq1(i, j));
-              else
+                ++br[1][kBrUpPosDeg];
+              } else {
                 advy = fy * (q1(i, j) * q1(i, j) / dq1(i, j) - q1(i, jm1) * q1(i, jm1) / dq1(i, jm1));
+                ++br[1][kBrUpPosFull];
+              }
             }
             if (xpp < T(0)) {
               if (dz2(ip1, j) < GX || dz2(ip1, jp1) < GX) {
                 advx = fx * (-(q1(i, j) * xpp / dq1(i, j)));
+                ++br[1][kBrCrNegDz];
               } else if (dq1(ip1, j) < GX) {
                 advx = fx * (-(q1(i, j) * xpp / dq1(i, j)));
+                ++br[1][kBrCrNegDp];
               } else {
                 T xpe = T(0.25f) * (p1(ip1, j) + p1(ip1, jp1) + p1(i, j) + p1(i, jp1));
                 advx = fx * (q1(ip1, j) * xpe / dq1(ip1, j) - q1(i, j) * xpp / dq1(i, j));
+                ++br[1][kBrCrNegFull];
               }
             } else {
               if (dz2(im1, j) < GX || dz2(im1, jp1) < GX) {
                 advx = fx * (q1(i, j) * xpp / dq1(i, j));
+                ++br[1][kBrCrPosDz];
               } else if (dq1(i - 1, j) < GX) {
                 advx = fx * (q1(i, j) * xpp / dq1(i, j));
+                ++br[1][kBrCrPosDp];
               } else {
                 T xpe = T(0.25f) * (p1(im1, j) + p1(im1, jp1) + p1(i - 2, j) + p1(i - 2, jp1));
                 advx = fx * (q1(i, j) * xpp / dq1(i, j) - q1(im1, j) * xpe / dq1(im1, j));
+                ++br[1][kBrCrPosFull];
               }
             }
-            wave_breaking(L, advx, advy, i, j);
+            wave_breaking(L, advx, advy, i, j, 1);
             XQ = XQ - advx - advy;
           }
           if (IFRIC != 1) XQ = XQ / (T(1) + FF);
-          if (m_abs(XQ) < EPS) XQ = T(0);
+          if (m_abs(XQ) < EPS) { XQ = T(0); ++br[1][kBrEpsZero]; }
           T lim = VL * DD;
-          if (XQ > lim) XQ = lim;
-          if (XQ < -lim) XQ = -lim;
+          if (XQ > lim) { XQ = lim; ++br[1][kBrVlimHi]; }
+          if (XQ < -lim) { XQ = -lim; ++br[1][kBrVlimLo]; }
           L.N(i, j, 2) = XQ;
         }
       }
@@ -1249,6 +1331,20 @@ void max_io_t(State<T> &S, int id, double *zw, double *zr, int *arr) {
 }  // namespace
 
 extern "C" {
+
+// Branch counters since the last call: out[d * n + k] for direction d = 0
+// (X), 1 (Y) and branch k of enum Branch, n = kNumBranch (returned); the
+// counters are cleared.
+int orc_branch_counts(void *hp, long long *out) {
+  Handle *h = (Handle *)hp;
+  long long(*br)[kNumBranch] = h->precision == 32 ? h->f->br : h->d->br;
+  for (int d = 0; d < 2; ++d)
+    for (int k = 0; k < kNumBranch; ++k) {
+      if (out) out[d * kNumBranch + k] = br[d][k];
+      br[d][k] = 0;
+    }
+  return kNumBranch;
+}
 
 void *orc_new(int precision, int pow_mode) {
   Handle *h = new Handle();

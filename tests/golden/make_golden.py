@@ -24,6 +24,7 @@ sys.path.insert(0, os.path.join(ROOT, "gpu-comcot_amd"))
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 from gcomcot import scenarios as S  # noqa: E402
+from helpers import breaking_state, surf_zone  # noqa: E402
 from oracle import bind  # noqa: E402
 
 BEACH = dict(embayments=2, beach_slope=3000.0, bay_floor=6.0, coast_frac=0.3, peak=3.0, cx=0.5)
@@ -42,6 +43,10 @@ def scenario_list():
          40),
         ("nest_sph_sphNL_R3T5", S.custom(nx=48, ny=40, laygov=0, child=child(0, 1, 3, 5), **BEACH),
          30),
+        # surf-zone runs: finite WAVE_BREAKING and the V_LIMIT clamp
+        ("surf_sph_nonlinear", surf_zone(0), 160),
+        ("surf_cart_nonlinear", surf_zone(1), 160),
+        ("surf_cart_nonlinear_nofric", surf_zone(1, fric_coef=0.0), 160),
     ]
 
 
@@ -114,6 +119,30 @@ def step_local(name, sc, op, seed, t=0, lid=1):
     return arrays
 
 
+def step_local_breaking(sc, seed, settle):
+    """CONMOME(_S) once on a state whose wet cells rise or fall at u times
+    sqrt(g*D) per second, u uniform in [0, 1): below, between and above the
+    0.08 and 0.65 thresholds of WAVE_BREAKING (moment.f90:1300-1335)."""
+    rng = np.random.default_rng(seed)
+    ref = bind.Reference(sc)
+    ref.step(settle)
+    st = ref.state(1)
+    for k in ("z", "m", "n"):
+        st[k + "2"] = st[k + "1"].copy()
+    ref.set_state(1, st)
+    ref.call("MASS", 1)  # level-2 elevation and total depth, consistent
+    st = breaking_state(rng, ref.state(1), sc.layers[0].h, sc.layers[0].dt)
+    ref.set_state(1, st)
+    ref.call("MOMENT", 1)
+    out = ref.state(1)
+    arrays = spec_arrays(sc)
+    for k in ("z1", "z2", "m1", "m2", "n1", "n2", "dz1", "dz2"):
+        arrays["in_" + k] = st[k]
+        arrays["out_" + k] = out[k]
+    arrays["op"] = np.array([bind.OP["MOMENT"], 1, 0], np.int64)
+    return arrays
+
+
 def main():
     bind.build(ref=True)
     flang = subprocess.run(["/opt/rocm/lib/llvm/bin/flang", "--version"], capture_output=True,
@@ -130,6 +159,10 @@ def main():
         ("local_moment_sph_nl", lambda: step_local("x", sl, "MOMENT", 3)),
         ("local_moment_cart_nl", lambda: step_local("y", cl, "MOMENT", 4)),
         ("local_moment_sph_lin", lambda: step_local("z", ll, "MOMENT", 5)),
+        ("local_moment_sph_brk", lambda: step_local_breaking(surf_zone(0), 6, 120)),
+        ("local_moment_cart_brk", lambda: step_local_breaking(surf_zone(1), 7, 120)),
+        ("local_moment_cart_brk_nofric",
+         lambda: step_local_breaking(surf_zone(1, fric_coef=0.0), 8, 120)),
     ]
     for name, fn in jobs:
         arrays = fn()

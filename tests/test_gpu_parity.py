@@ -10,7 +10,7 @@ import pytest
 
 from gcomcot import DeviceRun
 from gcomcot import scenarios as S
-from helpers import bits, diff_report, small_scenarios
+from helpers import compare_device_oracle, diff_report, small_scenarios
 from oracle.bind import Oracle
 
 pytestmark = pytest.mark.gpu
@@ -19,20 +19,7 @@ CASES = small_scenarios()
 
 
 def _compare(sc, dev, orc, tag):
-    for L in sc.layers:
-        g, o = dev.state(L.id), orc.state(L.id)
-        for k in ("z1", "m1", "n1"):
-            nb, mx = diff_report(g[k], o[k])
-            assert nb == 0, f"{tag} layer {L.id} {k}: {nb} cells differ (max {mx})"
-        gm, om = dev.maxima(L.id), orc.maxima(L.id)
-        nb, mx = diff_report(gm["zmax_wet"], om["zmax_wet"])
-        assert nb == 0, f"{tag} layer {L.id} zmax: {nb} cells differ (max {mx})"
-        assert np.array_equal(gm["arrival"], om["arrival"]), f"{tag} layer {L.id} arrival differs"
-    if len(sc.layers) > 1:
-        gx, gy = dev.flux_cache(1)
-        ox, oy = orc.flux(1)
-        assert np.array_equal(bits(gx), bits(ox)) and np.array_equal(bits(gy), bits(oy)), \
-            f"{tag} interface flux cache differs"
+    compare_device_oracle(sc, dev, orc, tag)
 
 
 @pytest.mark.parametrize("name,sc,steps", CASES, ids=[c[0] for c in CASES])
