@@ -35,6 +35,11 @@ const float kVLimit = 20.0f;
 const float kRadDeg = 0.01745329252f;
 const float kRadMin = 0.000290888208665721f;
 const float kCoriW = 7.2722e-5f;
+
+// Planes of at least this many cells run their cell loops on OpenMP threads.
+// Every cell of such a loop is computed from inputs that no other cell of the
+// same loop writes, so the result does not depend on the thread count.
+const size_t kParCells = 16384;
 const float kOSixty = 0.016666666667f;
 const float kUB = 99.0f;  // boundaries.f90:30
 
@@ -195,6 +200,7 @@ struct State {
     const T GX = T(kGx), EPS = T(kEps), ELMAX = T(kElmax);
     int is = 2, js = 2, ie, je;
     if (L.id == 1) { ie = L.nx - 1; je = L.ny - 1; } else { ie = L.nx; je = L.ny; }
+#pragma omp parallel for schedule(static) if (L.nxy() >= kParCells)
     for (int j = js; j <= je; ++j) {
       int jm1 = j - 1;
       for (int i = is; i <= ie; ++i) {
@@ -226,6 +232,7 @@ struct State {
     const T RX = L.rx, RY = L.ry;
     int is = 2, js = 2, ie, je;
     if (L.id == 1) { ie = L.nx - 1; je = L.ny - 1; } else { ie = L.nx; je = L.ny; }
+#pragma omp parallel for schedule(static) if (L.nxy() >= kParCells)
     for (int j = js; j <= je; ++j) {
       for (int i = is; i <= ie; ++i) {
         T h = L.H(i, j);
@@ -354,6 +361,7 @@ struct State {
     const T GX = T(kGx), EPS = T(kEps);
     int ie = L.nx - 1, je = L.ny - 1, is = 2, js = 2;
     if (L.id == 1) { is = 1; js = 1; }
+#pragma omp parallel for schedule(static) if (L.nxy() >= kParCells)
     for (int i = is; i <= ie; ++i) {
       int ip1 = i + 1;
       for (int j = js; j <= L.ny; ++j) {
@@ -368,6 +376,7 @@ struct State {
         } else L.M(i, j, 2) = T(0);
       }
     }
+#pragma omp parallel for schedule(static) if (L.nxy() >= kParCells)
     for (int j = js; j <= je; ++j) {
       int jp1 = j + 1;
       for (int i = is; i <= L.nx; ++i) {
@@ -389,13 +398,13 @@ struct State {
     const T GX = T(kGx), EPS = T(kEps), TWLVTH = T(0.08333333333333f);
     const T CONST = T(0.5f) * L.dt * T(kGrav);
     const T FM = L.fric_coef;
-    T FF = T(0);
     int ie = L.nx - 1, je = L.ny - 1, is, js;
     if (L.id == 1) { is = 1; js = 1; } else { is = 2; js = 2; }
+#pragma omp parallel for schedule(static) if (L.nxy() >= kParCells)
     for (int i = is; i <= ie; ++i) {
       int ip1 = i + 1;
       for (int j = js; j <= je + 1; ++j) {
-        T xm;
+        T xm, FF = T(0);
         if (L.H(i, j) > GX && L.H(ip1, j) > GX) {
           int jm1 = j - 1, jp1 = j + 1;
           if (jm1 < 1) jm1 = 1;
@@ -425,9 +434,11 @@ struct State {
         }
       }
     }
+#pragma omp parallel for schedule(static) if (L.nxy() >= kParCells)
     for (int j = js; j <= je; ++j) {
       int jp1 = j + 1;
       for (int i = is; i <= ie + 1; ++i) {
+        T FF = T(0);
         if (L.H(i, j) > GX && L.H(i, jp1) > GX) {
           int im1 = i - 1, ip1 = i + 1;
           if (im1 < 1) im1 = 1;
@@ -457,7 +468,8 @@ struct State {
   }
 
   // WAVE_BREAKING (moment.f90:1272-1384): damping of the convection terms
-  void wave_breaking(Layer<T> &L, T &advx, T &advy, int i, int j, int dir) {
+  void wave_breaking(Layer<T> &L, T &advx, T &advy, int i, int j, int dir, long long *cnt) {
+    long long *bd = cnt + dir * kNumBranch;
     const T G = T(kGrav);
     T c = m_sqrt(G * L.DZ(i, j, 2));
     T dzdt_f = T(0.08f) * c;
@@ -465,10 +477,10 @@ struct State {
     T dzdt = m_abs(L.Z(i, j, 2) - L.Z(i, j, 1)) / L.dt;
     if (dzdt > dzdt_f) {
       if (c > T(0)) {
-        ++br[dir][kBrBrkFinite];
-        if (dzdt > dzdt_i) ++br[dir][kBrBrkOver];
+        ++bd[kBrBrkFinite];
+        if (dzdt > dzdt_i) ++bd[kBrBrkOver];
       } else {
-        ++br[dir][kBrBrkDry];
+        ++bd[kBrBrkDry];
       }
       T ref = (dzdt - dzdt_f) / (dzdt_i - dzdt_f);
       T a = T(1);
@@ -476,7 +488,7 @@ struct State {
       advx = coef * advx;
       advy = coef * advy;
     } else {
-      ++br[dir][kBrBrkNone];
+      ++bd[kBrBrkNone];
     }
   }
 
@@ -499,6 +511,7 @@ struct State {
     const size_t nxy = L.nxy();
     // DP, DQ (moment.f90:345-366 / 849-870)
     std::vector<T> DP(2 * nxy), DQ(2 * nxy);
+#pragma omp parallel for schedule(static) if (nxy >= kParCells)
     for (int i = 1; i <= IX; ++i) {
       for (int j = 1; j <= JY; ++j) {
         int ip1 = i + 1;
@@ -527,16 +540,20 @@ struct State {
     auto q1 = [&](int i, int j) { return L.N(i, j, 1); };
     const int ISPANS = 10, ISPANE = IX - 10, JSPANS = 10, JSPANE = JY - 10;
 
-    T DD = T(0), DF = T(0), FF = T(0), XP = T(0), XQ = T(0);
+    // Branch counts of this call, summed over threads (each flux point
+    // counts once whatever thread computes it)
+    long long cnt[2 * kNumBranch] = {};
     // ---- X direction
     {
       int is = 2, js = 2, ie = IX - 1, je = JY;
       if (L.id == 1) { is = 1; js = 1; ie = IX - 1; je = JY - 1; }
+#pragma omp parallel for schedule(static) if (nxy >= kParCells) reduction(+ : cnt[:2 * kNumBranch])
       for (int i = is; i <= ie; ++i) {
         int ip1 = i + 1, im1 = i - 1;
         if (im1 <= 1) im1 = 1;
         if (ip1 >= IX) ip1 = IX;
         for (int j = js; j <= je; ++j) {
+          T DD = T(0), DF = T(0), FF = T(0), XP = T(0);
           int noflux = 0;
           int jp1 = j + 1, jm1 = j - 1;
           if (jm1 <= 1) jm1 = 1;
@@ -547,29 +564,29 @@ struct State {
             if (dz2(i, j) > GX && dz2(ip1, j) > GX) {
               DD = DP[nxy + L.ix(i, j)];
               DF = dp1(i, j);
-              ++br[0][kBrDdWet];
+              ++cnt[kBrDdWet];
             } else if (dz2(i, j) > GX && dz2(ip1, j) <= GX && L.H(ip1, j) + z2(i, j) > GX) {
               DD = sph ? L.HP(i, j) + T(0.5f) * (z2(i, j) + z2(ip1, j)) : T(0.5f) * dz2(i, j);
               DF = DD;
-              ++br[0][kBrDdEdgeA];
+              ++cnt[kBrDdEdgeA];
             } else if (dz2(i, j) <= GX && dz2(ip1, j) > GX && L.H(i, j) + z2(ip1, j) > GX) {
               DD = sph ? L.HP(i, j) + T(0.5f) * (z2(i, j) + z2(ip1, j)) : T(0.5f) * dz2(ip1, j);
               DF = DD;
-              ++br[0][kBrDdEdgeB];
+              ++cnt[kBrDdEdgeB];
             } else {
               noflux = 1;
             }
             if (DD < GX) noflux = 1;
           }
-          if (noflux == 1) { L.M(i, j, 2) = T(0); ++br[0][kBrNoFlux]; continue; }
-          if (DF < T(1.0e-3f)) { DF = T(1.0e-3f); ++br[0][kBrDfClamp]; }
+          if (noflux == 1) { L.M(i, j, 2) = T(0); ++cnt[kBrNoFlux]; continue; }
+          if (DF < T(1.0e-3f)) { DF = T(1.0e-3f); ++cnt[kBrDfClamp]; }
           T xqq = T(0.25f) * (q1(i, j) + q1(ip1, j) + q1(i, jm1) + q1(ip1, jm1));
           if (IFRIC != 1) {
             FF = CONST * FM * FM * m_sqrt(p1(i, j) * p1(i, j) + xqq * xqq) / pw(DF, T(2.3333333f));
-            ++br[0][kBrFricOn];
+            ++cnt[kBrFricOn];
           } else {
             FF = T(0);
-            ++br[0][kBrFricOff];
+            ++cnt[kBrFricOff];
           }
           if (sph) {
             if (MOD_SCM == 0 && j < JY) {
@@ -598,57 +615,57 @@ struct State {
             T fx = sph ? L.R(L.r21, i, j) : RX;
             T fy = sph ? L.R(L.r0, i, j) : RY;
             T advx = T(0), advy = T(0);
-            ++br[0][kBrAdv];
+            ++cnt[kBrAdv];
             if (p1(i, j) < T(0)) {
               if (dp1(ip1, j) < GX || dz2(ip1, j) < GX) {
                 advx = fx * (-(p1(i, j) * p1(i, j) / dp1(i, j)));
-                ++br[0][kBrUpNegDeg];
+                ++cnt[kBrUpNegDeg];
               } else {
                 advx = fx * (p1(ip1, j) * p1(ip1, j) / dp1(ip1, j) - p1(i, j) * p1(i, j) / dp1(i, j));
-                ++br[0][kBrUpNegFull];
+                ++cnt[kBrUpNegFull];
               }
             } else {
               if (dp1(im1, j) < GX || dz2(i, j) < GX) {
                 advx = fx * (p1(i, j) * p1(i, j) / dp1(i, j));
-                ++br[0][kBrUpPosDeg];
+                ++cnt[kBrUpPosDeg];
               } else {
                 advx = fx * (p1(i, j) * p1(i, j) / dp1(i, j) - p1(im1, j) * p1(im1, j) / dp1(im1, j));
-                ++br[0][kBrUpPosFull];
+                ++cnt[kBrUpPosFull];
               }
             }
             if (xqq < T(0)) {
               if (dz2(i, jp1) < GX || dz2(ip1, jp1) < GX) {
                 advy = fy * (-(p1(i, j) * xqq / dp1(i, j)));
-                ++br[0][kBrCrNegDz];
+                ++cnt[kBrCrNegDz];
               } else if (dp1(i, jp1) < GX) {
                 advy = fy * (-(p1(i, j) * xqq / dp1(i, j)));
-                ++br[0][kBrCrNegDp];
+                ++cnt[kBrCrNegDp];
               } else {
                 T xqe = T(0.25f) * (q1(i, jp1) + q1(ip1, jp1) + q1(i, j) + q1(ip1, j));
                 advy = fy * (p1(i, jp1) * xqe / dp1(i, jp1) - p1(i, j) * xqq / dp1(i, j));
-                ++br[0][kBrCrNegFull];
+                ++cnt[kBrCrNegFull];
               }
             } else {
               if (dz2(i, jm1) < GX || dz2(ip1, jm1) < GX) {
                 advy = fy * (p1(i, j) * xqq / dp1(i, j));
-                ++br[0][kBrCrPosDz];
+                ++cnt[kBrCrPosDz];
               } else if (dp1(i, jm1) < GX) {
                 advy = fy * (p1(i, j) * xqq / dp1(i, j));
-                ++br[0][kBrCrPosDp];
+                ++cnt[kBrCrPosDp];
               } else {
                 T xqe = T(0.25f) * (q1(i, jm1) + q1(ip1, jm1) + q1(i, j - 2) + q1(ip1, j - 2));
                 advy = fy * (p1(i, j) * xqq / dp1(i, j) - p1(i, jm1) * xqe / dp1(i, jm1));
-                ++br[0][kBrCrPosFull];
+                ++cnt[kBrCrPosFull];
               }
             }
-            wave_breaking(L, advx, advy, i, j, 0);
+            wave_breaking(L, advx, advy, i, j, 0, cnt);
             XP = XP - advx - advy;
           }
           if (IFRIC != 1) XP = XP / (T(1) + FF);
-          if (m_abs(XP) < EPS) { XP = T(0); ++br[0][kBrEpsZero]; }
+          if (m_abs(XP) < EPS) { XP = T(0); ++cnt[kBrEpsZero]; }
           T lim = VL * DD;
-          if (XP > lim) { XP = lim; ++br[0][kBrVlimHi]; }
-          if (XP < -lim) { XP = -lim; ++br[0][kBrVlimLo]; }
+          if (XP > lim) { XP = lim; ++cnt[kBrVlimHi]; }
+          if (XP < -lim) { XP = -lim; ++cnt[kBrVlimLo]; }
           L.M(i, j, 2) = XP;
         }
       }
@@ -657,11 +674,15 @@ struct State {
     {
       int is = 2, js = 2, ie = IX, je = JY - 1;
       if (L.id == 1) { is = 1; js = 1; ie = IX - 1; je = JY - 1; }
+#pragma omp parallel for schedule(static) if (nxy >= kParCells) reduction(+ : cnt[:2 * kNumBranch])
       for (int i = is; i <= ie; ++i) {
         int ip1 = i + 1, im1 = i - 1;
         if (im1 <= 1) im1 = 1;
         if (ip1 >= IX) ip1 = IX;
         for (int j = js; j <= je; ++j) {
+          // XP: CONMOME's Y equation writes its friction into the X variable
+          // (moment.f90:672), a value nothing reads afterwards
+          T DD = T(0), DF = T(0), FF = T(0), XP = T(0), XQ = T(0);
           int noflux = 0;
           int jp1 = j + 1, jm1 = j - 1;
           if (jm1 <= 1) jm1 = 1;
@@ -672,29 +693,29 @@ struct State {
             if (dz2(i, j) > GX && dz2(i, jp1) > GX) {
               DD = DQ[nxy + L.ix(i, j)];
               DF = dq1(i, j);
-              ++br[1][kBrDdWet];
+              ++cnt[kNumBranch + kBrDdWet];
             } else if (dz2(i, j) > GX && dz2(i, jp1) <= GX && L.H(i, jp1) + z2(i, j) > GX) {
               DD = sph ? L.HQ(i, j) + T(0.5f) * (z2(i, j) + z2(i, jp1)) : T(0.5f) * dz2(i, j);
               DF = DD;
-              ++br[1][kBrDdEdgeA];
+              ++cnt[kNumBranch + kBrDdEdgeA];
             } else if (dz2(i, j) <= GX && dz2(i, jp1) > GX && L.H(i, j) + z2(i, jp1) > GX) {
               DD = sph ? L.HQ(i, j) + T(0.5f) * (z2(i, j) + z2(i, jp1)) : T(0.5f) * dz2(i, jp1);
               DF = DD;
-              ++br[1][kBrDdEdgeB];
+              ++cnt[kNumBranch + kBrDdEdgeB];
             } else {
               noflux = 1;
             }
             if (DD < GX) noflux = 1;
           }
-          if (noflux == 1) { L.N(i, j, 2) = T(0); ++br[1][kBrNoFlux]; continue; }
-          if (DF < T(1.0e-3f)) { DF = T(1.0e-3f); ++br[1][kBrDfClamp]; }
+          if (noflux == 1) { L.N(i, j, 2) = T(0); ++cnt[kNumBranch + kBrNoFlux]; continue; }
+          if (DF < T(1.0e-3f)) { DF = T(1.0e-3f); ++cnt[kNumBranch + kBrDfClamp]; }
           T xpp = T(0.25f) * (p1(i, j) + p1(i, jp1) + p1(im1, j) + p1(im1, jp1));
           if (IFRIC != 1) {
             FF = CONST * FM * FM * m_sqrt(q1(i, j) * q1(i, j) + xpp * xpp) / pw(DF, T(2.3333333f));
-            ++br[1][kBrFricOn];
+            ++cnt[kNumBranch + kBrFricOn];
           } else {
             FF = T(0);
-            ++br[1][kBrFricOff];
+            ++cnt[kNumBranch + kBrFricOff];
           }
           if (sph) {
             if (MOD_SCM == 0 && i < IX) {
@@ -724,60 +745,64 @@ struct State {
             T fy = sph ? L.R(L.r0, i, j) : RY;
             T fx = sph ? L.R(L.r22, i, j) : RX;
             T advx = T(0), advy = T(0);
-            ++br[1][kBrAdv];
+            ++cnt[kNumBranch + kBrAdv];
             if (q1(i, j) < T(0)) {
               if (dq1(i, jp1) < GX || dz2(i, jp1) < GX) {
                 advy = fy * (-(q1(i, j) * q1(i, j) / dq1(i, j)));
-                ++br[1][kBrUpNegDeg];
+                ++cnt[kNumBranch + kBrUpNegDeg];
               } else {
                 advy = fy * (q1(i, jp1) * q1(i, jp1) / dq1(i, jp1) - q1(i, j) * q1(i, j) / dq1(i, j));
-                ++br[1][kBrUpNegFull];
+                ++cnt[kNumBranch + kBrUpNegFull];
               }
             } else {
               if (dq1(i, jm1) < GX || dz2(i, j) < GX) {
                 advy = fy * (q1(i, j) * q1(i, j) / dq1(i, j));
-                ++br[1][kBrUpPosDeg];
+                ++cnt[kNumBranch + kBrUpPosDeg];
               } else {
                 advy = fy * (q1(i, j) * q1(i, j) / dq1(i, j) - q1(i, jm1) * q1(i, jm1) / dq1(i, jm1));
-                ++br[1][kBrUpPosFull];
+                ++cnt[kNumBranch + kBrUpPosFull];
               }
             }
             if (xpp < T(0)) {
               if (dz2(ip1, j) < GX || dz2(ip1, jp1) < GX) {
                 advx = fx * (-(q1(i, j) * xpp / dq1(i, j)));
-                ++br[1][kBrCrNegDz];
+                ++cnt[kNumBranch + kBrCrNegDz];
               } else if (dq1(ip1, j) < GX) {
                 advx = fx * (-(q1(i, j) * xpp / dq1(i, j)));
-                ++br[1][kBrCrNegDp];
+                ++cnt[kNumBranch + kBrCrNegDp];
               } else {
                 T xpe = T(0.25f) * (p1(ip1, j) + p1(ip1, jp1) + p1(i, j) + p1(i, jp1));
                 advx = fx * (q1(ip1, j) * xpe / dq1(ip1, j) - q1(i, j) * xpp / dq1(i, j));
-                ++br[1][kBrCrNegFull];
+                ++cnt[kNumBranch + kBrCrNegFull];
               }
             } else {
               if (dz2(im1, j) < GX || dz2(im1, jp1) < GX) {
                 advx = fx * (q1(i, j) * xpp / dq1(i, j));
-                ++br[1][kBrCrPosDz];
+                ++cnt[kNumBranch + kBrCrPosDz];
               } else if (dq1(i - 1, j) < GX) {
                 advx = fx * (q1(i, j) * xpp / dq1(i, j));
-                ++br[1][kBrCrPosDp];
+                ++cnt[kNumBranch + kBrCrPosDp];
               } else {
                 T xpe = T(0.25f) * (p1(im1, j) + p1(im1, jp1) + p1(i - 2, j) + p1(i - 2, jp1));
                 advx = fx * (q1(i, j) * xpp / dq1(i, j) - q1(im1, j) * xpe / dq1(im1, j));
-                ++br[1][kBrCrPosFull];
+                ++cnt[kNumBranch + kBrCrPosFull];
               }
             }
-            wave_breaking(L, advx, advy, i, j, 1);
+            wave_breaking(L, advx, advy, i, j, 1, cnt);
             XQ = XQ - advx - advy;
           }
           if (IFRIC != 1) XQ = XQ / (T(1) + FF);
-          if (m_abs(XQ) < EPS) { XQ = T(0); ++br[1][kBrEpsZero]; }
+          if (m_abs(XQ) < EPS) { XQ = T(0); ++cnt[kNumBranch + kBrEpsZero]; }
           T lim = VL * DD;
-          if (XQ > lim) { XQ = lim; ++br[1][kBrVlimHi]; }
-          if (XQ < -lim) { XQ = -lim; ++br[1][kBrVlimLo]; }
+          if (XQ > lim) { XQ = lim; ++cnt[kNumBranch + kBrVlimHi]; }
+          if (XQ < -lim) { XQ = -lim; ++cnt[kNumBranch + kBrVlimLo]; }
           L.N(i, j, 2) = XQ;
         }
       }
+    }
+      for (int k = 0; k < kNumBranch; ++k) {
+      br[0][k] += cnt[k];
+      br[1][k] += cnt[kNumBranch + k];
     }
   }
 
@@ -968,6 +993,7 @@ struct State {
   void max_arrival(Layer<T> &L) {
     const T GX = T(kGx);
     const size_t n = L.nxy();
+#pragma omp parallel for schedule(static) if (n >= kParCells)
     for (size_t k = 0; k < n; ++k) {
       if (L.parent != -1 && ((int)(k % L.nx) == 0 || (int)(k / L.nx) == 0)) continue;
       T z2 = L.z[n + k];

@@ -87,9 +87,10 @@ def test_surf_zone_every_tile_shape():
 
 def _noisy_state(sc, seed, settle=120, noise=4.0, pairs=300):
     """Level 1 of the surf zone after `settle` steps, with N(0, noise) added to
-    the fluxes and `pairs` horizontally and vertically adjacent cell pairs
-    made 'thin' (total depth in [1E-5, 2.5E-5) m, zero flux on every face
-    of the pair, so continuity keeps Z exactly and DZ2 = Z + H in REAL)."""
+    the fluxes, level 2 equal to level 1, and `pairs` horizontally and
+    vertically adjacent cell pairs made 'thin' (total depth in [1E-5,
+    2.5E-5) m, zero flux on every face of the pair, so continuity keeps Z
+    exactly and DZ2 = Z + H in REAL)."""
     rng = np.random.default_rng(seed)
     base = Oracle(sc, 32, pow_mode=1)
     base.step(settle)
@@ -120,7 +121,9 @@ def _noisy_state(sc, seed, settle=120, noise=4.0, pairs=300):
             n[jj - 1, ii] = 0
         planted += 1
     zero = np.zeros_like(z)
-    return dict(z1=z, m1=m, n1=n, z2=zero, m2=zero, n2=zero, dz1=zero, dz2=zero)
+    # level 2 as the level copy leaves it (comcot.f90:475-512): OPEN reads
+    # the level-2 fluxes before MOMENT rewrites them (boundaries.f90:22-202)
+    return dict(z1=z, m1=m, n1=n, z2=z.copy(), m2=m.copy(), n2=n.copy(), dz1=zero, dz2=zero)
 
 
 @pytest.mark.parametrize("laycord,fric", [(0, 0.013), (1, 0.013), (1, 0.0)],
