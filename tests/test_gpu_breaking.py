@@ -100,6 +100,11 @@ def _noisy_state(sc, seed, settle=120, noise=4.0, pairs=300):
     z, m, n = (np.array(st[k], np.float32) for k in ("z1", "m1", "n1"))
     m = (m + rng.normal(0.0, noise, m.shape)).astype(np.float32)
     n = (n + rng.normal(0.0, noise, n.shape)).astype(np.float32)
+    # CONMOME(_S) never writes the fluxes of the last column and row of the
+    # outermost layer (moment.f90:377-387, 873-883), so a run holds 0 there
+    for f in (m, n):
+        f[:, -1] = 0.0
+        f[-1, :] = 0.0
     ny, nx = h.shape
     planted = 0
     for _ in range(pairs * 4):
