@@ -14,6 +14,7 @@
 
 #include "comcot_consts.h"
 #include "gcomcot_hip_internal.h"
+#include "hotstart.h"
 #include "kernels.h"
 #include "output.h"
 #include "source.h"
@@ -64,6 +65,21 @@ int create_layer(int id, int parent, int level, int laycord, int laygov, int rel
   if (id == 1) g_faults.clear();
   return tree_create_layer(g_tree, g_stream, id, parent, level, laycord, laygov, rel_size,
                            rel_time, corners, fric_switch, fric_coef, dt, dx, dy, nx, ny, H, Z);
+}
+
+// The device half of gcomcot_detmath_eval_ and of its no-allocation
+// variant: x[] (host) into d[0, n), the kernel writes d[n, 2n) and d[2n, 3n),
+// both copied out; returns after the copies have completed.
+int detmath_eval_staged(float *d, long long n, const float *x, float *pow_out,
+                        float *exp_out) {
+  const size_t bytes = (size_t)n * sizeof(float);
+  hipError_t e = hipMemcpyAsync(d, x, bytes, hipMemcpyHostToDevice, g_stream);
+  if (e == hipSuccess) e = launch_detmath_eval(d, n, d + n, d + 2 * n, g_stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(pow_out, d + n, bytes, hipMemcpyDeviceToHost, g_stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(exp_out, d + 2 * n, bytes, hipMemcpyDeviceToHost, g_stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(g_stream);
+  if (e != hipSuccess) return fail(207, "detmath eval failed: %s", hipGetErrorString(e));
+  return 0;
 }
 
 }  // namespace
@@ -455,6 +471,57 @@ int gcomcot_checkpoint_load_(const char *path, int *pathlen) {
   return 0;
 }
 
+int gcomcot_hotstart_write_(int *start_step, const char *dir, int *dirlen) {
+  if (!layer(1)) return fail(8, "no root layer");
+  if (!start_step || !dirlen || *dirlen < 0 || (*dirlen > 0 && !dir))
+    return fail(95, "hot start: bad arguments");
+  if (int e = ensure_init()) return e;
+  if (sync()) return 1;
+  return tree_hotstart_write(g_tree, g_stream, *start_step, std::string(dir ? dir : "", (size_t)*dirlen));
+}
+
+int gcomcot_hotstart_read_(int *start_type, int *start_step, const char *dir, int *dirlen,
+                           int *nmissing) {
+  if (!layer(1)) return fail(8, "no root layer");
+  if (!start_type || !start_step || !dirlen || *dirlen < 0 || (*dirlen > 0 && !dir))
+    return fail(95, "hot start: bad arguments");
+  if (*start_step < 1) return fail(95, "hot start: START_STEP %d below 1", *start_step);
+  if (int e = ensure_init()) return e;
+  if (sync()) return 1;
+  int miss = 0;
+  if (int e = tree_hotstart_read(g_tree, g_stream, *start_type, *start_step,
+                                 std::string(dir ? dir : "", (size_t)*dirlen), &miss))
+    return e;
+  // the resumed loop starts at root step K = START_STEP (comcot.f90:228-230,
+  // 322), so START_STEP - 1 root steps lie behind the loaded state
+  const int done = *start_step - 1;
+  HIPCHK(hipMemcpy(g_tree.ctl + g_tree.ctl_in, &done, sizeof(int), hipMemcpyHostToDevice));
+  const float next = loop_time(done) + g_tree.layers[1].dt;
+  for (size_t i = 0; i < g_faults.size();)
+    if (g_faults[i].t0 < next) g_faults.erase(g_faults.begin() + (long)i);
+    else ++i;
+  if (nmissing) *nmissing = miss;
+  return 0;
+}
+
+int gcomcot_hotstart_file_write_(const char *path, int *pathlen, const float *plane, int *nx,
+                                 int *ny, int *first) {
+  if (!path || !pathlen || *pathlen <= 0 || !plane || !nx || !ny || !first)
+    return fail(95, "hot start: bad arguments");
+  return hotstart_write_plane(std::string(path, (size_t)*pathlen).c_str(), plane, *nx, *ny,
+                              *first);
+}
+
+int gcomcot_hotstart_file_read_(const char *path, int *pathlen, float *plane, int *nx, int *ny,
+                                int *first) {
+  if (!path || !pathlen || *pathlen <= 0 || !plane || !nx || !ny || !first)
+    return fail(95, "hot start: bad arguments");
+  const std::string p(path, (size_t)*pathlen);
+  const int e = hotstart_read_plane(p.c_str(), plane, *nx, *ny, *first);
+  if (e == kHotstartMissing) return fail(96, "hot start: cannot open %s", p.c_str());
+  return e;
+}
+
 int gcomcot_sponge_set_(int *on) {
   LayerRec *R = layer(1);
   if (!R) return fail(8, "no root layer");
@@ -738,16 +805,9 @@ int gcomcot_detmath_eval_(long long *n, const float *x, float *pow_out, float *e
   const size_t bytes = (size_t)*n * sizeof(float);
   float *d = nullptr;
   HIPCHK(hipMalloc(&d, 3 * bytes));
-  hipError_t e = hipMemcpyAsync(d, x, bytes, hipMemcpyHostToDevice, g_stream);
-  if (e == hipSuccess) e = launch_detmath_eval(d, *n, d + *n, d + 2 * *n, g_stream);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(pow_out, d + *n, bytes, hipMemcpyDeviceToHost, g_stream);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(exp_out, d + 2 * *n, bytes, hipMemcpyDeviceToHost, g_stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(g_stream);
+  const int rc = detmath_eval_staged(d, *n, x, pow_out, exp_out);
   (void)hipFree(d);
-  if (e != hipSuccess) return fail(207, "detmath eval failed: %s", hipGetErrorString(e));
-  return 0;
+  return rc;
 }
 
 int gcomcot_set_tile_variant_(int *variant) {

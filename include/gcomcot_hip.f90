@@ -208,6 +208,22 @@ module gcomcot_hip
       integer(c_int) :: pathlen
     end function
 
+    ! The reference's hot-start files (HOT_START START_TYPE 0 / 1 / 2)
+    integer(c_int) function gcomcot_hotstart_write(start_step, dir, dirlen) &
+        bind(C, name="gcomcot_hotstart_write_")
+      import :: c_int, c_char
+      integer(c_int) :: start_step, dirlen
+      character(kind=c_char) :: dir(*)
+    end function
+
+    integer(c_int) function gcomcot_hotstart_read(start_type, start_step, dir, dirlen, &
+                                                  nmissing) &
+        bind(C, name="gcomcot_hotstart_read_")
+      import :: c_int, c_char
+      integer(c_int) :: start_type, start_step, dirlen, nmissing
+      character(kind=c_char) :: dir(*)
+    end function
+
     ! Boundary treatments of the outermost layer (SPONGE_COEF/SPONGE_LAYER, BC_WALL)
     subroutine gcomcot_sponge_coef(laycord, nx, ny, h, x, y, cx, cy) &
         bind(C, name="gcomcot_sponge_coef_")

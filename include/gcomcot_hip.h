@@ -211,6 +211,38 @@ int gcomcot_checkpoint_load_(const char *path, int *pathlen);
 int gcomcot_batch_checkpoint_save_(int *sid, const char *path, int *pathlen);
 int gcomcot_batch_checkpoint_load_(int *sid, const char *path, int *pathlen);
 
+/* The reference's hot-start files (HOT_START / HOTSTART_DATA,
+ * hotstart.f90:3-166, called at K = START_STEP in comcot.f90:330-333,
+ * where the GPU build prints that it is unsupported).  Five text files per
+ * layer in directory dir[0..*dirlen) (the working directory when empty):
+ * z1_LL_SSSSSS.dat, m1_, n1_, zmax1_, zmin1_ with LL the layer id and
+ * SSSSSS the start step, each holding cells I, J >= 1 of the outermost and
+ * I, J >= 2 of a nested layer, one '(10F12.5)' record group per row J.
+ * gcomcot_hotstart_write_ is START_TYPE 0: it writes level-1 Z, M, N of
+ * every layer, the maximum cuda_getzmax_ returns and zeros for Z_MIN (the
+ * reference never updates it).  The snapshot is meant for a state after
+ * START_STEP - 1 root steps, as the reference's.  gcomcot_hotstart_read_
+ * is START_TYPE 1 (the outermost layer) or 2 (every layer): it loads Z, M,
+ * N of level 1 and that maximum (Z_MIN is read and dropped: the device
+ * keeps none), with HOTSTART_DATA's fallback of a zero plane for a file
+ * that cannot be opened (*nmissing counts them), and sets the root-step
+ * count to START_STEP - 1 so that the loop continues at K = START_STEP.
+ * Every file is read and checked before any plane changes; a malformed or
+ * short file returns 92/93 and changes nothing.  The wet maximum and the
+ * arrival field are not in the files and keep their values.  The values
+ * are rounded to 5 decimals, so a resumed run differs from the run that
+ * wrote them; exact resumption is gcomcot_checkpoint_save_/load_. */
+int gcomcot_hotstart_write_(int *start_step, const char *dir, int *dirlen);
+int gcomcot_hotstart_read_(int *start_type, int *start_step, const char *dir, int *dirlen,
+                           int *nmissing);
+/* One such file and a host plane of *nx x *ny cells (column-major), for
+ * cells I, J >= *first (1 or 2); host only.  The reader returns 96 when the
+ * file cannot be opened. */
+int gcomcot_hotstart_file_write_(const char *path, int *pathlen, const float *plane, int *nx,
+                                 int *ny, int *first);
+int gcomcot_hotstart_file_read_(const char *path, int *pathlen, float *plane, int *nx, int *ny,
+                                int *first);
+
 /* Boundary treatments of the outermost layer.
  * gcomcot_sponge_coef_ is SPONGE_COEF (boundaries.f90:250-330, with H_CALC
  * :334-365) on the host, in REAL: cx[NX] = COEFX(I, .), cy[NY] = COEFY(., J)

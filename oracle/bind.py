@@ -381,6 +381,25 @@ class Reference:
         self.lib().ref_deform_io(_i(lid), _p(d), _p(h))
         return d.reshape(s.ny, s.nx), h.reshape(s.ny, s.nx)
 
+    def hot_start(self, start_type, start_step):
+        """The reference's HOT_START (hotstart.f90:3-25) in the working
+        directory: START_TYPE 0 writes every layer's snapshot files, 1 loads
+        layer 1, 2 every layer."""
+        self.lib().ref_hot_start(_i(start_type), _i(start_step))
+
+    def zmaxmin(self, lid, zmax=None, zmin=None):
+        """Z_MAX and Z_MIN of layer lid: set them when given, else return them."""
+        s = self.specs[lid]
+        if zmax is not None:
+            a = np.ascontiguousarray(zmax, np.float32).reshape(-1)
+            b = np.ascontiguousarray(zmin, np.float32).reshape(-1)
+            self.lib().ref_zmaxmin_io(_i(lid), _i(1), _p(a), _p(b))
+            return None
+        a = np.empty(s.nx * s.ny, np.float32)
+        b = np.empty(s.nx * s.ny, np.float32)
+        self.lib().ref_zmaxmin_io(_i(lid), _i(0), _p(a), _p(b))
+        return a.reshape(s.ny, s.nx), b.reshape(s.ny, s.nx)
+
     def max_amp(self, time, tend, tide=0.0):
         """The reference's MAX_AMP (output.f90:855-1031) at model time
         `time` (REAL): writes zmax files into the working directory when due."""

@@ -646,3 +646,43 @@ subroutine ref_cuda_getzmax(zmax, lid) bind(C, name="cuda_getzmax_")
   end if
   zmax(1:lo%nx*lo%ny) = reshape(zmaxr, (/lo%nx*lo%ny/))
 end subroutine ref_cuda_getzmax
+
+!.....HOT_START (hotstart.f90:3-25) on the driver's layers, in the current
+!     working directory: START_TYPE 0 writes the snapshot files of every
+!     layer, 1 loads layer 1 and 2 every layer
+subroutine ref_hot_start(start_type, start_step) bind(C, name="ref_hot_start")
+  use ref_state
+  implicit none
+  integer :: start_type, start_step
+  call hot_start(lo, la, start_type, start_step)
+end subroutine ref_hot_start
+
+!.....Z_MAX and Z_MIN of a layer, in (which = 1) or out (which = 0)
+subroutine ref_zmaxmin_io(id, which, zmax, zmin) bind(C, name="ref_zmaxmin_io")
+  use ref_state
+  implicit none
+  integer :: id, which, k, nx, ny
+  real :: zmax(*), zmin(*)
+  call ref_pick(id, k)
+  if (k .eq. 0) then
+     nx = lo%nx
+     ny = lo%ny
+     if (which .eq. 1) then
+        lo%z_max = reshape(zmax(1:nx*ny), (/nx, ny/))
+        lo%z_min = reshape(zmin(1:nx*ny), (/nx, ny/))
+     else
+        zmax(1:nx*ny) = reshape(lo%z_max, (/nx*ny/))
+        zmin(1:nx*ny) = reshape(lo%z_min, (/nx*ny/))
+     end if
+  else
+     nx = la(k)%nx
+     ny = la(k)%ny
+     if (which .eq. 1) then
+        la(k)%z_max = reshape(zmax(1:nx*ny), (/nx, ny/))
+        la(k)%z_min = reshape(zmin(1:nx*ny), (/nx, ny/))
+     else
+        zmax(1:nx*ny) = reshape(la(k)%z_max, (/nx*ny/))
+        zmin(1:nx*ny) = reshape(la(k)%z_min, (/nx*ny/))
+     end if
+  end if
+end subroutine ref_zmaxmin_io
