@@ -82,6 +82,34 @@ int detmath_eval_staged(float *d, long long n, const float *x, float *pow_out,
   return 0;
 }
 
+// Staging memory of gcomcot_detmath_eval_noalloc_: arguments, powers and
+// exponentials of up to kNoAllocMaxArgs values (12 MiB), a static device
+// array, so the entry point needs no allocation.  Each device holds its own
+// copy for the life of the process, at an address hipGetSymbolAddress gives
+// with that device current; host code uses that address only (the array's
+// name in host code does not denote device memory).  Every call on a device
+// shares its copy: calls that overlap on one device race.
+constexpr long long kNoAllocMaxArgs = 1LL << 20;
+__device__ __align__(256) float scratch[3 * kNoAllocMaxArgs];
+constexpr int kMaxDevices = 64;
+float *g_scratch[kMaxDevices] = {};
+
+// This device's scratch address, looked up on its first use.
+int noalloc_scratch(float **out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(208, "device %d beyond the scratch table", dev);
+  if (!g_scratch[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(208, "scratch address of device %d: %s", dev, hipGetErrorString(e));
+    g_scratch[dev] = static_cast<float *>(p);
+  }
+  *out = g_scratch[dev];
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -808,6 +836,20 @@ int gcomcot_detmath_eval_(long long *n, const float *x, float *pow_out, float *e
   const int rc = detmath_eval_staged(d, *n, x, pow_out, exp_out);
   (void)hipFree(d);
   return rc;
+}
+
+int gcomcot_detmath_eval_noalloc_(long long *n, const float *x, float *pow_out,
+                                  float *exp_out) {
+  if (!n || *n < 0 || (*n > 0 && (!x || !pow_out || !exp_out)))
+    return fail(36, "gcomcot_detmath_eval_noalloc_: n must be >= 0 and the arrays given");
+  if (*n > kNoAllocMaxArgs)
+    return fail(37, "gcomcot_detmath_eval_noalloc_: n = %lld exceeds the %lld arguments its "
+                "scratch holds", *n, kNoAllocMaxArgs);
+  if (*n == 0) return 0;
+  if (int e = ensure_init()) return e;
+  float *d = nullptr;
+  if (int e = noalloc_scratch(&d)) return e;
+  return detmath_eval_staged(d, *n, x, pow_out, exp_out);
 }
 
 int gcomcot_set_tile_variant_(int *variant) {

@@ -361,6 +361,17 @@ int gcomcot_division_check_(long long *pairs, unsigned long long *seed, long lon
  * the device arithmetic against the same functions compiled for the host. */
 int gcomcot_detmath_eval_(long long *n, const float *x, float *pow_out, float *exp_out);
 
+/* gcomcot_detmath_eval_ without any device allocation, for callers that
+ * cannot allocate (code under HIP graph capture, serving loops, harnesses
+ * that forbid hipMalloc).  No counterpart in the reference.  The device
+ * copies of x and of both results live in a static scratch array of the
+ * library (12 MiB per device, held until the process ends), so n is at
+ * most 1048576 (2^20); a larger n returns 37 and touches nothing.  Calls on
+ * one device share that array: calls that overlap on one device race.
+ * Same results and the other return codes as gcomcot_detmath_eval_. */
+int gcomcot_detmath_eval_noalloc_(long long *n, const float *x, float *pow_out,
+                                  float *exp_out);
+
 /* ------------------------------------------------------------------ */
 /* Part 3: scenario batch engine (no reference counterpart)            */
 /* ------------------------------------------------------------------ */

@@ -115,3 +115,19 @@ def test_detmath_eval_refuses_bad_arguments():
     assert last_error()[0] == 36
     n = ctypes.c_longlong(0)                       # nothing to do: no device needed
     assert lib().gcomcot_detmath_eval_(ctypes.byref(n), None, None, None) == 0
+
+
+def test_detmath_eval_noalloc_refuses_bad_arguments():
+    f = lib().gcomcot_detmath_eval_noalloc_
+    n = ctypes.c_longlong(-1)
+    assert f(ctypes.byref(n), None, None, None) == 36
+    n = ctypes.c_longlong(4)                       # a count without arrays
+    assert f(ctypes.byref(n), None, None, None) == 36
+    one = np.ones(1, np.float32)
+    p = one.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+    n = ctypes.c_longlong((1 << 20) + 1)           # beyond the scratch array
+    assert f(ctypes.byref(n), p, p, p) == 37
+    assert last_error()[0] == 37
+    n = ctypes.c_longlong(0)                       # nothing to do: no device needed
+    assert f(ctypes.byref(n), None, None, None) == 0
+
