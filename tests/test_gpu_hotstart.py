@@ -17,7 +17,7 @@ import pytest
 
 from gcomcot import DeviceRun
 from gcomcot import scenarios as S
-from gcomcot._lib import ip, lib
+from gcomcot._lib import ip, last_error, lib
 from helpers import BEACH, diff_report
 from oracle.bind import Oracle
 
@@ -148,6 +148,7 @@ def test_hotstart_type_1_and_missing_files(tmp_path):
     b = str(tmp_path).encode()
     miss = ctypes.c_int(0)
     assert lib().gcomcot_hotstart_read_(ip(2), ip(10), b, ip(len(b)), ctypes.byref(miss)) == 93
+    assert last_error()[0] == 93  # the refusal is reported once
     for k in ("z1", "m1", "n1"):
         _same(dev.state(1)[k], before[k], f"refused load keeps layer 1 {k}")
     dev.close()
