@@ -5,6 +5,7 @@
 // issued has completed, as the reference's calls do.
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -236,6 +237,122 @@ void all_grid_launch_(void) {
   sync();
 }
 
+static void copy_plane(float *host, float *dev, size_t n, bool to_dev) {
+  hipError_t e = to_dev ? hipMemcpy(dev, host, n * sizeof(float), hipMemcpyHostToDevice)
+                        : hipMemcpy(host, dev, n * sizeof(float), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) fail(204, "plane copy failed: %s", hipGetErrorString(e));
+}
+
+// ---- the reference's differential checks of the nest exchange
+// (GPUall_grids.cu:299-394, 626-643), which its Fortran calls under
+// DEBUG_CORE / DEBUG_ALL_GRID right after computing the same routine on the
+// host from the state that SET_ALL gave the device (all_grids.f90:363-366,
+// 416-418, 474-490, 638-650): the device runs its own JNQ/NEWQ or JNZ on
+// that state, and every value that differs from the host's by 1e-4 or more
+// (assert_diff with debug_option.h's TOLERANCE) is printed.
+static constexpr float kDbgTolerance = 1.0e-4f;
+static bool dbg_diff(float base, float eval) { return std::fabs(eval - base) >= kDbgTolerance; }
+
+static LayerRec *dbg_child(int pid, int cid, const char *who) {
+  LayerRec *P = layer(pid), *C = layer(cid);
+  if (!P || !C || C->parent != pid) {
+    fail(38, "%s: layer %d is not a child of layer %d", who, cid, pid);
+    return nullptr;
+  }
+  if (C->sc) {
+    fail(38, "%s: layer %d is coupled with SC_OPTION = 1", who, cid);
+    return nullptr;
+  }
+  return C;
+}
+
+void edgeinterp_dbglaunch_(float *mO, float *nO, int *idO, float *mA, float *nA, int *idA, int *t,
+                           float *yflux, float *xflux) {
+  (void)mO;
+  (void)nO;
+  LayerRec *C = dbg_child(*idO, *idA, "edgeinterp_dbglaunch_");
+  if (!C) return;
+  const LayerRec *P = layer(*idO);
+  if (*t < 1 || *t > C->rel_time) {
+    fail(39, "edgeinterp_dbglaunch_: sub-step %d outside 1..%d", *t, C->rel_time);
+    return;
+  }
+  const auto t0 = std::chrono::steady_clock::now();
+  hipError_t e = launch_jnq(nest_view(*P, *C), *t, g_stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(g_stream);
+  if (e != hipSuccess) { fail(209, "edgeinterp_dbglaunch_: %s", hipGetErrorString(e)); return; }
+  const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  printf("TIME SPENT ON GPU (EDGE_INTERP->%d) %f\n", C->id, sec);
+  const int nx = C->nx, ny = C->ny, pnx = P->nx, pny = P->ny;
+  std::vector<float> m(C->nxy()), n(C->nxy()), xf(2 * (size_t)pnx), yf(2 * (size_t)pny);
+  copy_plane(m.data(), C->m[C->cur], m.size(), false);
+  copy_plane(n.data(), C->n[C->cur], n.size(), false);
+  copy_plane(xf.data(), P->xflux, xf.size(), false);
+  copy_plane(yf.data(), P->yflux, yf.size(), false);
+  printf("==============lO:%d, lA:%d =============\n", P->id, C->id);
+  if (*t == 2) {  // the flux cache NEWQ keeps at its second sub-step
+    for (int i = C->corners[2] - 1; i <= C->corners[3]; ++i)
+      if (dbg_diff(yf[i], yflux[i]))
+        printf("(left-flux) i = %d, %f, %f, diff=%f\n", i, yf[i], yflux[i], std::fabs(yf[i] - yflux[i]));
+    for (int i = C->corners[2] - 1; i <= C->corners[3]; ++i) {
+      const int k = i + pny;
+      if (dbg_diff(yf[k], yflux[k]))
+        printf("(right-flux) i = %d, %f, %f, diff=%f\n", i, yf[k], yflux[k], std::fabs(yf[k] - yflux[k]));
+    }
+    for (int i = C->corners[0] - 1; i < C->corners[1]; ++i)
+      if (dbg_diff(xf[i], xflux[i]))
+        printf("(bottom-flux) i = %d, %f, %f, diff=%f\n", i, xf[i], xflux[i], std::fabs(xf[i] - xflux[i]));
+    for (int i = C->corners[0] - 1; i <= C->corners[1]; ++i) {
+      const int k = i + pnx;
+      if (dbg_diff(xf[k], xflux[k]))
+        printf("(top-flux) i = %d, %f, %f, diff=%f\n", i, xf[k], xflux[k], std::fabs(xf[k] - xflux[k]));
+    }
+  }
+  for (int j = 0; j < ny; ++j) {
+    const size_t k = (size_t)j * nx;
+    if (dbg_diff(m[k], mA[k]))
+      printf("(left) i = %d, %f, %f, diff=%f\n", j, m[k], mA[k], std::fabs(m[k] - mA[k]));
+  }
+  for (int j = 0; j < ny; ++j) {
+    const size_t k = (size_t)j * nx + nx - 1;
+    if (dbg_diff(m[k], mA[k]))
+      printf("(right) i = %d, %f, %f, diff=%f\n", j, m[k], mA[k], std::fabs(m[k] - mA[k]));
+  }
+  for (int i = 0; i < nx; ++i)
+    if (dbg_diff(n[i], nA[i]))
+      printf("(bottom) i = %d, %f, %f, diff=%f\n", i, n[i], nA[i], std::fabs(n[i] - nA[i]));
+  for (int i = 0; i < nx; ++i) {
+    const size_t k = (size_t)(ny - 1) * nx + i;
+    if (dbg_diff(n[k], nA[k]))
+      printf("(top) i = %d, %f, %f, diff=%f\n", i, n[k], nA[k], std::fabs(n[k] - nA[k]));
+  }
+  fflush(stdout);
+}
+
+void jnz_dbglaunch_(float *Z_f, int *outid, int *inid) {
+  LayerRec *C = dbg_child(*outid, *inid, "jnz_dbglaunch_");
+  if (!C) return;
+  LayerRec *P = layer(*outid);
+  hipError_t e = launch_jnz(nest_view(*P, *C), g_stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(g_stream);
+  if (e != hipSuccess) { fail(209, "jnz_dbglaunch_: %s", hipGetErrorString(e)); return; }
+  std::vector<float> z(P->nxy());
+  copy_plane(z.data(), P->z[1 - P->cur], z.size(), false);
+  printf("child corner %d %d %d %d\n", C->corners[0], C->corners[1], C->corners[2], C->corners[3]);
+  for (int j = 0; j < P->ny; ++j)
+    for (int i = 0; i < P->nx; ++i) {
+      const size_t k = (size_t)j * P->nx + i;
+      if (dbg_diff(z[k], Z_f[k])) {
+        unsigned a, b;
+        std::memcpy(&a, &z[k], 4);
+        std::memcpy(&b, &Z_f[k], 4);
+        printf("JNZ, layerid=%d(i,j)=(%d,%d), %f %f, diff=%f  %x, %x\n", C->id, i, j, z[k], Z_f[k],
+               (double)std::fabs(z[k] - Z_f[k]), a, b);
+      }
+    }
+  fflush(stdout);
+}
+
 void maxamp_launch_(const int *lid) {
   LayerRec *L = layer(*lid);
   if (!L) { fail(17, "maxamp_launch_: no layer %d", *lid); return; }
@@ -253,11 +370,6 @@ void cuda_update_layer_(int *lid) {
   L->cur = 1 - L->cur;
 }
 
-static void copy_plane(float *host, float *dev, size_t n, bool to_dev) {
-  hipError_t e = to_dev ? hipMemcpy(dev, host, n * sizeof(float), hipMemcpyHostToDevice)
-                        : hipMemcpy(host, dev, n * sizeof(float), hipMemcpyDeviceToHost);
-  if (e != hipSuccess) fail(204, "plane copy failed: %s", hipGetErrorString(e));
-}
 
 #define GC_ACCESSOR(name, expr, to_dev)                         \
   void name(float *Z, int *lid) {                               \

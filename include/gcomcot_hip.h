@@ -95,6 +95,25 @@ void gcomcot_get_z_(int *lid, float *l1, float *l2, float *l3, float *l4, int *i
  * device buffer of the library */
 void cuda_shutdown_(void);
 
+/* The reference's differential checks, called by its Fortran only when
+ * built with DEBUG_CORE / DEBUG_ALL_GRID (debug_option.h), right after the
+ * host computed the same routine from the state SET_ALL copied to the
+ * device.  Both run the device's own routine on that state and print every
+ * value that differs from the host's by 1e-4 or more, in the reference's
+ * format.
+ * replaces edgeinterp_dbglaunch_ (GPUall_grids.cu:299-394; called at
+ * all_grids.f90:490 and :650): JNQ (*t = 1) or NEWQ sub-step *t of child
+ * *idA from parent *idO, then compares the child's level-1 boundary fluxes
+ * (left/right columns of mA, bottom/top rows of nA) and, at *t = 2, the
+ * parent's flux cache YFLUX(NY,2)/XFLUX(NX,2) over the child's extent.
+ * mO/nO are unused, as in the reference. */
+void edgeinterp_dbglaunch_(float *mO, float *nO, int *idO, float *mA, float *nA, int *idA, int *t,
+                           float *yflux, float *xflux);
+/* replaces jnz_dbglaunch_ (GPUall_grids.cu:626-643; called at
+ * all_grids.f90:417): JNZ of child *inid into parent *outid, then compares
+ * the parent's whole level-2 elevation with Z_f. */
+void jnz_dbglaunch_(float *Z_f, int *outid, int *inid);
+
 /* ------------------------------------------------------------------ */
 /* Part 2: extensions (no reference counterpart)                       */
 /* ------------------------------------------------------------------ */

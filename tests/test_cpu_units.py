@@ -79,6 +79,22 @@ def test_unmodified_reference_links_against_the_library():
     assert "libgcomcot_hip.so" in ldd and "not found" not in ldd.split("libgcomcot_hip")[1][:80]
 
 
+@pytest.mark.skipif(not (os.path.exists(FLANG) and os.path.isdir("/root/reference")),
+                    reason="needs flang and the reference sources")
+def test_reference_debug_build_links_against_the_library():
+    """The reference compiled with its differential-debug switch (DEBUG_CORE,
+    debug_option.h) also calls edgeinterp_dbglaunch_ and jnz_dbglaunch_
+    (all_grids.f90:417, 490, 650): that build must link too."""
+    import subprocess
+    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "dropin-debug",
+                    "REF=/root/reference"], check=True, capture_output=True)
+    exe = os.path.join(ROOT, "oracle", "_ref", "dropin_debug", "comcot_hip_debug")
+    out = subprocess.run(["nm", "-D", "--undefined-only", exe], check=True,
+                         capture_output=True, text=True).stdout
+    undefined = {ln.split()[-1] for ln in out.splitlines() if ln.strip()}
+    assert {"edgeinterp_dbglaunch_", "jnz_dbglaunch_", "mass_launch_", "momt_launch_"} <= undefined
+
+
 def _libm():
     return ctypes.CDLL("libm.so.6")
 
