@@ -46,11 +46,12 @@ int num_tile_variants();
 void tile_variant_shape(int v, int *tx, int *ty);
 
 // Two root steps of a linear outermost layer without children in one
-// launch (temporal blocking); shape 0/1/2 = 64x8 / 32x16 / 64x16 tiles.
+// launch (temporal blocking); shape 0..6 = tiles of 64x8, 32x16, 64x16 with
+// 256 threads, 64x16 with 512 and 1024, 32x16 with 512, 64x32 with 1024.
 // Reads level 1, writes level 2 (two steps later); with zmax set, the
 // wet maximum and arrival take both steps and the counter advances by 2.
 hipError_t launch_step2(const DevLayer &L, bool sph, int shape, hipStream_t st);
-constexpr int kStep2Shapes = 3;
+constexpr int kStep2Shapes = 7;
 
 // Reference-semantics maximum (no wet test) plus the wet max/arrival.
 hipError_t launch_max(const DevLayer &L, float *zmax_raw, hipStream_t st);

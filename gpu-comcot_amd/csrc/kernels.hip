@@ -489,11 +489,12 @@ __device__ float nl_y(const DevLayer &L, const A &s, int i, int j, float brk) {
 }
 
 // Copy the pow/exp tables into LDS (read before the first __syncthreads).
+template <int NT = kThreads>
 __device__ __forceinline__ void stage_tables(gcdet::Tables &dst, int tid) {
   const double *src = reinterpret_cast<const double *>(&gcdet::kTab);
   double *d = reinterpret_cast<double *>(&dst);
   constexpr int n = sizeof(gcdet::Tables) / sizeof(double);
-  for (int k = tid; k < n; k += kThreads) d[k] = src[k];
+  for (int k = tid; k < n; k += NT) d[k] = src[k];
 }
 
 // Tile -> workgroup order: workgroup b runs on XCD b%8; XCD x gets the
@@ -710,7 +711,7 @@ __global__ void __launch_bounds__(kThreads)
 // bitwise the same; HBM traffic and launches per step halve, at the cost
 // of recomputing the halo ring (cheap for the linear equations).  The wet
 // maximum and arrival see both steps in order.
-template <bool SPH, int TX, int TY, bool IN>
+template <bool SPH, int TX, int TY, bool IN, int NT>
 __device__ __forceinline__ void tile_body2(const DevLayer &L, int t, int ti0, int tj0,
                                            float *sm, const gcdet::Tables *stab) {
   constexpr int HW = 4, P = TX + 2 * HW, R = TY + 2 * HW, A = P * R;
@@ -725,12 +726,12 @@ __device__ __forceinline__ void tile_body2(const DevLayer &L, int t, int ti0, in
   auto inside = [&](int i, int j) { return IN || (i >= 1 && i <= nx && j >= 1 && j <= ny); };
 
   if (SPH) {
-    for (int idx = tid; idx < kNumCoef * R; idx += kThreads) {
+    for (int idx = tid; idx < kNumCoef * R; idx += NT) {
       const int k = idx / R, j = gj0 + idx % R;
       crow[idx] = (IN || (j >= 1 && j <= ny)) ? L.coef[k * ny + (j - 1)] : 0.0f;
     }
   }
-  for (int idx = tid; idx < A; idx += kThreads) {
+  for (int idx = tid; idx < A; idx += NT) {
     const int gi = gi0 + idx % P, gj = gj0 + idx / P;
     const bool in = inside(gi, gj);
     const size_t g = in ? (size_t)(gj - 1) * nx + (gi - 1) : 0;
@@ -744,7 +745,7 @@ __device__ __forceinline__ void tile_body2(const DevLayer &L, int t, int ti0, in
   // step one, continuity on the tile + 3
   {
     constexpr int CX = TX + 6, CY = TY + 6;
-    for (int idx = tid; idx < CX * CY; idx += kThreads) {
+    for (int idx = tid; idx < CX * CY; idx += NT) {
       const int i = ti0 - 3 + idx % CX, j = tj0 - 3 + idx / CX;
       float z2 = 0.0f, dz2;
       if (inside(i, j)) mass_cell<false, SPH, IN>(L, s1, i, j, z2, dz2);
@@ -755,7 +756,7 @@ __device__ __forceinline__ void tile_body2(const DevLayer &L, int t, int ti0, in
   // step one, momentum on the tile + 2 (0 outside the MOMT ranges and the grid)
   {
     constexpr int CX = TX + 4, CY = TY + 4;
-    for (int idx = tid; idx < CX * CY; idx += kThreads) {
+    for (int idx = tid; idx < CX * CY; idx += NT) {
       const int i = ti0 - 2 + idx % CX, j = tj0 - 2 + idx / CX;
       float m = 0.0f, n = 0.0f;
       if (inside(i, j)) {
@@ -772,7 +773,7 @@ __device__ __forceinline__ void tile_body2(const DevLayer &L, int t, int ti0, in
   // step two, continuity on the tile + 1
   {
     constexpr int CX = TX + 2, CY = TY + 2;
-    for (int idx = tid; idx < CX * CY; idx += kThreads) {
+    for (int idx = tid; idx < CX * CY; idx += NT) {
       const int i = ti0 - 1 + idx % CX, j = tj0 - 1 + idx / CX;
       float z2 = 0.0f, dz2;
       if (inside(i, j)) mass_cell<false, SPH, IN>(L, s2, i, j, z2, dz2);
@@ -788,7 +789,7 @@ __device__ __forceinline__ void tile_body2(const DevLayer &L, int t, int ti0, in
     if (L.advance && t == 0 && tid == 0) L.ctl[1 - L.ctl_in] = step + 1;
   }
 #pragma unroll 1
-  for (int idx = tid; idx < TX * TY; idx += kThreads) {
+  for (int idx = tid; idx < TX * TY; idx += NT) {
     const int i = ti0 + idx % TX, j = tj0 + idx / TX;
     if (!IN && (i > nx || j > ny)) continue;
     const size_t g = (size_t)(j - 1) * nx + (i - 1);
@@ -827,20 +828,24 @@ __device__ __forceinline__ void tile_body2(const DevLayer &L, int t, int ti0, in
   }
 }
 
-template <bool SPH, int TX, int TY>
-__global__ void __launch_bounds__(kThreads)
+// NT threads per tile: small grids leave the chip with few workgroups per
+// CU, and more threads on the same LDS tile shorten each phase's chain of
+// dependent LDS and memory accesses there; the arithmetic of every cell is
+// the same whatever NT is.
+template <bool SPH, int TX, int TY, int NT>
+__global__ void __launch_bounds__(NT)
     k_step2(DevLayer L, int tiles_x, int ntiles, int per_xcd) {
   constexpr int HW = 4;
   __shared__ float sm[7 * (TX + 2 * HW) * (TY + 2 * HW) + kNumCoef * (TY + 2 * HW)];
   __shared__ gcdet::Tables stab;
   const int t = tile_of_block(per_xcd);
   if (t >= ntiles) return;
-  if (!SPH) stage_tables(stab, threadIdx.x);
+  if (!SPH) stage_tables<NT>(stab, threadIdx.x);
   const int ti0 = (t % tiles_x) * TX + 1, tj0 = (t / tiles_x) * TY + 1;
   const bool interior = ti0 - HW > kSpan + 2 && ti0 + TX + HW < L.nx - kSpan &&
                         tj0 - HW > kSpan + 2 && tj0 + TY + HW < L.ny - kSpan;
-  if (interior) tile_body2<SPH, TX, TY, true>(L, t, ti0, tj0, sm, &stab);
-  else tile_body2<SPH, TX, TY, false>(L, t, ti0, tj0, sm, &stab);
+  if (interior) tile_body2<SPH, TX, TY, true, NT>(L, t, ti0, tj0, sm, &stab);
+  else tile_body2<SPH, TX, TY, false, NT>(L, t, ti0, tj0, sm, &stab);
 }
 
 // ---------------------------------------------------------- strip kernel
@@ -1213,13 +1218,30 @@ __global__ void k_jnz_multi(const DevNest *__restrict__ nests, const int *__rest
   jnz_thread(N, (blockIdx.x - start[n]) * blockDim.x + threadIdx.x);
 }
 
-// Plain streaming copy (16 B per lane per access): the reference for
-// HBM bandwidth and for scaling the memory counters of rocprofv3.
-__global__ void __launch_bounds__(256) k_copy(const float4 *__restrict__ src,
-                                              float4 *__restrict__ dst, size_t n) {
-  const size_t stride = (size_t)gridDim.x * blockDim.x;
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-    dst[i] = src[i];
+// Streaming copy, the reference for HBM bandwidth and for scaling the
+// memory counters of rocprofv3: each workgroup copies one contiguous chunk
+// of kCopyUnroll x 256 16-byte vectors, all its loads issued before its
+// stores, non-temporal (streamed past the caches), and the hardware
+// dispatcher hands chunks to CUs as they free up.  Of the shapes measured
+// on MI355X (grid-stride loops with 1-8 loads in flight per lane and
+// 1024-8192 workgroups, chunks of 1-8 vectors per lane) this one moves the
+// most bytes per second.
+constexpr int kCopyUnroll = 4;
+typedef float CopyVec __attribute__((ext_vector_type(4)));
+__global__ void __launch_bounds__(256) k_copy(const CopyVec *__restrict__ src,
+                                              CopyVec *__restrict__ dst, size_t n) {
+  const size_t base = (size_t)blockIdx.x * (256 * kCopyUnroll) + threadIdx.x;
+  CopyVec v[kCopyUnroll];
+#pragma unroll
+  for (int u = 0; u < kCopyUnroll; ++u) {
+    const size_t i = base + (size_t)u * 256;
+    if (i < n) v[u] = __builtin_nontemporal_load(src + i);
+  }
+#pragma unroll
+  for (int u = 0; u < kCopyUnroll; ++u) {
+    const size_t i = base + (size_t)u * 256;
+    if (i < n) __builtin_nontemporal_store(v[u], dst + i);
+  }
 }
 
 // Self-check of the short divisions and square root against the IEEE
@@ -1692,11 +1714,11 @@ hipError_t launch_shape(const DevLayer &L, int mode, bool nl, bool sph, hipStrea
   return hipErrorInvalidValue;
 }
 
-template <bool SPH, int TX, int TY>
+template <bool SPH, int TX, int TY, int NT = kThreads>
 hipError_t launch2_t(const DevLayer &L, hipStream_t st) {
   const int tx = (L.nx + TX - 1) / TX, ty = (L.ny + TY - 1) / TY;
   const int nt = tx * ty, per = (nt + 7) / 8;
-  hipLaunchKernelGGL((k_step2<SPH, TX, TY>), dim3(8 * per), dim3(kThreads), 0, st, L, tx, nt,
+  hipLaunchKernelGGL((k_step2<SPH, TX, TY, NT>), dim3(8 * per), dim3(NT), 0, st, L, tx, nt,
                      per);
   return hipGetLastError();
 }
@@ -1737,12 +1759,19 @@ hipError_t launch_step(const DevLayer &L, int mode, bool nl, bool sph, int varia
 }
 
 hipError_t launch_step2(const DevLayer &L, bool sph, int shape, hipStream_t st) {
+#define GC_S2(TX, TY, NT) \
+  return sph ? launch2_t<true, TX, TY, NT>(L, st) : launch2_t<false, TX, TY, NT>(L, st)
   switch (shape) {
-    case 0: return sph ? launch2_t<true, 64, 8>(L, st) : launch2_t<false, 64, 8>(L, st);
-    case 1: return sph ? launch2_t<true, 32, 16>(L, st) : launch2_t<false, 32, 16>(L, st);
-    case 2: return sph ? launch2_t<true, 64, 16>(L, st) : launch2_t<false, 64, 16>(L, st);
+    case 0: GC_S2(64, 8, 256);
+    case 1: GC_S2(32, 16, 256);
+    case 2: GC_S2(64, 16, 256);
+    case 3: GC_S2(64, 16, 512);
+    case 4: GC_S2(64, 16, 1024);
+    case 5: GC_S2(32, 16, 512);
+    case 6: GC_S2(64, 32, 1024);
     default: return hipErrorInvalidValue;
   }
+#undef GC_S2
 }
 
 hipError_t launch_max(const DevLayer &L, float *zmax_raw, hipStream_t st) {
@@ -1902,9 +1931,12 @@ hipError_t launch_division_check(unsigned long long seed, int per_lane, unsigned
 }
 
 hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st) {
-  const size_t n = bytes / 16;
-  hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, st, static_cast<const float4 *>(src),
-                     static_cast<float4 *>(dst), n);
+  const size_t n = bytes / 16, per = 256 * (size_t)kCopyUnroll;
+  const size_t blocks = (n + per - 1) / per;
+  if (blocks == 0) return hipSuccess;
+  if (blocks > 0x7fffffffu) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_copy, dim3((unsigned)blocks), dim3(256), 0, st,
+                     static_cast<const CopyVec *>(src), static_cast<CopyVec *>(dst), n);
   return hipGetLastError();
 }
 

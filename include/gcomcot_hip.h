@@ -345,8 +345,10 @@ int gcomcot_output_flush_(long long *nfiles);
 int gcomcot_autotune_(int *reps, int *variant, int *blocking, float *ms_per_step);
 
 /* Two root steps per launch (temporal blocking) in gcomcot_run_steps_ for
- * a linear outermost layer without nested layers: *mode 0 = off, 1/2/3 =
- * on with 64x8 / 32x16 / 64x16 tiles (default 1).  Results are bitwise
+ * a linear outermost layer without nested layers: *mode 0 = off, 1..7 =
+ * on with tiles of 64x8, 32x16, 64x16 (256 threads), 64x16 (512 and 1024
+ * threads), 32x16 (512 threads), 64x32 (1024 threads) (default 1; the
+ * autotuner picks among them).  Results are bitwise
  * those of single steps; afterwards level 2 holds the state two steps
  * before level 1 when the last launch was a two-step one.  Returns 0 if
  * valid. */

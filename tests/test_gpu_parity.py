@@ -103,8 +103,10 @@ LINEAR = [CASES[0], CASES[1],
           ("ragged-linear-77x61", S.custom(laycord=0, laygov=0, nx=77, ny=61, cx=0.45), 90)]
 
 
-@pytest.mark.parametrize("mode", [0, 1, 2, 3], ids=["single", "2step-64x8", "2step-32x16",
-                                                    "2step-64x16"])
+@pytest.mark.parametrize("mode", range(8), ids=["single", "2step-64x8", "2step-32x16",
+                                                 "2step-64x16", "2step-64x16-512",
+                                                 "2step-64x16-1024", "2step-32x16-512",
+                                                 "2step-64x32-1024"])
 def test_linear_layers_one_or_two_steps_per_launch(mode):
     """Linear single layers run two root steps per launch by default; every
     tile shape of that kernel and the single-step kernel must give the
@@ -138,7 +140,8 @@ def test_autotune_mid_run_keeps_the_state(name, sc, steps):
     try:
         dev.run(37)
         v, b, ms = dev.autotune(reps=2)
-        assert ms > 0 and v >= 0 and 0 <= b <= 3
+        assert ms > 0 and v >= 0
+        assert 0 <= b <= 7
         if sc.layers[0].laygov != 0:
             assert b == 0  # two steps per launch are for linear layers only
         dev.run(40)
