@@ -28,8 +28,12 @@ run through the batch engine, against the same scenarios run one at a
 time through the whole-run device loop, and against the reference's
 per-step call sequence (drop-in entry points, synchronous per call)
 timed on a sample of steps.  Serving (serve.*): Poisson request streams
-drawn from the same generator plus 10 % 4096x3072 nonlinear cases, through
-the continuous-batching loop; p50/p99 time-to-forecast and scenarios/s.
+(200 requests at each rate) drawn from the same generator plus 10 %
+4096x3072 nonlinear cases with fresh sources, through the
+continuous-batching loop; p50/p99 time-to-forecast and scenarios/s over the
+whole stream and in steady state, and whether the offered rate is
+sustained.  The FP64 oracle's error on sampled served cases is part of the
+CPU-baseline leg (cpu_baseline.served_fp64).
 
 Prints one JSON line.
 """
@@ -68,8 +72,12 @@ def parse():
     ap.add_argument("--dropin-sample", type=int, default=100,
                     help="steps per scenario of the drop-in call-sequence sample")
     ap.add_argument("--serve", type=int, default=1, help="run the serving streams (0 = skip)")
-    ap.add_argument("--serve-requests", type=int, default=24)
-    ap.add_argument("--serve-rates", default="2,5,10,20")
+    ap.add_argument("--serve-requests", type=int, default=200)
+    ap.add_argument("--serve-rates", default="5,10,20")
+    ap.add_argument("--serve-pool", type=int, default=120,
+                    help="distinct small cases the serving streams draw from")
+    ap.add_argument("--serve-fp64", type=int, default=2,
+                    help="served small cases per rate checked against the FP64 oracle")
     ap.add_argument("--outputs", type=int, default=1,
                     help="time the device loop with reference-format output files (0 = skip)")
     ap.add_argument("--outputs-steps", type=int, default=1000)
@@ -337,31 +345,72 @@ def bench_batch(a):
 
 
 def bench_serve(a):
+    """Continuous serving (config 4): Poisson streams of --serve-requests
+    requests at each of --serve-rates, drawn from the config-3 generator
+    with a 10% share of 4096x3072 nonlinear cases.  Small cases come from
+    one pool shared by the rates; each rate gets fresh large cases (new
+    sources on the large world).  Returns the per-rate summaries and, for
+    the checker leg, the forecasts kept for --serve-fp64 of the smallest
+    small cases of each rate."""
     from gcomcot import scenarios as S
     from gcomcot.serve import Request, Server, poisson_arrivals, summarize
 
     rates = [float(r) for r in a.serve_rates.split(",") if r]
     nreq = a.serve_requests
     nbig = max(1, round(0.1 * nreq))
-    log(f"serve: generating {nbig} large nonlinear cases")
-    rng = np.random.default_rng(20261202 + 4000)
-    bigs = [S.nonlinear_layer(case=10 + i, steps=int(rng.integers(1000, 4001))) for i in range(nbig)]
-    out = {}
+    log(f"serve: generating a pool of {a.serve_pool} small cases")
+    pool = S.scenario_mix(n=a.serve_pool, case=10)
+    out, kept = {}, []
     for ri, rate in enumerate(rates):
+        log(f"serve: {nbig} fresh large cases for {rate:g} req/s")
+        bigs = S.nonlinear_layer_sources(nbig, case=10, seed=ri)
+        rng = np.random.default_rng(300 + ri)
+        small = [pool[k] for k in rng.integers(0, len(pool), nreq - nbig)]
+        cases = small + bigs
+        order = rng.permutation(len(cases))
+        t = poisson_arrivals(rate, len(cases), seed=100 + ri)
+        reqs = [Request(i, t[i], cases[j]) for i, j in enumerate(order)]
+        cost = {r.rid: r.scenario.cell_updates_per_step() * r.steps for r in reqs}
+        smalls = [r for r, j in zip(reqs, order) if j < len(small)]
+        check = {r.rid for r in sorted(smalls, key=lambda r: cost[r.rid])[:a.serve_fp64]}
         log(f"serve: {nreq} requests at {rate:g} req/s")
-        small = S.scenario_mix(n=nreq - nbig, case=10 + ri)
-        pool = small + bigs
-        order = np.random.default_rng(ri).permutation(len(pool))
-        t = poisson_arrivals(rate, len(pool), seed=100 + ri)
-        reqs = [Request(i, t[i], pool[j]) for i, j in enumerate(order)]
-        srv = Server(chunk=128)
+        srv = Server(chunk=128, keep_results=check)
         done = srv.serve(reqs)
-        s = summarize(done)
         srv.close()
+        s = summarize(done)
         s["rate_req_per_s"] = rate
+        s["large_cases"] = nbig
         s["peak_running"] = srv.peak_running
-        out[f"{rate:g}"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in s.items()}
-    return out
+        out[f"{rate:g}"] = _rounded(s)
+        kept += [(f"{rate:g}", r) for r in done if r.result is not None]
+    return out, kept
+
+
+def _rounded(d):
+    if isinstance(d, dict):
+        return {k: _rounded(v) for k, v in d.items()}
+    return round(d, 4) if isinstance(d, float) else d
+
+
+def cpu_served_accuracy(kept):
+    """Part of the CPU-baseline leg: the FP64 oracle's wet maximum and
+    arrival field for sampled served cases against the forecasts the
+    serving loop returned (FP32 on the device); the oracle is the checker."""
+    from oracle import bind
+    rows = []
+    for rate, r in kept:
+        sc = r.scenario
+        o64 = bind.Oracle(sc, 64, pow_mode=0)
+        o64.step(r.steps)
+        om = o64.maxima(1)
+        g = r.result
+        zd = np.abs(np.asarray(g["zmax_wet"], np.float64) - np.asarray(om["zmax_wet"], np.float64))
+        rows.append({"rate": rate, "case": sc.name, "cells": int(sc.cell_updates_per_step()),
+                     "steps": int(r.steps), "zmax_max_abs_m": float(zd.max()),
+                     "zmax_peak_m": float(np.max(om["zmax_wet"])),
+                     "arrival_cells": int(np.count_nonzero(om["arrival"] >= 0)),
+                     "arrival_mismatch": int(np.count_nonzero(g["arrival"] != om["arrival"]))})
+    return rows
 
 
 def cpu_accuracy():
@@ -414,7 +463,10 @@ def main():
     outputs = bench_outputs(a) if a.outputs else None
     source = bench_source(a) if a.source else None
     batch = bench_batch(a) if a.batch else None
-    serve = bench_serve(a) if a.serve else None
+    serve, kept = bench_serve(a) if a.serve else (None, [])
+    if cpu is not None and kept:
+        log(f"cpu baseline: FP64 oracle on {len(kept)} served cases")
+        cpu["served_fp64"] = cpu_served_accuracy(kept)
     copy = copy_probe(1024)
 
     line = {

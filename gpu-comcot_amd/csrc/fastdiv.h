@@ -60,15 +60,24 @@ GC_FD_HD bool div_inrange_applies(float a, float b, float q) {
 }
 
 #if defined(__HIPCC__) || defined(__HIP__)
-__device__ __forceinline__ float div_inrange(float a, float b) {
-  float y = __builtin_amdgcn_rcpf(b);
+// div_inrange split in two: the refined reciprocal of b (three operations
+// that depend on b alone), and the quotient from it.  div_by(a, b,
+// recip_refined(b)) is div_inrange(a, b) bit for bit, so quotients with one
+// divisor can share the first part.
+__device__ __forceinline__ float recip_refined(float b) {
+  const float y = __builtin_amdgcn_rcpf(b);
   const float e = fmaf(-b, y, 1.0f);
-  y = fmaf(e, y, y);
+  return fmaf(e, y, y);
+}
+__device__ __forceinline__ float div_by(float a, float b, float y) {
   float q = a * y;
   const float r0 = fmaf(-b, q, a);
   q = fmaf(r0, y, q);
   const float r1 = fmaf(-b, q, a);
   return fmaf(r1, y, q);
+}
+__device__ __forceinline__ float div_inrange(float a, float b) {
+  return div_by(a, b, recip_refined(b));
 }
 
 // sqrt(x) as the correctly rounded square root sequence computes it for

@@ -297,12 +297,19 @@ __device__ __forceinline__ float dq2_at(const DevLayer &L, const A &s, int i, in
 // convection terms where the surface rises faster than 0.08*sqrt(g*D),
 // else 1 (multiplying by 1.0f leaves every value, NaN and infinities
 // included, unchanged, so one factor per cell serves both directions).
+// rdt = recip_refined(DT): |Z2 - Z1| / DT is div_by with it, the IEEE
+// quotient wherever its value decides anything.  A wet cell's DZDT_F is at
+// least 0.08*sqrt(g*GX) (8e-4 m/s), so a numerator that leaves div_inrange's
+// range gives a quotient far below it either way; on a dry cell (DZ = 0,
+// DZDT_F = 0) only the quotient's sign matters, and both forms give +0 for
+// 0 and a positive value otherwise, whose size REF = DZDT/0 = +inf ignores.
 template <class A>
-__device__ __forceinline__ float breaking_coef(const DevLayer &L, const A &s, int i, int j) {
+__device__ __forceinline__ float breaking_coef(const DevLayer &L, const A &s, int i, int j,
+                                               float rdt) {
   float c = sqrt_inrange(kGrav * s.DZ(i, j));
   float dzdt_f = 0.08f * c;
   float dzdt_i = 0.65f * c;
-  float dzdt = fabsf(s.Z2(i, j) - s.Z1(i, j)) / L.dt;
+  float dzdt = div_by(fabsf(s.Z2(i, j) - s.Z1(i, j)), L.dt, rdt);
   if (dzdt > dzdt_f) {
     float ref = (dzdt - dzdt_f) / (dzdt_i - dzdt_f);
     return gcdet::expf_det(-(1.0f * ref), *s.tab);
@@ -377,7 +384,8 @@ __device__ float nl_x(const DevLayer &L, const A &s, int i, int j, float brk) {
     const int ia = pn ? ip1 : im1;
     const float da = dp1_at<IN>(L, s, ia, j);
     const bool degx = pn ? (da < kGx || s.DZ(ip1, j) < kGx) : (da < kGx || s.DZ(i, j) < kGx);
-    const float ax = div_inrange(p * p, d);
+    const float yd = recip_refined(d);  // shared by the two quotients over d
+    const float ax = div_by(p * p, d, yd);
     const float pa = s.M(ia, j);
     const float bx = div_inrange(pa * pa, da);
     const float advx = fx * (degx ? (pn ? -ax : ax) : (pn ? bx - ax : ax - bx));
@@ -385,7 +393,7 @@ __device__ float nl_x(const DevLayer &L, const A &s, int i, int j, float brk) {
     const int jn = qn ? jp1 : jm1, jf = qn ? j : j - 2;
     const float dn = dp1_at<IN>(L, s, i, jn);
     const bool degy = s.DZ(i, jn) < kGx || s.DZ(ip1, jn) < kGx || dn < kGx;
-    const float ay = div_inrange(p * xqq, d);
+    const float ay = div_by(p * xqq, d, yd);
     const float xqe = 0.25f * (s.N(i, jn) + s.N(ip1, jn) + s.N(i, jf) + s.N(ip1, jf));
     const float by = div_inrange(s.M(i, jn) * xqe, dn);
     float advy = fy * (degy ? (qn ? -ay : ay) : (qn ? by - ay : ay - by));
@@ -465,7 +473,8 @@ __device__ float nl_y(const DevLayer &L, const A &s, int i, int j, float brk) {
     const int ja = qn ? jp1 : jm1;
     const float da = dq1_at<IN>(L, s, i, ja);
     const bool degy = qn ? (da < kGx || s.DZ(i, jp1) < kGx) : (da < kGx || s.DZ(i, j) < kGx);
-    const float ay = div_inrange(q * q, d);
+    const float yd = recip_refined(d);  // shared by the two quotients over d
+    const float ay = div_by(q * q, d, yd);
     const float qa = s.N(i, ja);
     const float by = div_inrange(qa * qa, da);
     float advy = fy * (degy ? (qn ? -ay : ay) : (qn ? by - ay : ay - by));
@@ -474,7 +483,7 @@ __device__ float nl_y(const DevLayer &L, const A &s, int i, int j, float brk) {
     const float dn = dq1_at<IN>(L, s, pn ? ip1 : i - 1, j);
     const bool degx = pn ? (s.DZ(ip1, j) < kGx || s.DZ(ip1, jp1) < kGx || dn < kGx)
                          : (s.DZ(im1, j) < kGx || s.DZ(im1, jp1) < kGx || dn < kGx);
-    const float ax = div_inrange(q * xpp, d);
+    const float ax = div_by(q * xpp, d, yd);
     const float xpe = 0.25f * (s.M(in, j) + s.M(in, jp1) + s.M(ifar, j) + s.M(ifar, jp1));
     const float bx = div_inrange(s.N(in, j) * xpe, dq1_at<IN>(L, s, in, j));
     float advx = fx * (degx ? (pn ? -ax : ax) : (pn ? bx - ax : ax - bx));
@@ -586,6 +595,7 @@ __device__ __forceinline__ void tile_body(const DevLayer &L, int t, int ti0, int
     step = L.ctl[L.ctl_in] + 1;
     if (L.advance && t == 0 && tid == 0) L.ctl[1 - L.ctl_in] = step;
   }
+  const float rdt = recip_refined(L.dt);
 #pragma unroll 1
   for (int idx = tid; idx < TX * TY; idx += kThreads) {
     const int i = ti0 + idx % TX, j = tj0 + idx / TX;
@@ -612,7 +622,7 @@ __device__ __forceinline__ void tile_body(const DevLayer &L, int t, int ti0, int
             IN || (r ? (i <= nx - 1 && j <= ny - 1) : (i >= 2 && i <= nx - 1 && j >= 2));
         const bool my =
             IN || (r ? (i <= nx - 1 && j <= ny - 1) : (i >= 2 && j >= 2 && j <= ny - 1));
-        const float brk = L.linchk == 1 ? breaking_coef(L, s, i, j) : 1.0f;
+        const float brk = L.linchk == 1 ? breaking_coef(L, s, i, j, rdt) : 1.0f;
         m2v = mx ? nl_x<SPH, IN>(L, s, i, j, brk) : 0.0f;
         n2v = my ? nl_y<SPH, IN>(L, s, i, j, brk) : 0.0f;
       } else {
@@ -948,6 +958,7 @@ __global__ void __launch_bounds__(kThreads)
     if (L.advance && t == 0 && lane == 0) L.ctl[1 - L.ctl_in] = step;
   }
   const bool own = lane >= 2 && lane <= kStreamW - 3 && col_in;
+  const float rdt = recip_refined(L.dt);
   for (int j = j0; j <= j1; ++j) {
     store_row(j + 2, ph, pm, pn);
     zq2 = pz;
@@ -970,7 +981,7 @@ __global__ void __launch_bounds__(kThreads)
       if (NL) {
         const bool mx = rt ? (i <= nx - 1 && j <= ny - 1) : (i >= 2 && i <= nx - 1 && j >= 2);
         const bool my = rt ? (i <= nx - 1 && j <= ny - 1) : (i >= 2 && j >= 2 && j <= ny - 1);
-        const float brk = L.linchk == 1 ? breaking_coef(L, r, i, j) : 1.0f;
+        const float brk = L.linchk == 1 ? breaking_coef(L, r, i, j, rdt) : 1.0f;
         m2v = mx ? nl_x<SPH, false>(L, r, i, j, brk) : 0.0f;
         n2v = my ? nl_y<SPH, false>(L, r, i, j, brk) : 0.0f;
       } else {

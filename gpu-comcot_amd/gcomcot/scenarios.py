@@ -222,6 +222,36 @@ def nonlinear_layer(nx=4096, ny=3072, dx=0.5, case=0, steps=5000, embayments=3):
     return Scenario(f"nonlinear-{nx}x{ny}", [L], steps, {"seed": seed})
 
 
+def nonlinear_layer_sources(n, nx=4096, ny=3072, dx=0.5, case=0, steps=(1000, 4000),
+                            embayments=3, seed=0):
+    """n nonlinear layers of nonlinear_layer's world (one bathymetry, built
+    once) with fresh dipole sources: peak 0.5-3 m, sigma 20-80 km, separation
+    1-3 sigma, random position and strike; root step counts uniform in
+    `steps`.  The layers share the depth array (read only)."""
+    lon0, lat0 = 140.0, 10.0
+    lon1 = lon0 + (nx - 1) * dx / 60.0
+    lat1 = lat0 + (ny - 1) * dx / 60.0
+    wseed = SEED0 + 1000 + case
+    w = make_world(wseed, lon0, lat0, lon1, lat1, 8, embayments)
+    src0 = source(w, 1, 2.0, 40.0, 40.0, 80.0, 0.30, 0.5, 0.35)
+    base = _sph_root(w, nx, ny, dx, 1, src0, 0, 0.013, lon0, lat0)
+    lon = lon0 + np.arange(nx, dtype=np.float64) * dx / 60.0
+    lat = lat0 + np.arange(ny, dtype=np.float64) * dx / 60.0
+    LON, LAT = np.meshgrid(lon, lat)
+    rng = np.random.default_rng(SEED0 + 5000 + 97 * case + seed)
+    out = []
+    for k in range(n):
+        sig = float(rng.uniform(20.0, 80.0))
+        src = source(w, 1, float(rng.uniform(0.5, 3.0)), sig, sig, sig * float(rng.uniform(1.0, 3.0)),
+                     float(rng.uniform(0.25, 0.8)), float(rng.uniform(0.2, 0.8)),
+                     float(rng.uniform(0.0, math.pi)))
+        L = LayerSpec(**{**base.__dict__, "z0": eta(src, LON, LAT, base.h)})
+        out.append(Scenario(f"nonlinear-{nx}x{ny}-src{seed}.{k}", [L],
+                            int(rng.integers(steps[0], steps[1] + 1)),
+                            {"seed": wseed, "source": k}))
+    return out
+
+
 def two_level_nest(pnx=2048, pny=1536, dx=2.0, footprint=450, rel_size=4, rel_time=4, case=0,
             steps=3000, child_laygov=1, corner_i=30):
     """Spherical linear parent + Cartesian nonlinear child (2048x1536 + 1801x1801 by

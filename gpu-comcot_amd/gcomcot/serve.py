@@ -44,6 +44,8 @@ class Server:
 
     def __init__(self, budget_bytes=250 * 10 ** 9, chunk=128, keep_results=False,
                  arrival_threshold=0.01):
+        """keep_results: True keeps every forecast on its Request (result),
+        a set of request ids keeps those only."""
         self.budget = int(budget_bytes)
         self.chunk = int(chunk)
         self.keep = keep_results
@@ -84,7 +86,7 @@ class Server:
                 r = running.pop(sid)
                 mx = self.batch.maxima(sid, 1)  # the forecast: max-eta and arrival fields
                 r.done = clock() - t0
-                if self.keep:
+                if self.keep is True or (self.keep and r.rid in self.keep):
                     r.result = mx
                 self.batch.remove(sid)
                 used -= r.footprint
@@ -94,10 +96,36 @@ class Server:
         self.batch.close()
 
 
-def summarize(reqs):
-    ttf = np.array([r.done - r.arrival for r in reqs])
-    span = max(r.done for r in reqs) - min(r.arrival for r in reqs)
-    return {"requests": len(reqs), "p50_ttf_s": float(np.percentile(ttf, 50)),
-            "p99_ttf_s": float(np.percentile(ttf, 99)), "max_ttf_s": float(ttf.max()),
-            "scenarios_per_s": len(reqs) / span if span > 0 else float("inf"),
-            "span_s": span}
+def summarize(reqs, warmup=0.1):
+    """Time-to-forecast statistics of a served stream.  Over all requests:
+    p50/p99/max and scenarios/s from the first arrival to the last
+    completion.  Steady state: the requests after the first `warmup` share
+    of arrivals (the queue filling up) and before the arrivals stop; the
+    offered rate is sustained when completions keep pace with arrivals
+    (throughput over that window >= 90% of the offered rate) and the
+    time-to-forecast does not climb (median of the window's last quarter
+    below twice that of its first quarter)."""
+    rs = sorted(reqs, key=lambda r: r.arrival)
+    ttf = np.array([r.done - r.arrival for r in rs])
+    arr = np.array([r.arrival for r in rs])
+    done = np.array([r.done for r in rs])
+    span = done.max() - arr.min()
+    n = len(rs)
+    k0 = int(round(warmup * n))
+    win = slice(k0, n)
+    offered = (n - 1) / (arr[-1] - arr[0]) if n > 1 and arr[-1] > arr[0] else float("inf")
+    t_lo, t_hi = arr[k0], arr[-1]
+    completed = np.count_nonzero((done >= t_lo) & (done <= t_hi))
+    through = completed / (t_hi - t_lo) if t_hi > t_lo else float("inf")
+    w = ttf[win]
+    q = max(1, len(w) // 4)
+    climb = float(np.median(w[-q:]) / max(np.median(w[:q]), 1e-9))
+    out = {"requests": n, "p50_ttf_s": float(np.percentile(ttf, 50)),
+           "p99_ttf_s": float(np.percentile(ttf, 99)), "max_ttf_s": float(ttf.max()),
+           "scenarios_per_s": n / span if span > 0 else float("inf"), "span_s": float(span),
+           "offered_req_per_s": float(offered),
+           "steady": {"requests": int(len(w)), "p50_ttf_s": float(np.percentile(w, 50)),
+                      "p99_ttf_s": float(np.percentile(w, 99)),
+                      "completions_per_s": float(through), "ttf_climb": climb,
+                      "sustained": bool(through >= 0.9 * offered and climb < 2.0)}}
+    return out
