@@ -69,6 +69,8 @@ def parse():
                     help="also time the linear single layer and the two-level nest (0 = skip)")
     ap.add_argument("--batch", type=int, default=1, help="run the 64-scenario batch (0 = skip)")
     ap.add_argument("--batch-cases", type=int, default=64)
+    ap.add_argument("--batch-two-step", type=int, default=1,
+                    help="single-layer linear roots take two root steps per launch (0 = one)")
     ap.add_argument("--dropin-sample", type=int, default=100,
                     help="steps per scenario of the drop-in call-sequence sample")
     ap.add_argument("--serve", type=int, default=1, help="run the serving streams (0 = skip)")
@@ -90,6 +92,10 @@ def parse():
                     help="batch engine replays HIP graphs (0: enqueue each step)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run the streaming-copy probe on MIB MiB (counter calibration)")
+    ap.add_argument("--linear-probe", default="", metavar="NXxNY",
+                    help="only run the linear single layer of that size (profiling)")
+    ap.add_argument("--blocking", type=int, default=5,
+                    help="with --linear-probe: 0 one step per launch, 1-7 two-step shapes")
     return ap.parse_args()
 
 
@@ -100,6 +106,27 @@ def copy_probe(mib, reps=20):
     check(lib().gcomcot_hbm_copy_(ip(mib), ip(reps), ctypes.byref(ms)), "copy probe")
     gbs = 2 * (mib << 20) / (ms.value * 1e-3) / 1e9
     return {"mib": mib, "ms_per_copy": round(ms.value, 5), "read_plus_write_GBs": round(gbs, 1)}
+
+
+def linear_probe(a):
+    """The linear single layer (C0's generator at the given size), --steps
+    root steps through the device loop without graphs (one launch per step
+    or per two steps, as --blocking says), for kernel traces and counters."""
+    from gcomcot import DeviceRun, linear_layer
+    from gcomcot._lib import ip, lib
+    nx, ny = (int(v) for v in a.linear_probe.lower().split("x"))
+    sc = linear_layer(nx=nx, ny=ny, steps=a.steps)
+    lib().gcomcot_set_temporal_blocking_(ip(a.blocking))
+    run = DeviceRun(sc)
+    run.run(2, graph=False)
+    t0 = time.perf_counter()
+    run.run(a.steps, graph=False)
+    run.state(1)
+    wall = time.perf_counter() - t0
+    run.close()
+    return {"workload": sc.name, "blocking": a.blocking, "steps": a.steps,
+            "us_per_step": round(wall / a.steps * 1e6, 2),
+            "gcups": round(nx * ny * a.steps / wall / 1e9, 2)}
 
 
 def log(msg):
@@ -299,7 +326,7 @@ def bench_batch(a):
     upd = [sum(L.nx * L.ny * (L.rel_time if L.id > 1 else 1) for L in c.layers) * c.steps
            for c in cs]
     total = float(sum(upd))
-    b = Batch(graphs=bool(a.graphs))
+    b = Batch(graphs=bool(a.graphs), two_step=bool(a.batch_two_step))
     for c in cs:
         b.add(c)
     log("batch: running the batch engine")
@@ -450,6 +477,9 @@ def main():
     if a.copy_probe:
         print(json.dumps({"copy_probe": copy_probe(a.copy_probe)}))
         return
+    if a.linear_probe:
+        print(json.dumps({"linear_probe": linear_probe(a)}))
+        return
     from gcomcot import nonlinear_layer
 
     sc, cells, gcups, wall, kern_ms, achieved, traffic, arrived, ms_total, tile = bench_c1(a)
@@ -487,7 +517,9 @@ def main():
                    "cells": cells, "tile": list(tile), "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic},
+                     "traffic": traffic,
+                     "practical_peak": copy["read_plus_write_GBs"],
+                     "frac_of_practical": round(achieved / copy["read_plus_write_GBs"], 4)},
         "cpu_baseline": cpu,
         "kernel_ms_mean": round(kern_ms, 5),
         "event_ms_total": round(ms_total, 3),

@@ -59,6 +59,9 @@ struct Scenario {
 };
 
 enum GroupType { kStepGroup = 0, kJnqGroup = 1, kJnzGroup = 2, kScEdgeGroup = 3, kScJnzGroup = 4 };
+// Step-group mode beside kFused/kMassOnly/kMomentOnly: two root steps of
+// single-layer linear roots in one launch (k_batch2_grid).
+constexpr int kTwoStep = 3;
 
 struct Group {
   int type = kStepGroup;
@@ -76,12 +79,15 @@ struct Group {
 };
 
 struct Plan {
-  std::vector<Group> step[2];
-  int nsteps = 0;          // 1 or 2 root steps
+  // root steps of the plan: 2, or 4 when single-layer linear roots take one
+  // two-step launch per pair of root steps (their level parity comes back
+  // after two such launches)
+  std::vector<Group> step[4];
+  int nsteps = 0;
   char *arena = nullptr;
   int *heads = nullptr;
-  int heads_n[2] = {0, 0};
-  int heads_off[2] = {0, 0};
+  int heads_n[4] = {0, 0, 0, 0};
+  int heads_off[4] = {0, 0, 0, 0};
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   long long cells_per_step = 0;
@@ -96,6 +102,8 @@ struct Engine {
   float arr_thr = 0.01f;
   bool autotune = true;
   bool graphs = true;      // replay plans from HIP graphs (else enqueue each step)
+  bool two_step = true;    // single-layer linear roots: two root steps per launch
+  int step2_shape = 4;     // their tile shape (launch_step2's shapes)
   int forced_shape = -1;
   std::map<long long, int> tuned;  // (kind, size class) -> shape
   Plan plan;
@@ -192,10 +200,16 @@ void flush_steps(std::vector<Pending> &pend, std::vector<Group> &out) {
     long long cells = 0;
     for (auto &d : p.layers) cells += (long long)d.nx * d.ny;
     // fused launches: the tuned form and tile; continuity/momentum halves
-    // of nested roots: one workgroup per 64 x 8 tile
-    g.shape = p.mode == kFused ? shape_for(p.nl, p.sph, cells) : batch_num_shapes() / 2;
+    // of nested roots: one workgroup per 64 x 8 tile; two-step launches:
+    // the engine's two-step shape
     int tx = 64, ty = 8;
-    batch_shape(g.shape, &tx, &ty);
+    if (p.mode == kTwoStep) {
+      g.shape = g_eng.step2_shape;
+      step2_shape(g.shape, &tx, &ty);
+    } else {
+      g.shape = p.mode == kFused ? shape_for(p.nl, p.sph, cells) : batch_num_shapes() / 2;
+      batch_shape(g.shape, &tx, &ty);
+    }
     for (auto &d : p.layers) add_layer_items(g, d, tx, ty);
     g.recs = p.recs;
     split_queues(g);
@@ -215,16 +229,31 @@ void pend_add(std::vector<Pending> &pend, int mode, const Tree &T, const LayerRe
   pend.push_back(Pending{mode, L.nl, L.sph, {d}, {{&T, &L}}});
 }
 
+// A single-layer linear root (no sponge: it acts between steps) can take
+// two root steps per launch.
+bool two_step_root(const Tree &T) {
+  const LayerRec &R = T.layers[1];
+  return R.used && R.children.empty() && !R.nl && !R.sponge && T.ngauges == 0;
+}
+
 // Launch list of one root step of every scenario in `act`; rotates the
-// host-side time levels as the launches will.
-void emit_step(const std::vector<Scenario *> &act, std::vector<Group> &out) {
+// host-side time levels as the launches will.  pair = -1: every layer one
+// step per launch; 0 / 1: the first / second root step of a pair in which
+// the two-step roots take one launch (emitted with the first, nothing with
+// the second).
+void emit_step(const std::vector<Scenario *> &act, std::vector<Group> &out, int pair = -1) {
   std::vector<Pending> pend;
   size_t maxsib = 0;
   for (Scenario *s : act) {
     Tree &T = s->tree;
     LayerRec &R = *T.layer(1);
-    if (R.children.empty()) pend_add(pend, kFused, T, R, dev_view(T, R, true, true));
-    else pend_add(pend, kMassOnly, T, R, dev_view(T, R, false, false));
+    if (pair >= 0 && two_step_root(T)) {
+      if (pair == 0) pend_add(pend, kTwoStep, T, R, dev_view(T, R, true, true));
+    } else if (R.children.empty()) {
+      pend_add(pend, kFused, T, R, dev_view(T, R, true, true));
+    } else {
+      pend_add(pend, kMassOnly, T, R, dev_view(T, R, false, false));
+    }
     maxsib = std::max(maxsib, R.children.size());
   }
   flush_steps(pend, out);
@@ -283,6 +312,7 @@ void emit_step(const std::vector<Scenario *> &act, std::vector<Group> &out) {
   }
   flush_steps(pend, out);
   for (Scenario *s : act) {
+    if (pair == 1 && two_step_root(s->tree)) continue;  // rotated with the pair's first step
     LayerRec &R = *s->tree.layer(1);
     R.cur = 1 - R.cur;
     s->tree.ctl_in = 1 - s->tree.ctl_in;
@@ -365,7 +395,9 @@ int enqueue_plan_step(const Plan &P, int r) {
   hipError_t e = launch_fill_i32(P.heads + P.heads_off[r], P.heads_n[r], 0, st);
   if (e != hipSuccess) return fail(210, "batch head reset: %s", hipGetErrorString(e));
   for (const auto &g : P.step[r]) {
-    if (g.type == kStepGroup) {
+    if (g.type == kStepGroup && g.mode == kTwoStep) {
+      e = launch_batch2(step_launch(P, g), g.sph != 0, g.shape, st);
+    } else if (g.type == kStepGroup) {
       e = launch_batch(step_launch(P, g), g.mode, g.nl != 0, g.sph != 0, g.shape, st);
     } else if (g.type == kScEdgeGroup || g.type == kScJnzGroup) {
       for (size_t q = 0; q < g.scs.size() && e == hipSuccess; ++q)
@@ -452,35 +484,49 @@ int tune(const std::vector<Scenario *> &act) {
   return 0;
 }
 
+bool any_two_step(const std::vector<Scenario *> &act) {
+  if (!g_eng.two_step) return false;
+  for (Scenario *s : act)
+    if (two_step_root(s->tree)) return true;
+  return false;
+}
+
 int build_plan(const std::vector<Scenario *> &act) {
   invalidate();
   if (g_eng.autotune && g_eng.forced_shape < 0)
     if (int r = tune(act)) return r;
   Plan &P = g_eng.plan;
-  P.nsteps = 2;
-  emit_step(act, P.step[0]);
-  emit_step(act, P.step[1]);  // time levels are back where they started
+  // two steps, or two pairs when two-step roots are present: either way
+  // every time level is back where it started at the end of the plan
+  const bool pairs = any_two_step(act);
+  P.nsteps = pairs ? 4 : 2;
+  for (int k = 0; k < P.nsteps; ++k) emit_step(act, P.step[k], pairs ? k % 2 : -1);
   if (trace_on())
-    std::fprintf(stderr, "batch: plan of %zu + %zu launches\n", P.step[0].size(),
-                 P.step[1].size());
-  for (auto &g : P.step[0])
-    if (g.type == kStepGroup) P.cells_per_step += g.cells;
+    std::fprintf(stderr, "batch: plan of %d steps, %zu + %zu launches\n", P.nsteps,
+                 P.step[0].size(), P.step[1].size());
+  long long cells = 0;
+  for (int k = 0; k < P.nsteps; ++k)
+    for (auto &g : P.step[k])
+      if (g.type == kStepGroup) cells += g.cells * (g.mode == kTwoStep ? 2 : 1);
+  P.cells_per_step = cells / P.nsteps;
   if (int r = upload_plan(P, true)) return r;
-  if (!g_eng.graphs || P.step[0].size() + 1 > (size_t)kGraphMaxLaunches) {
+  size_t per_step = 0;
+  for (int k = 0; k < P.nsteps; ++k) per_step = std::max(per_step, P.step[k].size());
+  if (!g_eng.graphs || per_step + 1 > (size_t)kGraphMaxLaunches) {
     // a very deep sub-stepping (huge REL_TIME) is enqueued step by step:
     // graphs of that many nodes are not worth building
     g_eng.plan_valid = true;
     ++g_eng.plans_built;
     return 0;
   }
-  for (int r = 0; r < 2; ++r)
+  for (int r = 0; r < P.nsteps; ++r)
     for (const auto &g : P.step[r])
-      if (g.type == kStepGroup)
+      if (g.type == kStepGroup && g.mode != kTwoStep)
         HIPCHK(launch_batch(step_launch(P, g), g.mode, g.nl != 0, g.sph != 0, g.shape,
                             g_eng.st, true));
   HIPCHK(hipStreamBeginCapture(g_eng.st, hipStreamCaptureModeThreadLocal));
-  int r = enqueue_plan_step(P, 0);
-  if (!r) r = enqueue_plan_step(P, 1);
+  int r = 0;
+  for (int k = 0; k < P.nsteps && !r; ++k) r = enqueue_plan_step(P, k);
   hipError_t ce = hipStreamEndCapture(g_eng.st, &P.graph);
   if (r) return r;
   HIPCHK(ce);
@@ -507,21 +553,47 @@ int run(long long n) {
                    g_eng.plan_valid ? "kept" : "rebuilt");
     if (!g_eng.plan_valid)
       if (int r = build_plan(act)) return r;
-    for (long long k = 0; k + 1 < m; k += 2) {
-      if (g_eng.plan.exec) {
-        HIPCHK(hipGraphLaunch(g_eng.plan.exec, g_eng.st));
+    const Plan &P = g_eng.plan;
+    const int ns = P.nsteps;
+    for (long long k = 0; k + ns <= m; k += ns) {
+      if (P.exec) {
+        HIPCHK(hipGraphLaunch(P.exec, g_eng.st));
         ++g_eng.graph_replays;
       } else {
-        if (int r = enqueue_plan_step(g_eng.plan, 0)) return r;
-        if (int r = enqueue_plan_step(g_eng.plan, 1)) return r;
+        for (int q = 0; q < ns; ++q)
+          if (int r = enqueue_plan_step(P, q)) return r;
       }
     }
-    if (m % 2) {
-      // one more step from the plan's starting time levels, then the
-      // levels differ from the captured ones: rebuild next time
-      if (int r = enqueue_plan_step(g_eng.plan, 0)) return r;
+    long long rest = m % ns;
+    if (rest) {
+      // the rest of the epoch from the plan's starting time levels: its
+      // first pair of steps (two steps), or its first step (one); the
+      // levels then differ from the captured ones, so rebuild next time
       std::vector<Group> scratch;
-      emit_step(act, scratch);  // rotate host-side levels as that step did
+      if (rest >= 2) {
+        if (int r = enqueue_plan_step(P, 0)) return r;
+        if (int r = enqueue_plan_step(P, 1)) return r;
+        emit_step(act, scratch, ns == 4 ? 0 : -1);  // rotate host-side levels as they did
+        emit_step(act, scratch, ns == 4 ? 1 : -1);
+        rest -= 2;
+      }
+      if (rest == 1) {
+        if (ns == 2 && scratch.empty()) {
+          if (int r = enqueue_plan_step(P, 0)) return r;
+          emit_step(act, scratch);
+        } else {
+          // one step of every layer, two-step roots included: a plan of its own
+          HIPCHK(hipStreamSynchronize(g_eng.st));
+          Plan one;
+          one.nsteps = 1;
+          emit_step(act, one.step[0]);
+          int r = upload_plan(one);
+          if (!r) r = enqueue_plan_step(one, 0);
+          if (!r) r = hipStreamSynchronize(g_eng.st) == hipSuccess ? 0 : fail(211, "batch tail step");
+          free_plan(one);
+          if (r) return r;
+        }
+      }
       HIPCHK(hipStreamSynchronize(g_eng.st));
       invalidate();
     }
@@ -750,6 +822,15 @@ int gcomcot_batch_set_tuning_(int *autotune, int *shape) {
 
 int gcomcot_batch_set_graphs_(int *on) {
   g_eng.graphs = *on != 0;
+  invalidate();
+  return 0;
+}
+
+int gcomcot_batch_set_two_step_(int *on, int *shape) {
+  if (shape && (*shape < 0 || *shape >= kStep2Shapes))
+    return fail(44, "batch: two-step shape %d out of range", *shape);
+  g_eng.two_step = *on != 0;
+  if (shape) g_eng.step2_shape = *shape;
   invalidate();
   return 0;
 }

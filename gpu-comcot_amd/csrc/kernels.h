@@ -52,6 +52,8 @@ void tile_variant_shape(int v, int *tx, int *ty);
 // wet maximum and arrival take both steps and the counter advances by 2.
 hipError_t launch_step2(const DevLayer &L, bool sph, int shape, hipStream_t st);
 constexpr int kStep2Shapes = 7;
+// tile of two-step shape `shape`
+void step2_shape(int shape, int *tx, int *ty);
 
 // Reference-semantics maximum (no wet test) plus the wet max/arrival.
 hipError_t launch_max(const DevLayer &L, float *zmax_raw, hipStream_t st);
@@ -160,6 +162,9 @@ int batch_num_shapes();
 void batch_shape(int k, int *tx, int *ty);
 hipError_t launch_batch(const BatchLaunch &B, int mode, bool nl, bool sph, int shape,
                         hipStream_t st, bool prepare = false);
+// Two root steps of single-layer linear layers, one workgroup per item;
+// items tile each layer with step2_shape(shape) tiles.
+hipError_t launch_batch2(const BatchLaunch &B, bool sph, int shape, hipStream_t st);
 // JNQ/NEWQ and JNZ of several nests in one launch: nest n owns blocks
 // [start[n], start[n+1]) of 128 threads; *_block_count gives each one's share.
 int jnq_block_count(const DevNest &N);

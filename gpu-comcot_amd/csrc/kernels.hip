@@ -1702,6 +1702,31 @@ __global__ void __launch_bounds__(kThreads)
 }
 
 
+// Two root steps of single-layer linear scenarios, one workgroup per
+// (layer, tile) item of a TX x TY tiling, NT threads: the tile body of
+// k_step2 on each item's layer.
+template <bool SPH, int TX, int TY, int NT>
+__global__ void __launch_bounds__(NT)
+    k_batch2_grid(const BatchDesc *__restrict__ descs, const BatchItem *__restrict__ items,
+                  int nitems, int per_xcd) {
+  constexpr int HW = 4;
+  __shared__ float sm[7 * (TX + 2 * HW) * (TY + 2 * HW) + kNumCoef * (TY + 2 * HW)];
+  __shared__ gcdet::Tables stab;
+  const int it = tile_of_block(per_xcd);
+  if (it >= nitems) return;
+  if (!SPH) stage_tables<NT>(stab, threadIdx.x);
+  const BatchItem w = items[it];
+  const int slot = __builtin_amdgcn_readfirstlane(w.slot);
+  const int t = __builtin_amdgcn_readfirstlane(w.tile);
+  const DevLayer L = descs[slot].L;
+  const int tiles_x = descs[slot].tiles_x;
+  const int ti0 = (t % tiles_x) * TX + 1, tj0 = (t / tiles_x) * TY + 1;
+  const bool interior = ti0 - HW > kSpan + 2 && ti0 + TX + HW < L.nx - kSpan &&
+                        tj0 - HW > kSpan + 2 && tj0 + TY + HW < L.ny - kSpan;
+  if (interior) tile_body2<SPH, TX, TY, true, NT>(L, t, ti0, tj0, sm, &stab);
+  else tile_body2<SPH, TX, TY, false, NT>(L, t, ti0, tj0, sm, &stab);
+}
+
 template <int MODE, bool NL, bool SPH, int TX, int TY>
 hipError_t launch_t(const DevLayer &L, hipStream_t st) {
   const int tx = (L.nx + TX - 1) / TX, ty = (L.ny + TY - 1) / TY;
@@ -1769,17 +1794,46 @@ hipError_t launch_step(const DevLayer &L, int mode, bool nl, bool sph, int varia
   }
 }
 
+// Two-step shapes: tile TX x TY and threads NT (the same table for the
+// single-tree and the batch launches).
+#define GC_STEP2_SHAPES(X) \
+  X(0, 64, 8, 256) X(1, 32, 16, 256) X(2, 64, 16, 256) X(3, 64, 16, 512) X(4, 64, 16, 1024) \
+  X(5, 32, 16, 512) X(6, 64, 32, 1024)
+
 hipError_t launch_step2(const DevLayer &L, bool sph, int shape, hipStream_t st) {
-#define GC_S2(TX, TY, NT) \
-  return sph ? launch2_t<true, TX, TY, NT>(L, st) : launch2_t<false, TX, TY, NT>(L, st)
+#define GC_S2(K, TX, TY, NT) \
+  case K: return sph ? launch2_t<true, TX, TY, NT>(L, st) : launch2_t<false, TX, TY, NT>(L, st);
   switch (shape) {
-    case 0: GC_S2(64, 8, 256);
-    case 1: GC_S2(32, 16, 256);
-    case 2: GC_S2(64, 16, 256);
-    case 3: GC_S2(64, 16, 512);
-    case 4: GC_S2(64, 16, 1024);
-    case 5: GC_S2(32, 16, 512);
-    case 6: GC_S2(64, 32, 1024);
+    GC_STEP2_SHAPES(GC_S2)
+    default: return hipErrorInvalidValue;
+  }
+#undef GC_S2
+}
+
+void step2_shape(int shape, int *tx, int *ty) {
+#define GC_S2(K, TX, TY, NT) \
+  case K: *tx = TX; *ty = TY; return;
+  switch (shape) {
+    GC_STEP2_SHAPES(GC_S2)
+    default: *tx = 64; *ty = 16; return;
+  }
+#undef GC_S2
+}
+
+template <bool SPH, int TX, int TY, int NT>
+hipError_t launch_batch2_t(const BatchLaunch &B, hipStream_t st) {
+  const int per = (B.nitems + 7) / 8;
+  hipLaunchKernelGGL((k_batch2_grid<SPH, TX, TY, NT>), dim3(8 * per), dim3(NT), 0, st, B.descs,
+                     B.items, B.nitems, per);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch2(const BatchLaunch &B, bool sph, int shape, hipStream_t st) {
+  if (B.nitems <= 0) return hipSuccess;
+#define GC_S2(K, TX, TY, NT) \
+  case K: return sph ? launch_batch2_t<true, TX, TY, NT>(B, st) : launch_batch2_t<false, TX, TY, NT>(B, st);
+  switch (shape) {
+    GC_STEP2_SHAPES(GC_S2)
     default: return hipErrorInvalidValue;
   }
 #undef GC_S2

@@ -50,7 +50,11 @@ def _is_nl(s):
 class Batch:
     """The library's batch engine (one per process)."""
 
-    def __init__(self, arrival_threshold=0.01, autotune=True, shape=-1, graphs=True):
+    def __init__(self, arrival_threshold=0.01, autotune=True, shape=-1, graphs=True,
+                 two_step=True, two_step_shape=4):
+        """two_step: scenarios whose only layer is linear advance two root
+        steps per launch (tile shape two_step_shape, the library's two-step
+        shapes 0-6)."""
         L = lib()
         _setup(L)
         L.gcomcot_set_error_mode_(ip(0))
@@ -58,6 +62,8 @@ class Batch:
         L.gcomcot_batch_set_arrival_threshold_(fp(arrival_threshold))
         check(L.gcomcot_batch_set_tuning_(ip(1 if autotune else 0), ip(shape)), "batch tuning")
         check(L.gcomcot_batch_set_graphs_(ip(1 if graphs else 0)), "batch graphs")
+        check(L.gcomcot_batch_set_two_step_(ip(1 if two_step else 0), ip(two_step_shape)),
+              "batch two-step")
         self.specs = {}
 
     def add(self, sc, steps=None):

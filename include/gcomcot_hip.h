@@ -454,6 +454,12 @@ int gcomcot_batch_fault_deform_(int *sid, const float *params, int *fswitch);
 /* *on == 0: enqueue every root step instead of replaying HIP graphs (for
  * tools that cannot follow graph launches) */
 int gcomcot_batch_set_graphs_(int *on);
+/* *on != 0 (the default): scenarios whose only layer is linear advance two
+ * root steps per launch (the two-step kernel of gcomcot_set_temporal_
+ * blocking_, tile shape *shape = its mode - 1, default 4 = 64x16 tiles of
+ * 1024 threads); plans then cover four root steps.  Results are bitwise
+ * those of one step per launch.  shape may be NULL (kept). */
+int gcomcot_batch_set_two_step_(int *on, int *shape);
 int gcomcot_batch_info_(long long *launches, long long *graph_replays, long long *plans,
                         long long *cells_per_step, int *tuned_count);
 

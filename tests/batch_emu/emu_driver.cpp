@@ -102,6 +102,8 @@ int main(int argc, char **argv) {
   int shape = U(-1, 5);
   if (gcomcot_batch_reset_()) return 3;
   if (gcomcot_batch_set_tuning_(&on, &shape)) return 3;
+  int two = U(0, 3) ? 1 : 0, shape2 = U(0, 6);  // two root steps per launch for linear roots
+  if (gcomcot_batch_set_two_step_(&two, &shape2)) return 3;
   std::map<int, Case> live;
   int checked = 0;
   for (int r = 0; r < rounds; ++r) {
@@ -141,7 +143,8 @@ int main(int argc, char **argv) {
   long long l = 0, g = 0, p = 0, cps = 0;
   int tuned = 0;
   gcomcot_batch_info_(&l, &g, &p, &cps, &tuned);
-  std::printf("seed %u: %d scenarios verified, shape %d\n", seed, checked, shape);
+  std::printf("seed %u: %d scenarios verified, shape %d, two-step %d (shape %d)\n", seed,
+              checked, shape, two, shape2);
   (void)zero;
   return 0;
 }

@@ -158,6 +158,51 @@ void run_multi(void *a) {
   }
 }
 
+// two root steps per launch (k_batch2_grid): every plane moves by 2, the
+// maximum takes both steps and the counter advances by 2
+const int kStep2Tiles[][2] = {{64, 8}, {32, 16}, {64, 16}, {64, 16}, {64, 16}, {32, 16}, {64, 32}};
+struct Batch2Arg {
+  BatchLaunch B;
+  int shape;
+};
+void run_batch2(void *a) {
+  const Batch2Arg *x = (const Batch2Arg *)a;
+  const BatchLaunch &B = x->B;
+  const int TX = kStep2Tiles[x->shape][0], TY = kStep2Tiles[x->shape][1];
+  int nslot = 0;
+  for (int it = 0; it < B.nitems; ++it) nslot = std::max(nslot, B.items[it].slot + 1);
+  std::vector<std::vector<unsigned char>> cov(nslot);
+  for (int s = 0; s < nslot; ++s) {
+    const BatchDesc &bd = B.descs[s];
+    if (bd.tiles_x != (bd.L.nx + TX - 1) / TX || bd.ntiles != bd.tiles_x * ((bd.L.ny + TY - 1) / TY))
+      die("two-step descriptor tiling");
+    if (!bd.L.is_root) die("two-step launch of a nested layer");
+    cov[s].assign((size_t)bd.L.nx * bd.L.ny, 0);
+  }
+  for (int it = 0; it < B.nitems; ++it) {
+    const BatchItem w = B.items[it];
+    const BatchDesc &bd = B.descs[w.slot];
+    const DevLayer &d = bd.L;
+    const int i0 = (w.tile % bd.tiles_x) * TX, j0 = (w.tile / bd.tiles_x) * TY;
+    for (int j = j0; j < j0 + TY && j < d.ny; ++j)
+      for (int i = i0; i < i0 + TX && i < d.nx; ++i) {
+        const size_t c = (size_t)j * d.nx + i;
+        cov[w.slot][c]++;
+        d.z2[c] = d.z1[c] + 2.0f;
+        d.m2[c] = d.m1[c] + 2.0f;
+        d.n2[c] = d.n1[c] + 2.0f;
+        if (d.zmax) d.zmax[c] += 2.0f;
+      }
+    if (d.zmax && d.advance && w.tile == 0) {
+      d.ctl[1 - d.ctl_in] = d.ctl[d.ctl_in] + 2;
+      d.arrival[0] = d.ctl[1 - d.ctl_in];
+    }
+  }
+  for (auto &c : cov)
+    for (auto v : c)
+      if (v != 1) die("two-step launch coverage");
+}
+
 struct FillArg {
   int *p;
   int n, v;
@@ -213,6 +258,17 @@ hipError_t launch_batch(const BatchLaunch &B, int mode, bool, bool, int shape, h
   if (shape < 0 || shape >= 6) return hipErrorInvalidValue;
   BatchArg a{B, mode, shape};
   emu_submit(st, run_batch, &a, sizeof(a));
+  return hipSuccess;
+}
+void step2_shape(int shape, int *tx, int *ty) {
+  *tx = kStep2Tiles[shape][0];
+  *ty = kStep2Tiles[shape][1];
+}
+hipError_t launch_batch2(const BatchLaunch &B, bool, int shape, hipStream_t st) {
+  if (B.nitems <= 0) return hipSuccess;
+  if (shape < 0 || shape >= kStep2Shapes) return hipErrorInvalidValue;
+  Batch2Arg a{B, shape};
+  emu_submit(st, run_batch2, &a, sizeof(a));
   return hipSuccess;
 }
 int jnq_block_count(const DevNest &N) {

@@ -41,10 +41,13 @@ def _check_vs_oracle(b, sid, sc, orc, tag):
         assert np.array_equal(gm["arrival"], om["arrival"]), f"{tag} layer {L.id} arrival"
 
 
-def test_mixed_batch_matches_oracle():
+@pytest.mark.parametrize("two_step,graphs", [(True, True), (False, True), (True, False)],
+                         ids=["two-step", "one-step", "two-step-no-graphs"])
+def test_mixed_batch_matches_oracle(two_step, graphs):
     steps = [s + (i % 3) for i, (_, _, s) in enumerate(CASES)]  # odd and even counts
-    print(f"[gpu] batch of {len(CASES)} scenarios, steps {steps}", flush=True)
-    b = Batch()
+    print(f"[gpu] batch of {len(CASES)} scenarios, steps {steps}, two_step={two_step}, "
+          f"graphs={graphs}", flush=True)
+    b = Batch(two_step=two_step, graphs=graphs)
     sids = [b.add(sc, n) for (_, sc, _), n in zip(CASES, steps)]
     assert b.run(10 ** 6) == len(CASES)
     for (name, sc, _), sid, n in zip(CASES, sids, steps):
@@ -52,7 +55,23 @@ def test_mixed_batch_matches_oracle():
         assert b.progress(sid) == (n, n)
         _check_vs_oracle(b, sid, sc, _oracle(sc, n), name)
     info = b.info()
-    assert info["plans"] >= 2 and info["graph_replays"] > 0, info
+    assert info["plans"] >= 2 and (info["graph_replays"] > 0) == graphs, info
+    b.close()
+
+
+@pytest.mark.parametrize("shape", range(7))
+def test_two_step_shapes_in_the_batch(shape):
+    """Every two-step tile shape on single-layer linear roots, with step
+    counts that leave 1, 2 and 3 steps after the last four-step plan, beside
+    a nest and a nonlinear layer that take one step per launch."""
+    picks = [CASES[0], CASES[1], CASES[2], CASES[5]]
+    steps = [41, 42, 43, 44]
+    print(f"[gpu] batch two-step shape {shape}", flush=True)
+    b = Batch(two_step_shape=shape)
+    sids = [b.add(sc, n) for (_, sc, _), n in zip(picks, steps)]
+    b.run(10 ** 6)
+    for (name, sc, _), sid, n in zip(picks, sids, steps):
+        _check_vs_oracle(b, sid, sc, _oracle(sc, n), f"{name} two-step shape {shape}")
     b.close()
 
 
