@@ -102,10 +102,13 @@ struct Engine {
   float arr_thr = 0.01f;
   bool autotune = true;
   bool graphs = true;      // replay plans from HIP graphs (else enqueue each step)
-  bool two_step = true;    // single-layer linear roots: two root steps per launch
-  int step2_shape = 4;     // their tile shape (launch_step2's shapes)
+  bool two_step = true;    // single-layer linear roots may take two root steps per launch
+  int step2_shape = 4;     // their tile shape (launch_step2's shapes); -1: one step per launch
+  int forced_step2 = -1;   // a shape set by the caller (no tuning of it)
   int forced_shape = -1;
   std::map<long long, int> tuned;  // (kind, size class) -> shape
+  std::map<long long, float> tuned_ms;  // (kind, size class) -> its time per launch
+  std::map<long long, int> tuned2;  // size class of the two-step roots -> shape or -1
   Plan plan;
   bool plan_valid = false;
   char *arena = nullptr;   // descriptor/item memory reused by successive plans
@@ -478,6 +481,73 @@ int tune(const std::vector<Scenario *> &act) {
       if (tot < best_ms) { best_ms = tot; best = sh; }
     }
     g_eng.tuned[key] = best;
+    g_eng.tuned_ms[key] = best_ms / 3;
+  }
+  // Two root steps per launch for the single-layer linear roots: every
+  // two-step shape is timed on them (dry, as above) and the best is kept
+  // if it beats two launches of their tuned one-step shape.
+  if (g_eng.two_step && g_eng.forced_step2 < 0) {
+    std::vector<std::pair<const Tree *, const LayerRec *>> roots;
+    long long cells = 0;
+    int sph = -1;
+    bool mixed = false;
+    for (Scenario *s : act)
+      if (two_step_root(s->tree)) {
+        const LayerRec &R = *s->tree.layer(1);
+        roots.emplace_back(&s->tree, &R);
+        cells += R.nxy();
+        if (sph >= 0 && sph != R.sph) mixed = true;
+        sph = R.sph;
+      }
+    const long long key2 = (cells >= (4LL << 20) ? 1 : 0) + (mixed ? 4 : 2 * sph);
+    if (!roots.empty() && !g_eng.tuned2.count(key2)) {
+      // the one-step time of the same roots: their fused group's tuned shape
+      float one = 0;
+      for (auto &g : probe)
+        if (g.type == kStepGroup && g.mode == kFused && g.nl == 0 && !g.recs.empty() &&
+            g.recs[0].second->id == 1)  // the first flush: outermost layers only
+          one += g_eng.tuned_ms.count(kind_key(0, g.sph, g.cells))
+                     ? g_eng.tuned_ms[kind_key(0, g.sph, g.cells)]
+                     : 0.0f;
+      int best = -1;
+      float best_ms = 2 * one;  // two one-step launches
+      for (int sh = 0; sh < kStep2Shapes && one > 0; ++sh) {
+        int tx, ty;
+        step2_shape(sh, &tx, &ty);
+        Plan T;
+        T.nsteps = 1;
+        for (int part = 0; part < 2; ++part) {  // the spherical and the Cartesian ones
+          Group d;
+          d.type = kStepGroup;
+          d.mode = kTwoStep;
+          d.nl = 0;
+          d.sph = 1 - part;
+          d.shape = sh;
+          for (auto &tr : roots)
+            if (tr.second->sph == d.sph)
+              add_layer_items(d, dev_view(*tr.first, *tr.second, false, false), tx, ty);
+          if (d.descs.empty()) continue;
+          split_queues(d);
+          T.step[0].push_back(std::move(d));
+        }
+        if (int r = upload_plan(T)) return r;
+        float tot = 0;
+        for (int rep = 0; rep < 4; ++rep) {
+          if (rep == 1) HIPCHK(hipEventRecord(e0, g_eng.st));
+          if (int r = enqueue_plan_step(T, 0)) return r;
+        }
+        HIPCHK(hipEventRecord(e1, g_eng.st));
+        HIPCHK(hipEventSynchronize(e1));
+        HIPCHK(hipEventElapsedTime(&tot, e0, e1));
+        free_plan(T);
+        if (tot / 3 < best_ms) { best_ms = tot / 3; best = sh; }
+      }
+      g_eng.tuned2[key2] = best;
+      if (trace_on())
+        std::fprintf(stderr, "batch: two-step roots, %zu layers: one-step %.3f ms x 2, "
+                     "two-step shape %d %.3f ms\n", roots.size(), one, best, best_ms);
+    }
+    if (!roots.empty()) g_eng.step2_shape = g_eng.tuned2[key2];
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
@@ -485,7 +555,7 @@ int tune(const std::vector<Scenario *> &act) {
 }
 
 bool any_two_step(const std::vector<Scenario *> &act) {
-  if (!g_eng.two_step) return false;
+  if (!g_eng.two_step || g_eng.step2_shape < 0) return false;
   for (Scenario *s : act)
     if (two_step_root(s->tree)) return true;
   return false;
@@ -830,7 +900,8 @@ int gcomcot_batch_set_two_step_(int *on, int *shape) {
   if (shape && (*shape < 0 || *shape >= kStep2Shapes))
     return fail(44, "batch: two-step shape %d out of range", *shape);
   g_eng.two_step = *on != 0;
-  if (shape) g_eng.step2_shape = *shape;
+  g_eng.forced_step2 = shape ? *shape : -1;
+  g_eng.step2_shape = shape ? *shape : 4;
   invalidate();
   return 0;
 }

@@ -51,10 +51,11 @@ class Batch:
     """The library's batch engine (one per process)."""
 
     def __init__(self, arrival_threshold=0.01, autotune=True, shape=-1, graphs=True,
-                 two_step=True, two_step_shape=4):
-        """two_step: scenarios whose only layer is linear advance two root
-        steps per launch (tile shape two_step_shape, the library's two-step
-        shapes 0-6)."""
+                 two_step=True, two_step_shape=None):
+        """two_step: scenarios whose only layer is linear may advance two
+        root steps per launch, in tile shape two_step_shape (the library's
+        two-step shapes 0-6), or, when None, in the shape the autotuner
+        finds fastest if that beats two one-step launches."""
         L = lib()
         _setup(L)
         L.gcomcot_set_error_mode_(ip(0))
@@ -62,7 +63,8 @@ class Batch:
         L.gcomcot_batch_set_arrival_threshold_(fp(arrival_threshold))
         check(L.gcomcot_batch_set_tuning_(ip(1 if autotune else 0), ip(shape)), "batch tuning")
         check(L.gcomcot_batch_set_graphs_(ip(1 if graphs else 0)), "batch graphs")
-        check(L.gcomcot_batch_set_two_step_(ip(1 if two_step else 0), ip(two_step_shape)),
+        check(L.gcomcot_batch_set_two_step_(ip(1 if two_step else 0),
+                                            None if two_step_shape is None else ip(two_step_shape)),
               "batch two-step")
         self.specs = {}
 

@@ -454,11 +454,13 @@ int gcomcot_batch_fault_deform_(int *sid, const float *params, int *fswitch);
 /* *on == 0: enqueue every root step instead of replaying HIP graphs (for
  * tools that cannot follow graph launches) */
 int gcomcot_batch_set_graphs_(int *on);
-/* *on != 0 (the default): scenarios whose only layer is linear advance two
- * root steps per launch (the two-step kernel of gcomcot_set_temporal_
- * blocking_, tile shape *shape = its mode - 1, default 4 = 64x16 tiles of
- * 1024 threads); plans then cover four root steps.  Results are bitwise
- * those of one step per launch.  shape may be NULL (kept). */
+/* *on != 0 (the default): scenarios whose only layer is linear may advance
+ * two root steps per launch (the two-step kernel of gcomcot_set_temporal_
+ * blocking_, tile shape *shape = its mode - 1); plans then cover four root
+ * steps.  shape NULL (the default): the autotuner times every two-step
+ * shape on those layers and takes the fastest if it beats two one-step
+ * launches, else they keep one step per launch.  Results are bitwise those
+ * of one step per launch either way. */
 int gcomcot_batch_set_two_step_(int *on, int *shape);
 int gcomcot_batch_info_(long long *launches, long long *graph_replays, long long *plans,
                         long long *cells_per_step, int *tuned_count);

@@ -102,8 +102,15 @@ int main(int argc, char **argv) {
   int shape = U(-1, 5);
   if (gcomcot_batch_reset_()) return 3;
   if (gcomcot_batch_set_tuning_(&on, &shape)) return 3;
-  int two = U(0, 3) ? 1 : 0, shape2 = U(0, 6);  // two root steps per launch for linear roots
-  if (gcomcot_batch_set_two_step_(&two, &shape2)) return 3;
+  // two root steps per launch for linear roots: a given shape, or the tuner's choice
+  int two = U(0, 3) ? 1 : 0, shape2 = U(-1, 6);
+  if (std::getenv("EMU_TUNE_TWO")) {  // the tuner picks the one- and two-step shapes
+    two = 1;
+    shape2 = -1;
+    shape = -1;
+    if (gcomcot_batch_set_tuning_(&on, &shape)) return 3;
+  }
+  if (gcomcot_batch_set_two_step_(&two, shape2 >= 0 ? &shape2 : nullptr)) return 3;
   std::map<int, Case> live;
   int checked = 0;
   for (int r = 0; r < rounds; ++r) {

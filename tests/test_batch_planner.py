@@ -55,3 +55,15 @@ def test_large_layers(driver, seed):
     r = subprocess.run([driver, str(seed), "4"], capture_output=True, text=True, env=env,
                        timeout=600)
     assert r.returncode == 0, r.stderr[-2000:]
+
+
+@pytest.mark.parametrize("seed", [31, 32, 33])
+def test_tuned_two_step_choice(driver, seed):
+    """The tuner decides per plan whether linear roots take two steps per
+    launch (emulated timings vary, so both decisions occur)."""
+    env = dict(os.environ, EMU_TUNE_TWO="1", ASAN_OPTIONS="detect_leaks=1")
+    r = subprocess.run([driver, str(seed), "25"], capture_output=True, text=True, env=env,
+                       timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "two-step 1 (shape -1)" in r.stdout
+
