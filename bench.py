@@ -94,7 +94,7 @@ def parse():
                     help="only run the streaming-copy probe on MIB MiB (counter calibration)")
     ap.add_argument("--linear-probe", default="", metavar="NXxNY",
                     help="only run the linear single layer of that size (profiling)")
-    ap.add_argument("--blocking", type=int, default=5,
+    ap.add_argument("--blocking", type=int, default=4,
                     help="with --linear-probe: 0 one step per launch, 1-7 two-step shapes")
     return ap.parse_args()
 
@@ -111,17 +111,19 @@ def copy_probe(mib, reps=20):
 def linear_probe(a):
     """The linear single layer (C0's generator at the given size), --steps
     root steps through the device loop without graphs (one launch per step
-    or per two steps, as --blocking says), for kernel traces and counters."""
+    or per two steps, as --blocking says; tile variant --tile-variant when
+    set), for kernel traces and counters."""
     from gcomcot import DeviceRun, linear_layer
     from gcomcot._lib import ip, lib
     nx, ny = (int(v) for v in a.linear_probe.lower().split("x"))
     sc = linear_layer(nx=nx, ny=ny, steps=a.steps)
     lib().gcomcot_set_temporal_blocking_(ip(a.blocking))
+    if a.tile_variant >= 0:
+        lib().gcomcot_set_tile_variant_(ip(a.tile_variant))
     run = DeviceRun(sc)
     run.run(2, graph=False)
     t0 = time.perf_counter()
-    run.run(a.steps, graph=False)
-    run.state(1)
+    run.run(a.steps, graph=False)  # returns once the device is idle
     wall = time.perf_counter() - t0
     run.close()
     return {"workload": sc.name, "blocking": a.blocking, "steps": a.steps,
