@@ -76,6 +76,9 @@ struct Group {
   int head = 0;            // index of this launch's 8 heads
   long long cells = 0;     // cell updates of a step group
   std::vector<std::pair<const Tree *, const LayerRec *>> recs;  // layer of each desc
+  // the layers are single-layer scenarios: nothing else in the plan reads or
+  // writes them, so the group may run beside the nested scenarios' chain
+  bool side = false;
 };
 
 struct Plan {
@@ -97,6 +100,11 @@ struct Plan {
 struct Engine {
   bool init = false;
   hipStream_t st = nullptr;
+  // single-layer scenarios' launches run on st2, forked from st at the
+  // start of a plan and joined back at its end (both at default priority)
+  hipStream_t st2 = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+  bool concurrent = true;
   std::map<int, std::unique_ptr<Scenario>> scen;
   int next_sid = 1;
   float arr_thr = 0.01f;
@@ -124,6 +132,9 @@ int eng_init() {
   if (g_eng.init) return 0;
   HIPCHK(hipSetDevice(0));
   HIPCHK(hipStreamCreateWithFlags(&g_eng.st, hipStreamNonBlocking));
+  HIPCHK(hipStreamCreateWithFlags(&g_eng.st2, hipStreamNonBlocking));
+  HIPCHK(hipEventCreateWithFlags(&g_eng.fork, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&g_eng.join, hipEventDisableTiming));
   g_eng.init = true;
   return 0;
 }
@@ -138,6 +149,7 @@ void free_plan(Plan &P) {
 void invalidate() {
   if (g_eng.plan.arena || g_eng.plan.exec) {
     (void)hipStreamSynchronize(g_eng.st);
+    (void)hipStreamSynchronize(g_eng.st2);
     free_plan(g_eng.plan);
   }
   g_eng.plan_valid = false;
@@ -259,7 +271,10 @@ void emit_step(const std::vector<Scenario *> &act, std::vector<Group> &out, int 
     }
     maxsib = std::max(maxsib, R.children.size());
   }
+  const size_t first = out.size();
   flush_steps(pend, out);
+  for (size_t q = first; q < out.size(); ++q)  // fused and two-step roots without children
+    out[q].side = out[q].mode == kFused || out[q].mode == kTwoStep;
   for (size_t sib = 0; sib < maxsib; ++sib) {
     int maxrt = 0;
     for (Scenario *s : act) {
@@ -393,11 +408,9 @@ BatchLaunch step_launch(const Plan &P, const Group &g) {
   return B;
 }
 
-int enqueue_plan_step(const Plan &P, int r) {
-  hipStream_t st = g_eng.st;
-  hipError_t e = launch_fill_i32(P.heads + P.heads_off[r], P.heads_n[r], 0, st);
-  if (e != hipSuccess) return fail(210, "batch head reset: %s", hipGetErrorString(e));
-  for (const auto &g : P.step[r]) {
+int enqueue_group(const Plan &P, const Group &g, hipStream_t st) {
+  hipError_t e = hipSuccess;
+  {
     if (g.type == kStepGroup && g.mode == kTwoStep) {
       e = launch_batch2(step_launch(P, g), g.sph != 0, g.shape, st);
     } else if (g.type == kStepGroup) {
@@ -415,6 +428,41 @@ int enqueue_plan_step(const Plan &P, int r) {
     }
     if (e != hipSuccess) return fail(211, "batch launch: %s", hipGetErrorString(e));
     ++g_eng.launches;
+  }
+  return 0;
+}
+
+// Root steps r0..r1-1 of a plan.  The queue heads of all of them are
+// cleared first (each group has heads of its own).  Groups of single-layer
+// scenarios then run on the second stream, all their steps in order, while
+// the nested scenarios' chain (continuity halves, JNQ, children, JNZ,
+// momentum halves) runs on the first; the second stream joins back at the
+// end, so work enqueued on the first stream afterwards (and its
+// synchronisations) follows both.
+int enqueue_plan(const Plan &P, int r0, int r1) {
+  hipStream_t st = g_eng.st;
+  const int h0 = P.heads_off[r0], h1 = P.heads_off[r1 - 1] + P.heads_n[r1 - 1];
+  hipError_t e = launch_fill_i32(P.heads + h0, h1 - h0, 0, st);
+  if (e != hipSuccess) return fail(210, "batch head reset: %s", hipGetErrorString(e));
+  bool side = false, chain = false;
+  for (int r = r0; r < r1; ++r)
+    for (const auto &g : P.step[r]) (g.side ? side : chain) = true;
+  const bool fork = g_eng.concurrent && side && chain;
+  if (fork) {
+    HIPCHK(hipEventRecord(g_eng.fork, st));
+    HIPCHK(hipStreamWaitEvent(g_eng.st2, g_eng.fork, 0));
+    for (int r = r0; r < r1; ++r)
+      for (const auto &g : P.step[r])
+        if (g.side)
+          if (int q = enqueue_group(P, g, g_eng.st2)) return q;
+  }
+  for (int r = r0; r < r1; ++r)
+    for (const auto &g : P.step[r])
+      if (!fork || !g.side)
+        if (int q = enqueue_group(P, g, st)) return q;
+  if (fork) {
+    HIPCHK(hipEventRecord(g_eng.join, g_eng.st2));
+    HIPCHK(hipStreamWaitEvent(st, g_eng.join, 0));
   }
   return 0;
 }
@@ -472,7 +520,7 @@ int tune(const std::vector<Scenario *> &act) {
       float tot = 0;
       for (int rep = 0; rep < 4; ++rep) {
         if (rep == 1) HIPCHK(hipEventRecord(e0, g_eng.st));
-        if (int r = enqueue_plan_step(T, 0)) return r;
+        if (int r = enqueue_plan(T, 0, 1)) return r;
       }
       HIPCHK(hipEventRecord(e1, g_eng.st));
       HIPCHK(hipEventSynchronize(e1));
@@ -534,7 +582,7 @@ int tune(const std::vector<Scenario *> &act) {
         float tot = 0;
         for (int rep = 0; rep < 4; ++rep) {
           if (rep == 1) HIPCHK(hipEventRecord(e0, g_eng.st));
-          if (int r = enqueue_plan_step(T, 0)) return r;
+          if (int r = enqueue_plan(T, 0, 1)) return r;
         }
         HIPCHK(hipEventRecord(e1, g_eng.st));
         HIPCHK(hipEventSynchronize(e1));
@@ -595,8 +643,7 @@ int build_plan(const std::vector<Scenario *> &act) {
         HIPCHK(launch_batch(step_launch(P, g), g.mode, g.nl != 0, g.sph != 0, g.shape,
                             g_eng.st, true));
   HIPCHK(hipStreamBeginCapture(g_eng.st, hipStreamCaptureModeThreadLocal));
-  int r = 0;
-  for (int k = 0; k < P.nsteps && !r; ++k) r = enqueue_plan_step(P, k);
+  int r = enqueue_plan(P, 0, P.nsteps);
   hipError_t ce = hipStreamEndCapture(g_eng.st, &P.graph);
   if (r) return r;
   HIPCHK(ce);
@@ -630,8 +677,7 @@ int run(long long n) {
         HIPCHK(hipGraphLaunch(P.exec, g_eng.st));
         ++g_eng.graph_replays;
       } else {
-        for (int q = 0; q < ns; ++q)
-          if (int r = enqueue_plan_step(P, q)) return r;
+        if (int r = enqueue_plan(P, 0, ns)) return r;
       }
     }
     long long rest = m % ns;
@@ -641,15 +687,14 @@ int run(long long n) {
       // levels then differ from the captured ones, so rebuild next time
       std::vector<Group> scratch;
       if (rest >= 2) {
-        if (int r = enqueue_plan_step(P, 0)) return r;
-        if (int r = enqueue_plan_step(P, 1)) return r;
+        if (int r = enqueue_plan(P, 0, 2)) return r;
         emit_step(act, scratch, ns == 4 ? 0 : -1);  // rotate host-side levels as they did
         emit_step(act, scratch, ns == 4 ? 1 : -1);
         rest -= 2;
       }
       if (rest == 1) {
         if (ns == 2 && scratch.empty()) {
-          if (int r = enqueue_plan_step(P, 0)) return r;
+          if (int r = enqueue_plan(P, 0, 1)) return r;
           emit_step(act, scratch);
         } else {
           // one step of every layer, two-step roots included: a plan of its own
@@ -658,7 +703,7 @@ int run(long long n) {
           one.nsteps = 1;
           emit_step(act, one.step[0]);
           int r = upload_plan(one);
-          if (!r) r = enqueue_plan_step(one, 0);
+          if (!r) r = enqueue_plan(one, 0, 1);
           if (!r) r = hipStreamSynchronize(g_eng.st) == hipSuccess ? 0 : fail(211, "batch tail step");
           free_plan(one);
           if (r) return r;
@@ -684,6 +729,7 @@ int gcomcot_batch_reset_(void) {
   if (int e = eng_init()) return e;
   invalidate();
   (void)hipStreamSynchronize(g_eng.st);
+  (void)hipStreamSynchronize(g_eng.st2);
   for (auto &kv : g_eng.scen) tree_free(kv.second->tree);
   g_eng.scen.clear();
   if (g_eng.arena) (void)hipFree(g_eng.arena);
@@ -902,6 +948,13 @@ int gcomcot_batch_set_two_step_(int *on, int *shape) {
   g_eng.two_step = *on != 0;
   g_eng.forced_step2 = shape ? *shape : -1;
   g_eng.step2_shape = shape ? *shape : 4;
+  invalidate();
+  return 0;
+}
+
+int gcomcot_batch_set_streams_(int *n) {
+  if (*n < 1 || *n > 2) return fail(45, "batch: %d streams (1 or 2)", *n);
+  g_eng.concurrent = *n == 2;
   invalidate();
   return 0;
 }

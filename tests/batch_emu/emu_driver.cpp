@@ -150,8 +150,9 @@ int main(int argc, char **argv) {
   long long l = 0, g = 0, p = 0, cps = 0;
   int tuned = 0;
   gcomcot_batch_info_(&l, &g, &p, &cps, &tuned);
-  std::printf("seed %u: %d scenarios verified, shape %d, two-step %d (shape %d)\n", seed,
-              checked, shape, two, shape2);
+  std::printf("seed %u: %d scenarios verified, shape %d, two-step %d (shape %d), "
+              "%lld cross-stream waits\n", seed, checked, shape, two, shape2,
+              emu_cross_stream_waits());
   (void)zero;
   return 0;
 }

@@ -15,13 +15,16 @@ struct Work {
 };
 struct EmuGraph {
   std::vector<Work> work;
+  int forked = 0;  // streams that joined the capture and have not been joined back
 };
 struct EmuStream {
   bool capturing = false;
+  bool joined = false;  // capturing because it waited on an event of a capture
   EmuGraph *graph = nullptr;
 };
 struct EmuEvent {
-  int dummy = 0;
+  EmuStream *src = nullptr;  // stream of the last record
+  EmuGraph *graph = nullptr; // capture it was recorded in, if any
 };
 
 static std::map<void *, size_t> g_alloc;
@@ -92,6 +95,10 @@ hipError_t hipStreamBeginCapture(hipStream_t s, hipStreamCaptureMode) {
   return hipSuccess;
 }
 hipError_t hipStreamEndCapture(hipStream_t s, hipGraph_t *g) {
+  if (s->graph && s->graph->forked != 0) {
+    std::fprintf(stderr, "emu: capture ended with a forked stream not joined back\n");
+    std::abort();
+  }
   s->capturing = false;
   *g = s->graph;
   s->graph = nullptr;
@@ -108,8 +115,31 @@ hipError_t hipGraphLaunch(hipGraphExec_t e, hipStream_t st) {
 hipError_t hipGraphExecDestroy(hipGraphExec_t e) { delete e; return hipSuccess; }
 hipError_t hipGraphDestroy(hipGraph_t g) { delete g; return hipSuccess; }
 hipError_t hipEventCreate(hipEvent_t *e) { *e = new EmuEvent(); return hipSuccess; }
+hipError_t hipEventCreateWithFlags(hipEvent_t *e, unsigned) { *e = new EmuEvent(); return hipSuccess; }
 hipError_t hipEventDestroy(hipEvent_t e) { delete e; return hipSuccess; }
-hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
+hipError_t hipEventRecord(hipEvent_t e, hipStream_t st) {
+  e->src = st;
+  e->graph = st && st->capturing ? st->graph : nullptr;
+  return hipSuccess;
+}
+static long long g_waits = 0;
+long long emu_cross_stream_waits() { return g_waits; }
+hipError_t hipStreamWaitEvent(hipStream_t s, hipEvent_t e, unsigned) {
+  if (e->src && e->src != s) ++g_waits;
+  if (!e->graph || e->src == s) return hipSuccess;  // outside a capture work runs in order
+  if (!s->capturing) {  // fork: s joins the capture
+    s->capturing = true;
+    s->joined = true;
+    s->graph = e->graph;
+    e->graph->forked++;
+  } else if (e->src->joined && e->src->graph == s->graph) {  // join back
+    e->src->capturing = false;
+    e->src->joined = false;
+    e->src->graph = nullptr;
+    s->graph->forked--;
+  }
+  return hipSuccess;
+}
 hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
 hipError_t hipEventElapsedTime(float *ms, hipEvent_t, hipEvent_t) {
   static float t = 1.0f;

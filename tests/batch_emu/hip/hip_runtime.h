@@ -2,7 +2,9 @@
 // tree and batch engine sources (tree.hip, batch.hip) call, so that their
 // planning logic compiles with g++ and runs on the CPU: "device" memory is
 // host memory, streams run work immediately, a captured graph records the
-// launches issued between begin and end of capture and replays them.
+// launches issued between begin and end of capture and replays them (a
+// stream that waits on an event recorded in a capture joins that capture,
+// and leaves it when the capturing stream waits on its event).
 #pragma once
 #include <cstddef>
 #include <cstdint>
@@ -16,6 +18,7 @@ typedef struct EmuNode *hipGraphNode_t;
 enum hipMemcpyKind { hipMemcpyHostToDevice, hipMemcpyDeviceToHost, hipMemcpyDeviceToDevice };
 enum { hipStreamNonBlocking = 1 };
 enum hipStreamCaptureMode { hipStreamCaptureModeThreadLocal = 1 };
+enum { hipEventDisableTiming = 2 };
 struct hipDeviceProp_t {
   char name[256];
   char gcnArchName[256];
@@ -45,6 +48,8 @@ hipError_t hipGraphLaunch(hipGraphExec_t, hipStream_t);
 hipError_t hipGraphExecDestroy(hipGraphExec_t);
 hipError_t hipGraphDestroy(hipGraph_t);
 hipError_t hipEventCreate(hipEvent_t *);
+hipError_t hipEventCreateWithFlags(hipEvent_t *, unsigned);
+hipError_t hipStreamWaitEvent(hipStream_t, hipEvent_t, unsigned);
 hipError_t hipEventDestroy(hipEvent_t);
 hipError_t hipEventRecord(hipEvent_t, hipStream_t);
 hipError_t hipEventSynchronize(hipEvent_t);
@@ -54,3 +59,5 @@ hipError_t hipEventElapsedTime(float *, hipEvent_t, hipEvent_t);
 void emu_submit(hipStream_t st, void (*fn)(void *), void *arg, size_t arg_bytes);
 // bytes currently allocated through hipMalloc
 long long emu_live_bytes();
+// hipStreamWaitEvent calls on an event of another stream (forks and joins)
+long long emu_cross_stream_waits();

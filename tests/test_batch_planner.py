@@ -10,7 +10,8 @@ chunks, adds scenarios while others run and removes finished ones, and
 checks that every scenario ends with exactly its own number of root steps
 (REL_TIME times that in its child), continuity before momentum, JNQ/JNZ in
 the right sub-steps, tiles covering each layer once per launch, queue heads
-cleared, and no device memory left after a reset.
+cleared, forked streams joined back before a capture ends, and no device
+memory left after a reset.
 """
 import os
 import shutil
@@ -22,15 +23,15 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 EMU = os.path.join(HERE, "batch_emu")
 CSRC = os.path.join(ROOT, "gpu-comcot_amd", "csrc")
-BUILD = os.path.join(EMU, "_build")
 
 
 @pytest.fixture(scope="module")
-def driver():
+def driver(tmp_path_factory):
     if shutil.which("g++") is None:
         pytest.skip("g++ not available")
-    os.makedirs(BUILD, exist_ok=True)
-    exe = os.path.join(BUILD, "emu_driver")
+    # a directory per test process: parallel workers must not overwrite
+    # each other's executable while it runs
+    exe = str(tmp_path_factory.mktemp("batch_emu") / "emu_driver")
     cmd = ["g++", "-std=c++17", "-O1", "-g", "-fsanitize=address,undefined",
            "-fno-omit-frame-pointer", "-I", EMU, "-I", CSRC, "-x", "c++",
            os.path.join(CSRC, "tree.hip"), os.path.join(CSRC, "batch.hip"), "-x", "none",
@@ -47,6 +48,18 @@ def test_random_mixes(driver, seed):
                        timeout=300)
     assert r.returncode == 0, r.stderr[-2000:]
     assert "verified" in r.stdout
+
+
+def test_single_layer_scenarios_run_beside_the_nested_chain(driver):
+    """Single-layer scenarios' launches fork onto a second stream and join
+    back within each plan, captured into the graphs and enqueued directly,
+    and every scenario still ends on its own step count."""
+    import re
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1")
+    r = subprocess.run([driver, "5", "25"], capture_output=True, text=True, env=env, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    m = re.search(r"(\d+) cross-stream waits", r.stdout)
+    assert m and int(m.group(1)) > 0, r.stdout
 
 
 @pytest.mark.parametrize("seed", [21, 22])

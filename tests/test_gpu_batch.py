@@ -41,13 +41,16 @@ def _check_vs_oracle(b, sid, sc, orc, tag):
         assert np.array_equal(gm["arrival"], om["arrival"]), f"{tag} layer {L.id} arrival"
 
 
-@pytest.mark.parametrize("two_step,graphs", [(True, True), (False, True), (True, False)],
-                         ids=["two-step", "one-step", "two-step-no-graphs"])
-def test_mixed_batch_matches_oracle(two_step, graphs):
+@pytest.mark.parametrize("two_step,graphs,streams",
+                         [(True, True, 2), (False, True, 2), (True, False, 2), (True, True, 1)],
+                         ids=["two-step", "one-step", "two-step-no-graphs", "one-stream"])
+def test_mixed_batch_matches_oracle(two_step, graphs, streams):
+    """Single-layer and nested scenarios together; with two streams the
+    single-layer ones run beside the nested chain."""
     steps = [s + (i % 3) for i, (_, _, s) in enumerate(CASES)]  # odd and even counts
     print(f"[gpu] batch of {len(CASES)} scenarios, steps {steps}, two_step={two_step}, "
-          f"graphs={graphs}", flush=True)
-    b = Batch(two_step=two_step, graphs=graphs)
+          f"graphs={graphs}, streams={streams}", flush=True)
+    b = Batch(two_step=two_step, graphs=graphs, streams=streams)
     sids = [b.add(sc, n) for (_, sc, _), n in zip(CASES, steps)]
     assert b.run(10 ** 6) == len(CASES)
     for (name, sc, _), sid, n in zip(CASES, sids, steps):
