@@ -76,9 +76,9 @@ struct Group {
   int head = 0;            // index of this launch's 8 heads
   long long cells = 0;     // cell updates of a step group
   std::vector<std::pair<const Tree *, const LayerRec *>> recs;  // layer of each desc
-  // the layers are single-layer scenarios: nothing else in the plan reads or
-  // writes them, so the group may run beside the nested scenarios' chain
-  bool side = false;
+  // stream lane: 0 and 2 = the two chains of nested scenarios, 1 = the
+  // single-layer scenarios; the lanes share no layer, so they run side by side
+  int lane = 0;
 };
 
 struct Plan {
@@ -100,11 +100,11 @@ struct Plan {
 struct Engine {
   bool init = false;
   hipStream_t st = nullptr;
-  // single-layer scenarios' launches run on st2, forked from st at the
-  // start of a plan and joined back at its end (both at default priority)
-  hipStream_t st2 = nullptr;
-  hipEvent_t fork = nullptr, join = nullptr;
-  bool concurrent = true;
+  // lanes 1 and 2 of a plan run on st2 and st3, forked from st at the start
+  // of the plan and joined back at its end (all at default priority)
+  hipStream_t st2 = nullptr, st3 = nullptr;
+  hipEvent_t fork = nullptr, join2 = nullptr, join3 = nullptr;
+  int streams = 3;
   std::map<int, std::unique_ptr<Scenario>> scen;
   int next_sid = 1;
   float arr_thr = 0.01f;
@@ -133,8 +133,10 @@ int eng_init() {
   HIPCHK(hipSetDevice(0));
   HIPCHK(hipStreamCreateWithFlags(&g_eng.st, hipStreamNonBlocking));
   HIPCHK(hipStreamCreateWithFlags(&g_eng.st2, hipStreamNonBlocking));
+  HIPCHK(hipStreamCreateWithFlags(&g_eng.st3, hipStreamNonBlocking));
   HIPCHK(hipEventCreateWithFlags(&g_eng.fork, hipEventDisableTiming));
-  HIPCHK(hipEventCreateWithFlags(&g_eng.join, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&g_eng.join2, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&g_eng.join3, hipEventDisableTiming));
   g_eng.init = true;
   return 0;
 }
@@ -150,6 +152,7 @@ void invalidate() {
   if (g_eng.plan.arena || g_eng.plan.exec) {
     (void)hipStreamSynchronize(g_eng.st);
     (void)hipStreamSynchronize(g_eng.st2);
+    (void)hipStreamSynchronize(g_eng.st3);
     free_plan(g_eng.plan);
   }
   g_eng.plan_valid = false;
@@ -271,10 +274,7 @@ void emit_step(const std::vector<Scenario *> &act, std::vector<Group> &out, int 
     }
     maxsib = std::max(maxsib, R.children.size());
   }
-  const size_t first = out.size();
   flush_steps(pend, out);
-  for (size_t q = first; q < out.size(); ++q)  // fused and two-step roots without children
-    out[q].side = out[q].mode == kFused || out[q].mode == kTwoStep;
   for (size_t sib = 0; sib < maxsib; ++sib) {
     int maxrt = 0;
     for (Scenario *s : act) {
@@ -335,6 +335,36 @@ void emit_step(const std::vector<Scenario *> &act, std::vector<Group> &out, int 
     R.cur = 1 - R.cur;
     s->tree.ctl_in = 1 - s->tree.ctl_in;
   }
+}
+
+// emit_step over the active scenarios in lanes that can run side by side:
+// single-layer scenarios in lane 1, the nested ones in lanes 0 and 2 (with
+// three streams; balanced by cell updates per root step, in scenario order).
+void emit_lanes(const std::vector<Scenario *> &act, std::vector<Group> &out, int pair) {
+  std::vector<Scenario *> single, nested[2];
+  long long load[2] = {0, 0};
+  for (Scenario *s : act) {
+    const LayerRec &R = *s->tree.layer(1);
+    if (R.children.empty()) {
+      single.push_back(s);
+      continue;
+    }
+    long long c = 0;
+    for (const LayerRec &L : s->tree.layers)
+      if (L.used) c += (long long)L.nx * L.ny * (L.id == 1 ? 1 : L.rel_time);
+    const int q = g_eng.streams >= 3 && load[1] < load[0] ? 1 : 0;
+    nested[q].push_back(s);
+    load[q] += c;
+  }
+  auto add = [&](const std::vector<Scenario *> &v, int lane) {
+    if (v.empty()) return;
+    const size_t first = out.size();
+    emit_step(v, out, pair);
+    for (size_t k = first; k < out.size(); ++k) out[k].lane = lane;
+  };
+  add(nested[0], 0);
+  add(single, g_eng.streams >= 2 ? 1 : 0);
+  add(nested[1], 2);
 }
 
 size_t al(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -433,36 +463,39 @@ int enqueue_group(const Plan &P, const Group &g, hipStream_t st) {
 }
 
 // Root steps r0..r1-1 of a plan.  The queue heads of all of them are
-// cleared first (each group has heads of its own).  Groups of single-layer
-// scenarios then run on the second stream, all their steps in order, while
-// the nested scenarios' chain (continuity halves, JNQ, children, JNZ,
-// momentum halves) runs on the first; the second stream joins back at the
-// end, so work enqueued on the first stream afterwards (and its
-// synchronisations) follows both.
+// cleared first (each group has heads of its own).  Each lane's groups then
+// run on a stream of their own, all their steps in order: lane 0 on st,
+// lanes 1 and 2 on st2 and st3, forked from st after the clearing and
+// joined back at the end, so work enqueued on st afterwards (and its
+// synchronisations) follows every lane.  A lane chains continuity halves,
+// JNQ, children, JNZ and momentum halves of its scenarios as one stream
+// would; lanes touch disjoint scenarios.
 int enqueue_plan(const Plan &P, int r0, int r1) {
   hipStream_t st = g_eng.st;
   const int h0 = P.heads_off[r0], h1 = P.heads_off[r1 - 1] + P.heads_n[r1 - 1];
   hipError_t e = launch_fill_i32(P.heads + h0, h1 - h0, 0, st);
   if (e != hipSuccess) return fail(210, "batch head reset: %s", hipGetErrorString(e));
-  bool side = false, chain = false;
+  bool used[3] = {false, false, false};
   for (int r = r0; r < r1; ++r)
-    for (const auto &g : P.step[r]) (g.side ? side : chain) = true;
-  const bool fork = g_eng.concurrent && side && chain;
-  if (fork) {
-    HIPCHK(hipEventRecord(g_eng.fork, st));
-    HIPCHK(hipStreamWaitEvent(g_eng.st2, g_eng.fork, 0));
+    for (const auto &g : P.step[r]) used[g.lane] = true;
+  hipStream_t lane_st[3] = {st, g_eng.st2, g_eng.st3};
+  hipEvent_t join[3] = {nullptr, g_eng.join2, g_eng.join3};
+  bool forked = false;
+  for (int l = 1; l < 3; ++l) {
+    if (!used[l]) continue;
+    if (!forked) HIPCHK(hipEventRecord(g_eng.fork, st));
+    forked = true;
+    HIPCHK(hipStreamWaitEvent(lane_st[l], g_eng.fork, 0));
+  }
+  for (int l = 0; l < 3; ++l)
     for (int r = r0; r < r1; ++r)
       for (const auto &g : P.step[r])
-        if (g.side)
-          if (int q = enqueue_group(P, g, g_eng.st2)) return q;
-  }
-  for (int r = r0; r < r1; ++r)
-    for (const auto &g : P.step[r])
-      if (!fork || !g.side)
-        if (int q = enqueue_group(P, g, st)) return q;
-  if (fork) {
-    HIPCHK(hipEventRecord(g_eng.join, g_eng.st2));
-    HIPCHK(hipStreamWaitEvent(st, g_eng.join, 0));
+        if (g.lane == l)
+          if (int q = enqueue_group(P, g, lane_st[l])) return q;
+  for (int l = 1; l < 3; ++l) {
+    if (!used[l]) continue;
+    HIPCHK(hipEventRecord(join[l], lane_st[l]));
+    HIPCHK(hipStreamWaitEvent(st, join[l], 0));
   }
   return 0;
 }
@@ -618,7 +651,7 @@ int build_plan(const std::vector<Scenario *> &act) {
   // every time level is back where it started at the end of the plan
   const bool pairs = any_two_step(act);
   P.nsteps = pairs ? 4 : 2;
-  for (int k = 0; k < P.nsteps; ++k) emit_step(act, P.step[k], pairs ? k % 2 : -1);
+  for (int k = 0; k < P.nsteps; ++k) emit_lanes(act, P.step[k], pairs ? k % 2 : -1);
   if (trace_on())
     std::fprintf(stderr, "batch: plan of %d steps, %zu + %zu launches\n", P.nsteps,
                  P.step[0].size(), P.step[1].size());
@@ -730,6 +763,7 @@ int gcomcot_batch_reset_(void) {
   invalidate();
   (void)hipStreamSynchronize(g_eng.st);
   (void)hipStreamSynchronize(g_eng.st2);
+  (void)hipStreamSynchronize(g_eng.st3);
   for (auto &kv : g_eng.scen) tree_free(kv.second->tree);
   g_eng.scen.clear();
   if (g_eng.arena) (void)hipFree(g_eng.arena);
@@ -953,8 +987,8 @@ int gcomcot_batch_set_two_step_(int *on, int *shape) {
 }
 
 int gcomcot_batch_set_streams_(int *n) {
-  if (*n < 1 || *n > 2) return fail(45, "batch: %d streams (1 or 2)", *n);
-  g_eng.concurrent = *n == 2;
+  if (*n < 1 || *n > 3) return fail(45, "batch: %d streams (1 to 3)", *n);
+  g_eng.streams = *n;
   invalidate();
   return 0;
 }

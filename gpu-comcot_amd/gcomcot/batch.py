@@ -51,13 +51,13 @@ class Batch:
     """The library's batch engine (one per process)."""
 
     def __init__(self, arrival_threshold=0.01, autotune=True, shape=-1, graphs=True,
-                 two_step=True, two_step_shape=None, streams=2):
+                 two_step=True, two_step_shape=None, streams=3):
         """two_step: scenarios whose only layer is linear may advance two
         root steps per launch, in tile shape two_step_shape (the library's
         two-step shapes 0-6), or, when None, in the shape the autotuner
-        finds fastest if that beats two one-step launches.  streams=2:
-        single-layer scenarios run beside the nested scenarios' chain of
-        launches; 1: everything on one stream."""
+        finds fastest if that beats two one-step launches.  streams: 3 =
+        single-layer scenarios and two chains of nested scenarios side by
+        side, 2 = single-layer scenarios beside one chain, 1 = one stream."""
         L = lib()
         _setup(L)
         L.gcomcot_set_error_mode_(ip(0))

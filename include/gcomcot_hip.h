@@ -462,11 +462,13 @@ int gcomcot_batch_set_graphs_(int *on);
  * launches, else they keep one step per launch.  Results are bitwise those
  * of one step per launch either way. */
 int gcomcot_batch_set_two_step_(int *on, int *shape);
-/* *n = 2 (the default): within each plan the launches of single-layer
- * scenarios run on a second stream beside the nested scenarios' chain of
- * launches (forked at the plan's start, joined at its end, captured into
- * the plan's graph as a parallel branch); *n = 1: one stream.  The same
- * bits either way.  Returns 0, or 45 for another *n. */
+/* Streams of a plan: *n = 3 (the default): the launches of single-layer
+ * scenarios run on a second stream and the nested scenarios are split into
+ * two chains of launches (balanced by cell updates), one on the first
+ * stream and one on a third, all forked at the plan's start and joined at
+ * its end (parallel branches of the plan's graph); *n = 2: the nested
+ * scenarios in one chain; *n = 1: everything on one stream.  The same bits
+ * every way.  Returns 0, or 45 for another *n. */
 int gcomcot_batch_set_streams_(int *n);
 int gcomcot_batch_info_(long long *launches, long long *graph_replays, long long *plans,
                         long long *cells_per_step, int *tuned_count);

@@ -42,11 +42,13 @@ def _check_vs_oracle(b, sid, sc, orc, tag):
 
 
 @pytest.mark.parametrize("two_step,graphs,streams",
-                         [(True, True, 2), (False, True, 2), (True, False, 2), (True, True, 1)],
-                         ids=["two-step", "one-step", "two-step-no-graphs", "one-stream"])
+                         [(True, True, 3), (False, True, 3), (True, False, 3), (True, True, 2),
+                          (True, True, 1)],
+                         ids=["two-step", "one-step", "two-step-no-graphs", "two-streams",
+                              "one-stream"])
 def test_mixed_batch_matches_oracle(two_step, graphs, streams):
-    """Single-layer and nested scenarios together; with two streams the
-    single-layer ones run beside the nested chain."""
+    """Single-layer and nested scenarios together; with more than one stream
+    the single-layer ones run beside the nested chain(s)."""
     steps = [s + (i % 3) for i, (_, _, s) in enumerate(CASES)]  # odd and even counts
     print(f"[gpu] batch of {len(CASES)} scenarios, steps {steps}, two_step={two_step}, "
           f"graphs={graphs}, streams={streams}", flush=True)
