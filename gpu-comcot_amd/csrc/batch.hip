@@ -17,7 +17,11 @@
 //
 // which is the per-scenario order of the drop-in path (all_grids.f90:34-
 // 308, comcot.f90:359-439), so every scenario's planes are bit-identical
-// to a run of that scenario alone.  The launches of two root steps are
+// to a run of that scenario alone.  Single-layer linear roots may instead
+// take one launch per two root steps.  The phases are emitted in lanes of
+// disjoint scenarios (single-layer scenarios; two halves of the nested
+// ones) that run on three streams, forked and joined within each plan.
+// The launches of a plan (two root steps, four with two-step roots) are
 // captured in a HIP graph and replayed; the plan is rebuilt when the set
 // of active scenarios changes (a scenario finishes, is added or removed).
 //
