@@ -216,7 +216,7 @@ def bench_other_configs(a):
                           ("nest_2048x1536_1801x1801", two_level_nest(), BYTES_PER_CELL["linear"])):
         log(f"{name}: {sc.name}, {sc.steps} steps")
         run = DeviceRun(sc)
-        tuned = run.autotune() if len(sc.layers) == 1 else None
+        tuned = run.autotune()
         run.run(10, graph=bool(a.graphs))
         t0 = time.perf_counter()
         run.run(sc.steps, graph=bool(a.graphs))
@@ -225,8 +225,7 @@ def bench_other_configs(a):
         run.close()
         gcups = cells * sc.steps / wall / 1e9
         out[name] = {"workload": sc.name, "steps": sc.steps, "cell_updates_per_step": cells,
-                     "tuned": None if tuned is None else
-                     {"tile_variant": tuned[0], "two_step_shape": tuned[1]},
+                     "tuned": {"tile_variant": tuned[0], "two_step_shape": tuned[1]},
                      "gcups": round(gcups, 3), "ms_per_step": round(wall * 1e3 / sc.steps, 5),
                      "algorithmic_GBs": round(gcups * bpc, 1)}
     return out

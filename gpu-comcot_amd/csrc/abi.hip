@@ -803,6 +803,40 @@ int gcomcot_autotune_(int *reps, int *variant, int *blocking, float *ms_per_step
     *out = ms / (*reps * steps_per_launch);
     return 0;
   };
+  if (!fused_only) {
+    // nested trees: each layer without nested layers of its own runs fused
+    // launches; its tile variant is chosen on its own (dry launches on that
+    // layer); the others run the split halves, whose tile is fixed.  The
+    // reported time sums those layers' fused launches per root step, the
+    // reported variant is the one of the layer with the most cell updates.
+    float sum = 0.0f;
+    long long most = -1;
+    for (LayerRec &L : g_tree.layers) {
+      if (!L.used || L.id == 1 || !L.children.empty()) continue;
+      int per_root = 1;
+      for (const LayerRec *q = &L; q->id != 1; q = &g_tree.layers[q->parent]) per_root *= q->rel_time;
+      DevLayer dl = dev_view(g_tree, L, false, false);
+      float lbest = 1e30f;
+      int lv = 0;
+      for (int v = 0; v < num_tile_variants(); ++v) {
+        float ms;
+        if (int r = time_it([&] { return launch_step(dl, kFused, L.nl != 0, L.sph != 0, v, g_stream); },
+                            1, &ms))
+          return r;
+        if (ms < lbest) { lbest = ms; lv = v; }
+      }
+      L.variant = lv;
+      sum += lbest * per_root;
+      const long long upd = (long long)L.nx * L.ny * per_root;
+      if (upd > most) { most = upd; bv = lv; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (variant) *variant = bv;
+    if (blocking) *blocking = g_tree.blocking;
+    if (ms_per_step) *ms_per_step = sum;
+    return 0;
+  }
   if (fused_only) {
     for (int v = 0; v < num_tile_variants(); ++v) {
       float ms;
@@ -967,6 +1001,7 @@ int gcomcot_detmath_eval_noalloc_(long long *n, const float *x, float *pow_out,
 int gcomcot_set_tile_variant_(int *variant) {
   if (*variant < 0 || *variant >= num_tile_variants()) return 1;
   g_tree.variant = *variant;
+  for (LayerRec &L : g_tree.layers) L.variant = -1;  // the autotuner's per-layer choices
   return 0;
 }
 

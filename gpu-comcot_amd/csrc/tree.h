@@ -45,6 +45,7 @@ struct LayerRec {
   int sc_band[4][4] = {};
   size_t sc_jnz_cells = 0;
   int cur = 0;  // buffer index holding time level 1
+  int variant = -1;  // tile variant of this layer's fused launches (-1: the tree's)
   std::vector<int> children;
   size_t nxy() const { return (size_t)nx * ny; }
 };

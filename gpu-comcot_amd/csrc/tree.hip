@@ -337,7 +337,8 @@ DevNest nest_view(const LayerRec &P, const LayerRec &C) {
 int tree_launch(const Tree &T, const LayerRec &L, int mode, bool with_max, bool advance,
                 hipStream_t st) {
   DevLayer d = dev_view(T, L, with_max, advance);
-  hipError_t e = launch_step(d, mode, L.nl != 0, L.sph != 0, T.variant, st);
+  const int v = mode == kFused && L.variant >= 0 ? L.variant : T.variant;
+  hipError_t e = launch_step(d, mode, L.nl != 0, L.sph != 0, v, st);
   if (e != hipSuccess) return fail(200 + (int)e, "step kernel launch failed: %s", hipGetErrorString(e));
   return 0;
 }

@@ -102,8 +102,11 @@ class DeviceRun:
             check(rc, f"init layer {s.id}")
 
     def autotune(self, reps=5):
-        """Pick the fastest step kernel shape for this layer (library autotuner);
-        returns (tile variant, two-step shape or 0, ms per root step)."""
+        """Pick the fastest step kernel shape (library autotuner): for a
+        single layer its tile variant or two-step shape, for a nest the
+        tile variant of each layer that runs fused steps; returns (tile
+        variant, two-step mode or 0, ms per root step of the tuned
+        launches)."""
         v, b, ms = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_float(0)
         check(lib().gcomcot_autotune_(ip(reps), ctypes.byref(v), ctypes.byref(b),
                                       ctypes.byref(ms)), "autotune")

@@ -341,7 +341,12 @@ int gcomcot_output_flush_(long long *nfiles);
  * with the maximum and the step counter switched off (such launches read
  * level 1 and write only level 2, so the state is unchanged); keep the
  * fastest for gcomcot_run_steps_ / gcomcot_time_steps_.  Reports the
- * choice and its time per root step.  Returns 0 or an error code. */
+ * choice and its time per root step.  In a tree with nested layers each
+ * layer without nested layers of its own (the ones that run fused steps)
+ * gets its own fastest tile variant the same way; *variant is then that of
+ * the layer with the most cell updates per root step, *ms_per_step the sum
+ * of those layers' fused launches per root step.  gcomcot_set_tile_variant_
+ * clears the per-layer choices.  Returns 0 or an error code. */
 int gcomcot_autotune_(int *reps, int *variant, int *blocking, float *ms_per_step);
 
 /* Two root steps per launch (temporal blocking) in gcomcot_run_steps_ for

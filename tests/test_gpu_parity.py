@@ -34,8 +34,11 @@ def test_device_loop_matches_oracle(name, sc, steps, graph):
     dev.close()
 
 
-@pytest.mark.parametrize("name,sc,steps", CASES[:5], ids=[c[0] for c in CASES[:5]])
+@pytest.mark.parametrize("name,sc,steps", CASES, ids=[c[0] for c in CASES])
 def test_every_tile_shape(name, sc, steps):
+    """Every tile variant of the fused step (tiles and strips), single layers
+    and the nested layers of every nest pairing (the autotuner may give a
+    nested layer any of them)."""
     from gcomcot._lib import lib, ip
     import ctypes
     n = ctypes.c_int(0)
@@ -127,8 +130,8 @@ def test_linear_layers_one_or_two_steps_per_launch(mode):
         lib().gcomcot_set_temporal_blocking_(ip(1))
 
 
-@pytest.mark.parametrize("name,sc,steps", [CASES[0], CASES[2], CASES[4]],
-                         ids=[CASES[0][0], CASES[2][0], CASES[4][0]])
+@pytest.mark.parametrize("name,sc,steps", [CASES[0], CASES[2], CASES[4], CASES[6], CASES[7]],
+                         ids=[CASES[0][0], CASES[2][0], CASES[4][0], CASES[6][0], CASES[7][0]])
 def test_autotune_mid_run_keeps_the_state(name, sc, steps):
     """The library autotuner times dry launches on the live layer: a run
     tuned half way must end exactly where the oracle does."""
@@ -142,7 +145,7 @@ def test_autotune_mid_run_keeps_the_state(name, sc, steps):
         v, b, ms = dev.autotune(reps=2)
         assert ms > 0 and v >= 0
         assert 0 <= b <= 7
-        if sc.layers[0].laygov != 0:
+        if sc.layers[0].laygov != 0 and len(sc.layers) == 1:
             assert b == 0  # two steps per launch are for linear layers only
         dev.run(40)
         _compare(sc, dev, orc, f"{name} tuned")
