@@ -155,6 +155,37 @@ def cpu_baseline(sc, nsteps):
                       f"{dt:.1f} s"}
 
 
+def host_cpu():
+    """nproc and model name of the host the CPU figures were taken on."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "model": model}
+
+
+def cpu_all_cores(sc, nsteps):
+    """The FP32 restatement (the reference's libm pow) on every OpenMP
+    thread the host gives it, on the same sample: the cell loops are
+    Jacobi-style, so its results equal the serial run's.  Reported beside
+    the single-core reference, labelled as the restatement."""
+    from oracle import bind
+    runner = bind.Oracle(sc, 32, pow_mode=0)
+    runner.step(1)
+    t0 = time.perf_counter()
+    runner.step(nsteps)
+    dt = time.perf_counter() - t0
+    threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+    return {"value": sc.cell_updates_per_step() * nsteps / dt / 1e9, "unit": "GCUPS",
+            "cores": threads, "kind": "port",
+            "sample": f"{nsteps} steps (after 1 untimed), {threads} OpenMP threads, {dt:.1f} s"}
+
+
 def traffic_entry(workload_key):
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
@@ -227,7 +258,8 @@ def bench_other_configs(a):
         out[name] = {"workload": sc.name, "steps": sc.steps, "cell_updates_per_step": cells,
                      "tuned": {"tile_variant": tuned[0], "two_step_shape": tuned[1]},
                      "gcups": round(gcups, 3), "ms_per_step": round(wall * 1e3 / sc.steps, 5),
-                     "algorithmic_GBs": round(gcups * bpc, 1)}
+                     "algorithmic_GBs": round(gcups * bpc, 1),
+                     "frac_of_8TBs": round(gcups * bpc / HBM_PEAK_GBS, 4)}
     return out
 
 
@@ -487,7 +519,11 @@ def main():
     cpu = None
     if a.cpu_steps > 0:
         log("cpu baseline")
-        cpu = cpu_baseline(nonlinear_layer(nx=a.nx, ny=a.ny, steps=a.cpu_steps), a.cpu_steps)
+        csc = nonlinear_layer(nx=a.nx, ny=a.ny, steps=a.cpu_steps)
+        cpu = cpu_baseline(csc, a.cpu_steps)
+        cpu["host"] = host_cpu()
+        log("cpu baseline: the restatement on all cores")
+        cpu["all_cores_restatement"] = cpu_all_cores(csc, a.cpu_steps)
         if a.accuracy:
             cpu["accuracy"] = cpu_accuracy()
     others = bench_other_configs(a) if a.others else None
@@ -519,8 +555,11 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
+                     "traffic_per_cell_update": round(traffic / cells, 2) if traffic else None,
                      "practical_peak": copy["read_plus_write_GBs"],
-                     "frac_of_practical": round(achieved / copy["read_plus_write_GBs"], 4)},
+                     "frac_of_practical": round(achieved / copy["read_plus_write_GBs"], 4),
+                     "limit": "VALU issue: the nonlinear step's instruction stream at one "
+                              "wave64 instruction per SIMD every 4 cycles"},
         "cpu_baseline": cpu,
         "kernel_ms_mean": round(kern_ms, 5),
         "event_ms_total": round(ms_total, 3),
