@@ -1176,11 +1176,51 @@ __global__ void k_jnq(DevNest N, int t) {
 
 // JNZ (all_grids.f90:342-421): parent Z2 over the footprint = mean of the
 // wet child cells, summed in the Fortran order (KI outer, KJ inner).
+// For R = 2..5 every load of the R x R child block is issued first (for
+// one KI, KJ a wave reads one child row, R columns apart, and the R values
+// of KI fill it in), then the sums run in that order; other R take the
+// loop below.
+template <int R>
+__device__ __forceinline__ void jnz_block(const DevNest &N, int i, int j) {
+  // 0-based first child cell: II = (I-IS) R + 1 + KI, KI = 1..R (1-based)
+  const int i0 = (i - N.is) * R + 1, j0 = (j - N.js) * R + 1;
+  const bool even = (N.rel_time % 2) == 0;
+  float z2[R][R], hh[R][R], z1[R][R];
+#pragma unroll
+  for (int kj = 0; kj < R; ++kj)
+#pragma unroll
+    for (int ki = 0; ki < R; ++ki) {
+      const size_t cg = (size_t)(j0 + kj) * N.cnx + (i0 + ki);
+      z2[ki][kj] = N.cz2[cg];
+      hh[ki][kj] = N.ch[cg];
+      z1[ki][kj] = even ? N.cz1[cg] : 0.0f;
+    }
+  const float half = (float)(R * R) / 2.0f;
+  float sum = 0.0f, count = 0.0f;
+#pragma unroll
+  for (int ki = 0; ki < R; ++ki)
+#pragma unroll
+    for (int kj = 0; kj < R; ++kj)
+      if (hh[ki][kj] + z2[ki][kj] > kGx) {
+        if (even) sum = sum + 0.5f * (z1[ki][kj] + z2[ki][kj]);
+        else sum = sum + z2[ki][kj];
+        count = count + 1.0f;
+      }
+  N.pz2_out[(size_t)(j - 1) * N.pnx + (i - 1)] = count > half ? sum / count : 0.0f;
+}
+
 __device__ void jnz_thread(const DevNest &N, int g) {
   const int w = N.ie - N.is + 1, hgt = N.je - N.js + 1;
   if (g >= w * hgt) return;
   const int i = N.is + g % w, j = N.js + g / w;
   const int ir = N.rel_size;
+  switch (ir) {
+    case 2: jnz_block<2>(N, i, j); return;
+    case 3: jnz_block<3>(N, i, j); return;
+    case 4: jnz_block<4>(N, i, j); return;
+    case 5: jnz_block<5>(N, i, j); return;
+    default: break;
+  }
   const float half = (float)(ir * ir) / 2.0f;
   const int i0 = (i - N.is) * ir + 1, j0 = (j - N.js) * ir + 1;
   const bool even = (N.rel_time % 2) == 0;
