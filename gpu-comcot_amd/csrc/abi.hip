@@ -792,6 +792,24 @@ int gcomcot_autotune_(int *reps, int *variant, int *blocking, float *ms_per_step
   HIPCHK(hipEventCreate(&e1));
   float best = 1e30f;
   int bv = g_tree.variant, bb = 0;
+  // The GPU's clocks settle over the first ~0.1 s of sustained work; the
+  // candidates are timed after that, so that the first ones are not timed
+  // at lower clocks than the last (dry launches of the current variant,
+  // at most kSettleLaunches of them).
+  {
+    constexpr int kSettleLaunches = 4000;
+    constexpr float kSettleMs = 100.0f;
+    HIPCHK(hipEventRecord(e0, g_stream));
+    float ms = 0.0f;
+    for (int k = 0, batch = 8; k < kSettleLaunches && ms < kSettleMs; k += batch) {
+      for (int q = 0; q < batch; ++q)
+        HIPCHK(launch_step(d, kFused, R->nl != 0, R->sph != 0, g_tree.variant, g_stream));
+      HIPCHK(hipEventRecord(e1, g_stream));
+      HIPCHK(hipEventSynchronize(e1));
+      HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+      batch = std::min(2 * batch, 256);
+    }
+  }
   auto time_it = [&](auto launch, int steps_per_launch, float *out) -> int {
     HIPCHK(launch());
     HIPCHK(hipEventRecord(e0, g_stream));
