@@ -458,6 +458,37 @@ static int run_plain(int n, int use_graph) {
       if (int r = tree_enqueue_root_step(g_tree, g_stream)) return r;
     return sync();
   }
+  g_tree.mass_ahead = false;
+  if (tree_fuses_halves(g_tree)) {
+    // every root step but the last computes the next one's continuity in
+    // its momentum launch; the first step (outside any graph) leaves the
+    // tree in the state each captured pair starts and ends in
+    int left = n;
+    if (left > 0) {
+      if (int r = tree_enqueue_root_step(g_tree, g_stream, left > 1)) return r;
+      --left;
+    }
+    if (use_graph && left >= 3 && tree_launches_per_step(g_tree) <= kGraphMaxLaunches) {
+      hipGraph_t graph = nullptr;
+      hipGraphExec_t exec = nullptr;
+      HIPCHK(hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal));
+      int r = tree_enqueue_root_step(g_tree, g_stream, true);
+      if (!r) r = tree_enqueue_root_step(g_tree, g_stream, true);
+      hipError_t ce = hipStreamEndCapture(g_stream, &graph);
+      if (r) return r;
+      HIPCHK(ce);
+      HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+      const int reps = (left - 1) / 2;  // at least one step stays for the end
+      for (int k = 0; k < reps; ++k) HIPCHK(hipGraphLaunch(exec, g_stream));
+      HIPCHK(hipStreamSynchronize(g_stream));
+      (void)hipGraphExecDestroy(exec);
+      (void)hipGraphDestroy(graph);
+      left -= 2 * reps;
+    }
+    for (; left > 0; --left)
+      if (int r = tree_enqueue_root_step(g_tree, g_stream, left > 1)) return r;
+    return sync();
+  }
   if (use_graph && n >= 2 && tree_launches_per_step(g_tree) <= kGraphMaxLaunches) {
     // Two root steps bring every layer's level parity back to where it
     // started (children with odd REL_TIME flip once per root step), so a
@@ -729,8 +760,9 @@ int gcomcot_time_steps_(int *nsteps, float *ms_total, float *ms_each) {
   std::vector<hipEvent_t> ev(n + 1);
   for (auto &e : ev) HIPCHK(hipEventCreate(&e));
   HIPCHK(hipEventRecord(ev[0], g_stream));
+  g_tree.mass_ahead = false;
   for (int k = 0; k < n; ++k) {
-    if (int r = tree_enqueue_root_step(g_tree, g_stream)) return r;
+    if (int r = tree_enqueue_root_step(g_tree, g_stream, k + 1 < n)) return r;
     HIPCHK(hipEventRecord(ev[k + 1], g_stream));
   }
   HIPCHK(hipEventSynchronize(ev[n]));
@@ -914,6 +946,11 @@ int gcomcot_gauges_read_(float *series, int *max_steps, int *nout) {
     for (long long k = 0; k < count; ++k)
       series[(size_t)g * *max_steps + k] = ring[(size_t)g * cap + (size_t)((first + k - 1) % cap)];
   *nout = (int)count;
+  return 0;
+}
+
+int gcomcot_set_fused_halves_(int *on) {
+  g_tree.fuse_halves = *on != 0;
   return 0;
 }
 

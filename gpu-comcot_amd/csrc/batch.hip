@@ -63,9 +63,11 @@ struct Scenario {
 };
 
 enum GroupType { kStepGroup = 0, kJnqGroup = 1, kJnzGroup = 2, kScEdgeGroup = 3, kScJnzGroup = 4 };
-// Step-group mode beside kFused/kMassOnly/kMomentOnly: two root steps of
+// Step-group mode beside the step modes of kernels.h: two root steps of
 // single-layer linear roots in one launch (k_batch2_grid).
-constexpr int kTwoStep = 3;
+constexpr int kTwoStep = 4;
+static_assert(kTwoStep != kFused && kTwoStep != kMassOnly && kTwoStep != kMomentOnly &&
+                  kTwoStep != kMomentMass, "group modes must differ");
 
 struct Group {
   int type = kStepGroup;
@@ -115,6 +117,7 @@ struct Engine {
   bool autotune = true;
   bool graphs = true;      // replay plans from HIP graphs (else enqueue each step)
   bool two_step = true;    // single-layer linear roots may take two root steps per launch
+  bool fuse_halves = true; // nested linear roots: momentum + next continuity in one launch
   int step2_shape = 4;     // their tile shape (launch_step2's shapes); -1: one step per launch
   int forced_step2 = -1;   // a shape set by the caller (no tuning of it)
   int forced_shape = -1;
@@ -251,6 +254,13 @@ void pend_add(std::vector<Pending> &pend, int mode, const Tree &T, const LayerRe
   pend.push_back(Pending{mode, L.nl, L.sph, {d}, {{&T, &L}}});
 }
 
+// A nested linear root whose momentum launch also takes the next root
+// step's continuity (see kMomentMass).
+bool fuses_halves(const Tree &T) {
+  const LayerRec &R = T.layers[1];
+  return g_eng.fuse_halves && R.used && !R.children.empty() && !R.nl;
+}
+
 // A single-layer linear root (no sponge: it acts between steps) can take
 // two root steps per launch.
 bool two_step_root(const Tree &T) {
@@ -273,7 +283,7 @@ void emit_step(const std::vector<Scenario *> &act, std::vector<Group> &out, int 
       if (pair == 0) pend_add(pend, kTwoStep, T, R, dev_view(T, R, true, true));
     } else if (R.children.empty()) {
       pend_add(pend, kFused, T, R, dev_view(T, R, true, true));
-    } else {
+    } else if (!T.mass_ahead) {  // else the previous momentum launch did it
       pend_add(pend, kMassOnly, T, R, dev_view(T, R, false, false));
     }
     maxsib = std::max(maxsib, R.children.size());
@@ -330,7 +340,12 @@ void emit_step(const std::vector<Scenario *> &act, std::vector<Group> &out, int 
   for (Scenario *s : act) {
     Tree &T = s->tree;
     LayerRec &R = *T.layer(1);
-    if (!R.children.empty()) pend_add(pend, kMomentOnly, T, R, dev_view(T, R, true, true));
+    if (R.children.empty()) continue;
+    // a nested linear root's next continuity rides in its momentum launch
+    // (a scenario that finishes keeps it in level 2, where nothing reads it)
+    T.mass_ahead = fuses_halves(T);
+    if (T.mass_ahead) pend_add(pend, kMomentMass, T, R, fused_halves_view(T, R));
+    else pend_add(pend, kMomentOnly, T, R, dev_view(T, R, true, true));
   }
   flush_steps(pend, out);
   for (Scenario *s : act) {
@@ -513,16 +528,19 @@ int tune(const std::vector<Scenario *> &act) {
     // are restored afterwards
     std::vector<std::vector<int>> cur;
     std::vector<int> ctl;
+    std::vector<bool> ahead;
     for (Scenario *s : act) {
       std::vector<int> c;
       for (auto &L : s->tree.layers) c.push_back(L.cur);
       cur.push_back(c);
       ctl.push_back(s->tree.ctl_in);
+      ahead.push_back(s->tree.mass_ahead);
     }
     emit_step(act, probe);
     for (size_t i = 0; i < act.size(); ++i) {
       for (int k = 0; k < kMaxLayers; ++k) act[i]->tree.layers[k].cur = cur[i][k];
       act[i]->tree.ctl_in = ctl[i];
+      act[i]->tree.mass_ahead = ahead[i];
     }
   }
   hipEvent_t e0, e1;
@@ -650,6 +668,28 @@ int build_plan(const std::vector<Scenario *> &act) {
   invalidate();
   if (g_eng.autotune && g_eng.forced_shape < 0)
     if (int r = tune(act)) return r;
+  // nested linear roots without their next continuity in level 2 get it
+  // now, outside the plan, so that every step of the plan has the same
+  // launches (each takes the following step's continuity along)
+  {
+    Plan pre;
+    pre.nsteps = 1;
+    std::vector<Pending> pend;
+    for (Scenario *s : act)
+      if (fuses_halves(s->tree) && !s->tree.mass_ahead) {
+        Tree &T = s->tree;
+        pend_add(pend, kMassOnly, T, *T.layer(1), dev_view(T, *T.layer(1), false, false));
+        T.mass_ahead = true;
+      }
+    flush_steps(pend, pre.step[0]);
+    if (!pre.step[0].empty()) {
+      int r = upload_plan(pre);
+      if (!r) r = enqueue_plan(pre, 0, 1);
+      if (!r) r = hipStreamSynchronize(g_eng.st) == hipSuccess ? 0 : fail(211, "batch continuity");
+      free_plan(pre);
+      if (r) return r;
+    }
+  }
   Plan &P = g_eng.plan;
   // two steps, or two pairs when two-step roots are present: either way
   // every time level is back where it started at the end of the plan
@@ -936,6 +976,9 @@ int gcomcot_batch_fault_deform_(int *sid, const float *params, int *fswitch) {
   Scenario *s = scenario(*sid);
   if (!s) return fail(31, "batch: no scenario %d", *sid);
   if (int e = eng_init()) return e;
+  // the plan may take this scenario's next continuity as already done;
+  // the deformation changes level 1, so that continuity is computed again
+  invalidate();
   HIPCHK(hipStreamSynchronize(g_eng.st));
   return tree_apply_fault(s->tree, g_eng.st, params, *fswitch);
 }
@@ -986,6 +1029,12 @@ int gcomcot_batch_set_two_step_(int *on, int *shape) {
   g_eng.two_step = *on != 0;
   g_eng.forced_step2 = shape ? *shape : -1;
   g_eng.step2_shape = shape ? *shape : 4;
+  invalidate();
+  return 0;
+}
+
+int gcomcot_batch_set_fused_halves_(int *on) {
+  g_eng.fuse_halves = *on != 0;
   invalidate();
   return 0;
 }

@@ -255,6 +255,7 @@ int tree_hotstart_write(Tree &T, hipStream_t st, int start_step, const std::stri
 
 int tree_hotstart_read(Tree &T, hipStream_t st, int start_type, int start_step,
                        const std::string &dir, int *missing) {
+  T.mass_ahead = false;  // level 2 is no longer the next continuity
   if (start_type != 1 && start_type != 2)
     return fail(94, "hot start: START_TYPE %d is not a load (1 or 2)", start_type);
   HIPCHK(hipStreamSynchronize(st));

@@ -37,7 +37,14 @@ struct DevLayer {
   const float *sponge_cx, *sponge_cy;
 };
 
-enum StepMode { kFused = 0, kMassOnly = 1, kMomentOnly = 2 };
+// kMomentMass: the momentum half of root step k of a linear outermost
+// layer with nested layers, with the continuity (+ open boundary) of step
+// k+1 in the same launch.  Nothing runs between the two in the root step
+// order: momentum(k) reads Z(k) (after JNZ) and the level-1 fluxes and
+// writes M(k), N(k) to level 2; continuity(k+1) reads Z(k), M(k), N(k)
+// and writes Z(k+1) into the buffer that held Z(k-1), which linear
+// momentum does not read.  The view's z1 is Z(k), its z2 that buffer.
+enum StepMode { kFused = 0, kMassOnly = 1, kMomentOnly = 2, kMomentMass = 3 };
 
 // One pass over all tiles of a layer (tile shape chosen by variant).
 hipError_t launch_step(const DevLayer &L, int mode, bool nl, bool sph, int variant,

@@ -658,14 +658,106 @@ __device__ __forceinline__ void tile_body(const DevLayer &L, int t, int ti0, int
   }
 }
 
+// kMomentMass for one tile of a linear outermost layer (see kernels.h).
+// Stages Z(k), M(k-1), N(k-1), H with a two-cell halo; computes momentum
+// (MOMT_S / MOMT_C, and SPONGE_LAYER) on the tile plus one cell to the
+// left and below, into LDS; then, per tile cell, stores M(k), N(k), takes
+// the wet maximum / arrival / MAX_AMP of Z(k) and computes continuity +
+// open boundary of step k+1 from Z(k), M(k), N(k): the open boundary and
+// the divergence read the fluxes of the cell and of its left and lower
+// neighbours only.  Each value comes from the same functions and operands
+// as in the momentum-only and continuity-only launches.
+template <bool SPH, int TX, int TY, bool IN>
+__device__ __forceinline__ void tile_body_mm(const DevLayer &L, int t, int ti0, int tj0,
+                                             float *sm, const gcdet::Tables *stab) {
+  constexpr int P = TX + 4, R = TY + 4, A = P * R;
+  float *pz = sm, *pm0 = sm + A, *pn0 = sm + 2 * A, *ph = sm + 3 * A;
+  float *pm = sm + 4 * A, *pn = sm + 5 * A, *crow = sm + 6 * A;
+  // momentum reads Z(k) as its level 2 and the level-1 fluxes; the next
+  // continuity reads Z(k) as its level 1 and the new fluxes
+  Lds<P> sa{pz, pm0, pn0, ph, pz, nullptr, ti0 - 2, tj0 - 2, stab, crow, R};
+  Lds<P> sb{pz, pm, pn, ph, pz, nullptr, ti0 - 2, tj0 - 2, stab, crow, R};
+  const int nx = L.nx, ny = L.ny, tid = threadIdx.x;
+  if (SPH) {
+    for (int idx = tid; idx < kNumCoef * R; idx += kThreads) {
+      const int k = idx / R, j = tj0 - 2 + idx % R;
+      crow[idx] = (IN || (j >= 1 && j <= ny)) ? L.coef[k * ny + (j - 1)] : 0.0f;
+    }
+  }
+  for (int idx = tid; idx < A; idx += kThreads) {
+    const int gi = ti0 - 2 + idx % P, gj = tj0 - 2 + idx / P;
+    const bool in = IN || (gi >= 1 && gi <= nx && gj >= 1 && gj <= ny);
+    const size_t g = in ? (size_t)(gj - 1) * nx + (gi - 1) : 0;
+    ph[idx] = in ? L.h[g] : 0.0f;
+    pz[idx] = in ? L.z1[g] : 0.0f;
+    pm0[idx] = in ? L.m1[g] : 0.0f;
+    pn0[idx] = in ? L.n1[g] : 0.0f;
+  }
+  __syncthreads();
+
+  // momentum of step k on the tile + one cell left and below (MOMT ranges
+  // of the outermost layer: M on I <= NX-1, N on J <= NY-1; 0 elsewhere)
+  {
+    constexpr int CX = TX + 1, CY = TY + 1;
+    for (int idx = tid; idx < CX * CY; idx += kThreads) {
+      const int i = ti0 - 1 + idx % CX, j = tj0 - 1 + idx / CX;
+      float m = 0.0f, n = 0.0f;
+      if (IN || (i >= 1 && i <= nx && j >= 1 && j <= ny)) {
+        if (IN || i <= nx - 1) m = lin_x<SPH, IN>(L, sa, i, j);
+        if (IN || j <= ny - 1) n = lin_y<SPH, IN>(L, sa, i, j);
+        if (L.sponge_cx) {  // SPONGE_LAYER (boundaries.f90:228-239)
+          const float cx = L.sponge_cx[i - 1], cy = L.sponge_cy[j - 1];
+          if (cx > 0.0f || cy > 0.0f) {
+            m = m + cx * L.dt * sa.M(i, j);
+            n = n + cy * L.dt * sa.N(i, j);
+          }
+        }
+      }
+      pm[sa.o(i, j)] = m;
+      pn[sa.o(i, j)] = n;
+    }
+  }
+  __syncthreads();
+
+  int step = 0;
+  if (L.zmax) {
+    step = L.ctl[L.ctl_in] + 1;
+    if (L.advance && t == 0 && tid == 0) L.ctl[1 - L.ctl_in] = step;
+  }
+#pragma unroll 1
+  for (int idx = tid; idx < TX * TY; idx += kThreads) {
+    const int i = ti0 + idx % TX, j = tj0 + idx / TX;
+    if (!IN && (i > nx || j > ny)) continue;
+    const size_t g = (size_t)(j - 1) * nx + (i - 1);
+    L.m2[g] = sb.M(i, j);
+    L.n2[g] = sb.N(i, j);
+    const float zk = sa.Z2(i, j);
+    if (L.zmax && sa.H(i, j) + zk > kGx) {
+      const float zm = L.zmax[g];
+      if (zk >= L.arr_thr && !(zm >= L.arr_thr)) L.arrival[g] = step;
+      if (zk > zm) L.zmax[g] = zk;
+    }
+    if (L.zraw && !(L.zraw[g] >= zk)) L.zraw[g] = zk;
+    float z2, dz2;
+    mass_cell<false, SPH, IN>(L, sb, i, j, z2, dz2);
+    L.z2[g] = z2;
+  }
+}
+
 template <int MODE, bool NL, bool SPH, int TX, int TY>
 __device__ __forceinline__ void tile_step(const DevLayer &L, int t, int tiles_x, float *sm,
                                           const gcdet::Tables *stab) {
   const int ti0 = (t % tiles_x) * TX + 1, tj0 = (t / tiles_x) * TY + 1;
   const bool interior = ti0 - 2 > kSpan + 2 && ti0 + TX + 2 < L.nx - kSpan &&
                         tj0 - 2 > kSpan + 2 && tj0 + TY + 2 < L.ny - kSpan;
-  if (interior) tile_body<MODE, NL, SPH, TX, TY, true>(L, t, ti0, tj0, sm, stab);
-  else tile_body<MODE, NL, SPH, TX, TY, false>(L, t, ti0, tj0, sm, stab);
+  if constexpr (MODE == kMomentMass) {
+    static_assert(!NL, "momentum + next continuity is built for linear layers");
+    if (interior) tile_body_mm<SPH, TX, TY, true>(L, t, ti0, tj0, sm, stab);
+    else tile_body_mm<SPH, TX, TY, false>(L, t, ti0, tj0, sm, stab);
+  } else {
+    if (interior) tile_body<MODE, NL, SPH, TX, TY, true>(L, t, ti0, tj0, sm, stab);
+    else tile_body<MODE, NL, SPH, TX, TY, false>(L, t, ti0, tj0, sm, stab);
+  }
 }
 
 // Waves per SIMD the register allocation of the step kernels aims for.
@@ -1821,6 +1913,11 @@ hipError_t launch_step(const DevLayer &L, int mode, bool nl, bool sph, int varia
     if (nl) return sph ? launch_stream<true, true>(L, seg, st) : launch_stream<true, false>(L, seg, st);
     return sph ? launch_stream<false, true>(L, seg, st) : launch_stream<false, false>(L, seg, st);
   }
+  if (mode == kMomentMass) {  // outermost linear layers with nested layers: 64x8
+    if (nl) return hipErrorInvalidValue;
+    return sph ? launch_t<kMomentMass, false, true, 64, 8>(L, st)
+               : launch_t<kMomentMass, false, false, 64, 8>(L, st);
+  }
   // split continuity / momentum launches (parents of nested layers) use a 64x8 tile
   const int v = mode == kFused ? variant : 0;
   switch (kVariants[v][1] * 1000 + kVariants[v][2]) {
@@ -1978,6 +2075,7 @@ hipError_t launch_batch(const BatchLaunch &B, int mode, bool nl, bool sph, int s
     GC_CASE(kMassOnly, true, true) GC_CASE(kMassOnly, true, false)
     GC_CASE(kMomentOnly, false, true) GC_CASE(kMomentOnly, false, false)
     GC_CASE(kMomentOnly, true, true) GC_CASE(kMomentOnly, true, false)
+    GC_CASE(kMomentMass, false, true) GC_CASE(kMomentMass, false, false)
 #undef GC_CASE
     return hipErrorInvalidValue;
   }

@@ -97,6 +97,7 @@ int ensure_deform(LayerRec &L) {
 }  // namespace
 
 int tree_apply_fault(Tree &T, hipStream_t st, const float *params, int fswitch) {
+  T.mass_ahead = false;  // level 2 is no longer the next continuity
   LayerRec *R = T.layer(1);
   if (!R) return fail(8, "fault: no root layer");
   if (fswitch != 0 && fswitch != 9) return fail(60, "fault: epicentre switch %d not 0 or 9", fswitch);

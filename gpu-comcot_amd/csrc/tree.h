@@ -65,6 +65,12 @@ struct Tree {
   int variant = 0;     // tile variant of the step kernels
   int blocking = 1;    // two steps per launch for linear single layers: 0 off, 1+shape
   int raw_max = 0;     // device loop also keeps MAX_AMP's maximum of the outermost layer
+  // a linear outermost layer with nested layers: its momentum launch also
+  // computes the next root step's continuity when that step follows
+  // (fuse_halves); mass_ahead: level 2 already holds that continuity (a
+  // cache: continuity from level 1 gives the same bits when recomputed)
+  int fuse_halves = 1;
+  bool mass_ahead = false;
   LayerRec *layer(int id) {
     return (id >= 1 && id < kMaxLayers && layers[id].used) ? &layers[id] : nullptr;
   }
@@ -122,7 +128,13 @@ int tree_launch(const Tree &T, const LayerRec &L, int mode, bool with_max, bool 
 // Sub-steps of every child of P for one step of P (ALL_GRID order).
 int tree_enqueue_children(Tree &T, LayerRec &P, bool with_max, hipStream_t st);
 // One complete root step: mass+open, nests, momentum + max, rotation.
-int tree_enqueue_root_step(Tree &T, hipStream_t st);
+// next: another root step follows at once (its continuity may then ride in
+// this step's momentum launch, tree_fuses_halves).
+int tree_enqueue_root_step(Tree &T, hipStream_t st, bool next = false);
+bool tree_fuses_halves(const Tree &T);
+// The view of the launch that ends root step k and starts step k+1 (z1 =
+// Z(k), z2 = the buffer for Z(k+1)).
+DevLayer fused_halves_view(const Tree &T, const LayerRec &R);
 // True when the tree is one linear layer (two steps per launch apply).
 bool tree_step2_eligible(const Tree &T);
 // Two root steps of such a tree in one launch (temporal blocking).

@@ -180,6 +180,7 @@ int tree_create_layer(Tree &T, hipStream_t st, int id, int parent, int level, in
     T.ctl_in = 0;
     HIPCHK(hipMemsetAsync(T.ctl, 0, 2 * sizeof(int), st));
   }
+  T.mass_ahead = false;
   LayerRec &L = T.layers[id];
   L.used = true;
   L.id = id;
@@ -408,15 +409,37 @@ int tree_enqueue_children(Tree &T, LayerRec &P, bool with_max, hipStream_t st) {
 
 // comcot.f90:359-439: continuity + open boundary, nested layers, momentum
 // with the wet max/arrival, level rotation.
-int tree_enqueue_root_step(Tree &T, hipStream_t st) {
+bool tree_fuses_halves(const Tree &T) {
+  const LayerRec &R = T.layers[1];
+  return T.fuse_halves && R.used && !R.children.empty() && !R.nl;
+}
+
+DevLayer fused_halves_view(const Tree &T, const LayerRec &R) {
+  DevLayer d = dev_view(T, R, true, true);
+  d.z1 = R.z[1 - R.cur];  // Z(k): continuity of step k, JNZ
+  d.z2 = R.z[R.cur];      // Z(k-1), which linear momentum does not read
+  return d;
+}
+
+int tree_enqueue_root_step(Tree &T, hipStream_t st, bool next) {
   LayerRec *R = T.layer(1);
   if (!R) return fail(8, "no root layer");
   if (R->children.empty()) {
     if (int r = tree_launch(T, *R, kFused, true, true, st)) return r;
   } else {
-    if (int r = tree_launch(T, *R, kMassOnly, false, false, st)) return r;
+    if (!T.mass_ahead)
+      if (int r = tree_launch(T, *R, kMassOnly, false, false, st)) return r;
+    T.mass_ahead = false;
     if (int r = tree_enqueue_children(T, *R, true, st)) return r;
-    if (int r = tree_launch(T, *R, kMomentOnly, true, true, st)) return r;
+    if (next && tree_fuses_halves(T)) {
+      const DevLayer d = fused_halves_view(T, *R);
+      hipError_t e = launch_step(d, kMomentMass, false, R->sph != 0, T.variant, st);
+      if (e != hipSuccess)
+        return fail(200 + (int)e, "step kernel launch failed: %s", hipGetErrorString(e));
+      T.mass_ahead = true;
+    } else {
+      if (int r = tree_launch(T, *R, kMomentOnly, true, true, st)) return r;
+    }
   }
   R->cur = 1 - R->cur;
   T.ctl_in = 1 - T.ctl_in;
@@ -633,6 +656,7 @@ int tree_checkpoint_save(const Tree &T, hipStream_t st, const char *path, long l
 }
 
 int tree_checkpoint_load(Tree &T, hipStream_t st, const char *path, long long *done) {
+  T.mass_ahead = false;  // level 2 is no longer the next continuity
   File F;
   F.f = std::fopen(path, "rb");
   if (!F.f) return fail(72, "checkpoint: cannot open %s", path);

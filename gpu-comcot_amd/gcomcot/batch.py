@@ -51,13 +51,15 @@ class Batch:
     """The library's batch engine (one per process)."""
 
     def __init__(self, arrival_threshold=0.01, autotune=True, shape=-1, graphs=True,
-                 two_step=True, two_step_shape=None, streams=3):
+                 two_step=True, two_step_shape=None, streams=3, fused_halves=True):
         """two_step: scenarios whose only layer is linear may advance two
         root steps per launch, in tile shape two_step_shape (the library's
         two-step shapes 0-6), or, when None, in the shape the autotuner
         finds fastest if that beats two one-step launches.  streams: 3 =
         single-layer scenarios and two chains of nested scenarios side by
-        side, 2 = single-layer scenarios beside one chain, 1 = one stream."""
+        side, 2 = single-layer scenarios beside one chain, 1 = one stream.
+        fused_halves: the momentum launch of a nested linear root also
+        computes its next continuity (same bits, one launch fewer per step)."""
         L = lib()
         _setup(L)
         L.gcomcot_set_error_mode_(ip(0))
@@ -69,6 +71,8 @@ class Batch:
                                             None if two_step_shape is None else ip(two_step_shape)),
               "batch two-step")
         check(L.gcomcot_batch_set_streams_(ip(streams)), "batch streams")
+        check(L.gcomcot_batch_set_fused_halves_(ip(1 if fused_halves else 0)),
+              "batch fused halves")
         self.specs = {}
 
     def add(self, sc, steps=None):

@@ -112,6 +112,12 @@ class DeviceRun:
                                       ctypes.byref(ms)), "autotune")
         return v.value, b.value, ms.value
 
+    def set_fused_halves(self, on=True):
+        """For a linear outermost layer with nested layers: the momentum
+        launch of each root step but the last of a run also computes the
+        next root step's continuity (on, the default) or not; same bits."""
+        check(lib().gcomcot_set_fused_halves_(ip(1 if on else 0)), "fused halves")
+
     # ------------------------------------------------------------ sources
     def set_coords(self, lids=None):
         """Give the library each layer's X (NX) / Y (NY) coordinates."""

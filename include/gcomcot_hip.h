@@ -359,6 +359,17 @@ int gcomcot_autotune_(int *reps, int *variant, int *blocking, float *ms_per_step
  * valid. */
 int gcomcot_set_temporal_blocking_(int *mode);
 
+/* For an outermost linear layer with nested layers: the momentum launch of
+ * each root step but the last of a gcomcot_run_steps_ / gcomcot_time_steps_
+ * call also computes the next root step's continuity and open boundary
+ * (*on = 1, the default), which the root step order allows (nothing runs
+ * between the two), saving one launch and one pass over the layer per
+ * step; *on = 0: separate continuity and momentum launches.  The same bits
+ * either way; between calls the state is always that of a whole step.
+ * gcomcot_batch_set_fused_halves_ is the same switch for the batch engine.
+ * Returns 0. */
+int gcomcot_set_fused_halves_(int *on);
+
 /* Measurement aid: *reps copies of a *mib MiB buffer with a plain
  * streaming kernel; *ms_per_copy = mean duration (HIP events).  Gives the
  * chip's practical HBM bandwidth and the scale of rocprofv3's memory
@@ -475,6 +486,10 @@ int gcomcot_batch_set_two_step_(int *on, int *shape);
  * scenarios in one chain; *n = 1: everything on one stream.  The same bits
  * every way.  Returns 0, or 45 for another *n. */
 int gcomcot_batch_set_streams_(int *n);
+/* Nested linear roots in the batch: each momentum launch also computes the
+ * next root step's continuity (*on = 1, the default; see
+ * gcomcot_set_fused_halves_).  Same bits.  Returns 0. */
+int gcomcot_batch_set_fused_halves_(int *on);
 int gcomcot_batch_info_(long long *launches, long long *graph_replays, long long *plans,
                         long long *cells_per_step, int *tuned_count);
 

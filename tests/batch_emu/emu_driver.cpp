@@ -111,6 +111,8 @@ int main(int argc, char **argv) {
     if (gcomcot_batch_set_tuning_(&on, &shape)) return 3;
   }
   if (gcomcot_batch_set_two_step_(&two, shape2 >= 0 ? &shape2 : nullptr)) return 3;
+  int fuse = std::getenv("EMU_FUSE") ? std::atoi(std::getenv("EMU_FUSE")) : 1;
+  if (gcomcot_batch_set_fused_halves_(&fuse)) return 3;
   std::map<int, Case> live;
   int checked = 0;
   for (int r = 0; r < rounds; ++r) {
@@ -151,8 +153,8 @@ int main(int argc, char **argv) {
   int tuned = 0;
   gcomcot_batch_info_(&l, &g, &p, &cps, &tuned);
   std::printf("seed %u: %d scenarios verified, shape %d, two-step %d (shape %d), "
-              "%lld cross-stream waits\n", seed, checked, shape, two, shape2,
-              emu_cross_stream_waits());
+              "%lld cross-stream waits, fused halves %d (%lld tiles)\n", seed, checked, shape,
+              two, shape2, emu_cross_stream_waits(), fuse, emu_fused_tiles());
   (void)zero;
   return 0;
 }

@@ -30,6 +30,8 @@ namespace {
 // kernels.hip kBatchShapes; shapes 3-5 are the same tiles in the other launch form
 const int kShapes[][2] = {{64, 8}, {32, 16}, {64, 16}};
 
+long long g_fused_tiles = 0;  // tiles of momentum + next continuity launches
+
 void tile_op(const DevLayer &d, int mode, int t, int tiles_x, int TX, int TY,
              std::vector<unsigned char> &cov) {
   const int i0 = (t % tiles_x) * TX, j0 = (t / tiles_x) * TY;
@@ -37,6 +39,16 @@ void tile_op(const DevLayer &d, int mode, int t, int tiles_x, int TX, int TY,
     for (int i = i0; i < i0 + TX && i < d.nx; ++i) {
       const size_t c = (size_t)j * d.nx + i;
       cov[c]++;
+      if (mode == kMomentMass) {  // z1 = Z(k), z2 = the buffer of Z(k-1), then Z(k+1)
+        if (i == i0 && j == j0) ++g_fused_tiles;
+        if (d.z1[c] != d.z2[c] + 1.0f) die("fused momentum before continuity");
+        if (!d.is_root) die("fused halves of a nested layer");
+        d.m2[c] = d.m1[c] + 1.0f;
+        d.n2[c] = d.n1[c] + 1.0f;
+        if (d.zmax) d.zmax[c] += 1.0f;
+        d.z2[c] = d.z1[c] + 1.0f;  // the next root step's continuity
+        continue;
+      }
       if (mode != kMomentOnly) d.z2[c] = d.z1[c] + 1.0f;
       if (mode == kMomentOnly && d.z2[c] != d.z1[c] + 1.0f) die("momentum before continuity");
       if (mode != kMassOnly) {
@@ -297,3 +309,5 @@ hipError_t launch_fill_i32(int *p, int n, int v, hipStream_t st) {
 }
 
 }  // namespace gc
+
+long long emu_fused_tiles() { return g_fused_tiles; }

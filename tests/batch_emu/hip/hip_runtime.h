@@ -61,3 +61,5 @@ void emu_submit(hipStream_t st, void (*fn)(void *), void *arg, size_t arg_bytes)
 long long emu_live_bytes();
 // hipStreamWaitEvent calls on an event of another stream (forks and joins)
 long long emu_cross_stream_waits();
+// tiles run by momentum + next continuity launches
+long long emu_fused_tiles();

@@ -10,8 +10,9 @@ chunks, adds scenarios while others run and removes finished ones, and
 checks that every scenario ends with exactly its own number of root steps
 (REL_TIME times that in its child), continuity before momentum, JNQ/JNZ in
 the right sub-steps, tiles covering each layer once per launch, queue heads
-cleared, forked streams joined back before a capture ends, and no device
-memory left after a reset.
+cleared, forked streams joined back before a capture ends, a fused momentum +
+next continuity launch only where that step's continuity is done, and no
+device memory left after a reset.
 """
 import os
 import shutil
@@ -80,3 +81,20 @@ def test_tuned_two_step_choice(driver, seed):
     assert r.returncode == 0, r.stderr[-2000:]
     assert "two-step 1 (shape -1)" in r.stdout
 
+
+@pytest.mark.parametrize("fuse", [1, 0])
+@pytest.mark.parametrize("seed", [51, 52])
+def test_fused_halves(driver, seed, fuse):
+    """Nested linear roots whose momentum launch also computes the next root
+    step's continuity: each such launch finds the continuity of its own
+    step done (prologue or the previous fused launch), scenarios added
+    later start without it, and every scenario ends on its own step count;
+    switched off, none is fused."""
+    import re
+    env = dict(os.environ, EMU_FUSE=str(fuse), ASAN_OPTIONS="detect_leaks=1")
+    r = subprocess.run([driver, str(seed), "25"], capture_output=True, text=True, env=env,
+                       timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    m = re.search(r"fused halves (\d) \((\d+) tiles\)", r.stdout)
+    assert m and int(m.group(1)) == fuse, r.stdout
+    assert (int(m.group(2)) > 0) == (fuse == 1), r.stdout

@@ -41,18 +41,20 @@ def _check_vs_oracle(b, sid, sc, orc, tag):
         assert np.array_equal(gm["arrival"], om["arrival"]), f"{tag} layer {L.id} arrival"
 
 
-@pytest.mark.parametrize("two_step,graphs,streams",
-                         [(True, True, 3), (False, True, 3), (True, False, 3), (True, True, 2),
-                          (True, True, 1)],
+@pytest.mark.parametrize("two_step,graphs,streams,fused",
+                         [(True, True, 3, True), (False, True, 3, True), (True, False, 3, True),
+                          (True, True, 2, True), (True, True, 1, True), (True, True, 3, False),
+                          (True, False, 1, False)],
                          ids=["two-step", "one-step", "two-step-no-graphs", "two-streams",
-                              "one-stream"])
-def test_mixed_batch_matches_oracle(two_step, graphs, streams):
+                              "one-stream", "separate-halves", "separate-halves-no-graphs"])
+def test_mixed_batch_matches_oracle(two_step, graphs, streams, fused):
     """Single-layer and nested scenarios together; with more than one stream
-    the single-layer ones run beside the nested chain(s)."""
+    the single-layer ones run beside the nested chain(s); nested linear roots
+    with their momentum launch computing the next continuity or not."""
     steps = [s + (i % 3) for i, (_, _, s) in enumerate(CASES)]  # odd and even counts
     print(f"[gpu] batch of {len(CASES)} scenarios, steps {steps}, two_step={two_step}, "
-          f"graphs={graphs}, streams={streams}", flush=True)
-    b = Batch(two_step=two_step, graphs=graphs, streams=streams)
+          f"graphs={graphs}, streams={streams}, fused_halves={fused}", flush=True)
+    b = Batch(two_step=two_step, graphs=graphs, streams=streams, fused_halves=fused)
     sids = [b.add(sc, n) for (_, sc, _), n in zip(CASES, steps)]
     assert b.run(10 ** 6) == len(CASES)
     for (name, sc, _), sid, n in zip(CASES, sids, steps):

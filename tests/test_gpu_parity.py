@@ -57,6 +57,34 @@ def test_every_tile_shape(name, sc, steps):
         lib().gcomcot_set_tile_variant_(ip(0))
 
 
+NESTS = [c for c in CASES if c[0].startswith("nest")]
+
+
+@pytest.mark.parametrize("name,sc,steps", NESTS, ids=[c[0] for c in NESTS])
+@pytest.mark.parametrize("fused", [True, False], ids=["fused-halves", "separate-halves"])
+def test_nest_root_halves_fused_or_separate(name, sc, steps, fused):
+    """A linear root with nested layers, its momentum launch also computing
+    the next root step's continuity or not, over calls of 1, 2, 7 and 8
+    steps with and without graphs, timed steps and a drop-in step between
+    them: every call ends on a whole step, so the mix lands on the oracle."""
+    orc = Oracle(sc, 32, pow_mode=1)
+    orc.step(37)
+    dev = DeviceRun(sc)
+    dev.set_fused_halves(fused)
+    try:
+        for n, graph in [(1, True), (2, False), (7, True), (8, True), (3, False)]:
+            print(f"[gpu] {name} fused={fused} run {n} graph={graph}", flush=True)
+            dev.run(n, graph=graph)
+        print(f"[gpu] {name} fused={fused} timed 5 + drop-in 1 + run 10", flush=True)
+        dev.timed(5)
+        dev.dropin_step()
+        dev.run(10, graph=True)
+        _compare(sc, dev, orc, f"{name} fused={fused}")
+    finally:
+        dev.set_fused_halves(True)
+        dev.close()
+
+
 def test_reference_entry_points_match_oracle():
     """The Fortran driver's call sequence through the reference symbols,
     with layers registered by gcomcot_init_layer_ from full PARA planes."""
