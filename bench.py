@@ -71,6 +71,8 @@ def parse():
     ap.add_argument("--batch-cases", type=int, default=64)
     ap.add_argument("--batch-two-step", type=int, default=1,
                     help="single-layer linear roots take two root steps per launch (0 = one)")
+    ap.add_argument("--batch-streams", type=int, default=3,
+                    help="streams of a batch plan (1-3; 1 makes kernel times additive in a trace)")
     ap.add_argument("--dropin-sample", type=int, default=100,
                     help="steps per scenario of the drop-in call-sequence sample")
     ap.add_argument("--serve", type=int, default=1, help="run the serving streams (0 = skip)")
@@ -359,7 +361,7 @@ def bench_batch(a):
     upd = [sum(L.nx * L.ny * (L.rel_time if L.id > 1 else 1) for L in c.layers) * c.steps
            for c in cs]
     total = float(sum(upd))
-    b = Batch(graphs=bool(a.graphs), two_step=bool(a.batch_two_step))
+    b = Batch(graphs=bool(a.graphs), two_step=bool(a.batch_two_step), streams=a.batch_streams)
     for c in cs:
         b.add(c)
     log("batch: running the batch engine")
