@@ -193,6 +193,12 @@ module gcomcot_hip
       integer(c_int) :: mode
     end function
 
+    integer(c_int) function gcomcot_set_fused_halves(on) &
+        bind(C, name="gcomcot_set_fused_halves_")
+      import :: c_int
+      integer(c_int) :: on
+    end function
+
     ! Exact checkpoints of the device state
     integer(c_int) function gcomcot_checkpoint_save(path, pathlen) &
         bind(C, name="gcomcot_checkpoint_save_")
@@ -359,6 +365,12 @@ module gcomcot_hip
     integer(c_int) function gcomcot_batch_remove(sid) bind(C, name="gcomcot_batch_remove_")
       import :: c_int
       integer(c_int) :: sid
+    end function
+
+    integer(c_int) function gcomcot_batch_set_fused_halves(on) &
+        bind(C, name="gcomcot_batch_set_fused_halves_")
+      import :: c_int
+      integer(c_int) :: on
     end function
   end interface
 end module gcomcot_hip
