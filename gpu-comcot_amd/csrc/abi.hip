@@ -954,6 +954,13 @@ int gcomcot_set_fused_halves_(int *on) {
   return 0;
 }
 
+int gcomcot_set_store_policy_(int *mode) {
+  if (*mode < 0 || *mode > 2) return fail(46, "store policy %d is not 0, 1 or 2", *mode);
+  if (sync()) return 1;  // launches in flight keep the policy they were built with
+  g_store_policy = *mode;
+  return 0;
+}
+
 int gcomcot_set_temporal_blocking_(int *mode) {
   if (*mode < 0 || *mode > kStep2Shapes) return fail(28, "temporal blocking mode %d out of range", *mode);
   g_tree.blocking = *mode;

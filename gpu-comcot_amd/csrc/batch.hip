@@ -126,6 +126,7 @@ struct Engine {
   std::map<long long, int> tuned2;  // size class of the two-step roots -> shape or -1
   Plan plan;
   bool plan_valid = false;
+  int plan_policy = -1;    // the store policy its launches were built with
   char *arena = nullptr;   // descriptor/item memory reused by successive plans
   size_t arena_bytes = 0;
   long long launches = 0, graph_replays = 0, plans_built = 0;
@@ -666,6 +667,7 @@ bool any_two_step(const std::vector<Scenario *> &act) {
 
 int build_plan(const std::vector<Scenario *> &act) {
   invalidate();
+  g_eng.plan_policy = g_store_policy;
   if (g_eng.autotune && g_eng.forced_shape < 0)
     if (int r = tune(act)) return r;
   // nested linear roots without their next continuity in level 2 get it
@@ -742,6 +744,7 @@ int run(long long n) {
     if (act.empty()) break;
     long long m = n;
     for (Scenario *s : act) m = std::min(m, s->total - s->done);
+    if (g_eng.plan_valid && g_eng.plan_policy != g_store_policy) invalidate();
     if (trace_on())
       std::fprintf(stderr, "batch: %zu active, epoch of %lld steps, plan %s\n", act.size(), m,
                    g_eng.plan_valid ? "kept" : "rebuilt");

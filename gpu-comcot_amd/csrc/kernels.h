@@ -35,6 +35,13 @@ struct DevLayer {
   // SPONGE_LAYER's coefficients COEFX per column I and COEFY per row J
   // (boundaries.f90:206-246), applied after the momentum pass when != null
   const float *sponge_cx, *sponge_cy;
+  // 1: the single-layer step kernels (launch_step, launch_step2) write the
+  // new level-2 planes Z2, M2, N2 of interior tiles with nontemporal
+  // stores.  For a layer whose one step moves more bytes than the chip's
+  // last-level cache holds, the next step finds none of them in a cache
+  // anyway, and streaming them keeps the depth and maximum planes there.
+  // The batch kernels always store through the caches.
+  int stream_out;
 };
 
 // kMomentMass: the momentum half of root step k of a linear outermost

@@ -30,6 +30,21 @@ namespace gc {
 
 namespace {
 
+// Store of a new level-2 plane value; ST: streamed past the caches
+// (DevLayer::stream_out).  ST is a template argument of the single-layer
+// kernels (k_step, k_step2), chosen at launch: a run-time test around the
+// store lets the compiler merge the two stores and drop the hint, and a
+// third tile body in every kernel costs the register allocation of the
+// others (more spills).  Edge tiles and the batch kernels store through
+// the caches.
+template <bool ST>
+__device__ __forceinline__ void st_plane(float *p, float v) {
+  if constexpr (ST)
+    __builtin_nontemporal_store(v, p);
+  else
+    *p = v;
+}
+
 constexpr int kThreads = 256;
 
 template <int P>
@@ -521,7 +536,7 @@ __device__ __forceinline__ int tile_of_block(int per_xcd) {
 // publishes them).  IN: the tile and its halo lie more than SPAN+2 cells
 // inside the grid, so no index clamp, boundary ring, ghost row or range
 // limit of the Fortran loops can apply and all of them compile away.
-template <int MODE, bool NL, bool SPH, int TX, int TY, bool IN>
+template <int MODE, bool NL, bool SPH, int TX, int TY, bool IN, bool ST>
 __device__ __forceinline__ void tile_body(const DevLayer &L, int t, int ti0, int tj0, float *sm,
                                           const gcdet::Tables *stab) {
   constexpr int P = TX + 4, R = TY + 4, A = P * R;
@@ -607,7 +622,7 @@ __device__ __forceinline__ void tile_body(const DevLayer &L, int t, int ti0, int
       else if (i == 1 && j == 1) zo = s.Z2(2, 2);  // UPDATE ghosts, all_grids.f90:323-324
       else if (i == 1) zo = s.Z2(2, j);
       else zo = s.Z2(i, 2);
-      L.z2[g] = zo;
+      st_plane<ST>(L.z2 + g, zo);
       if (MODE == kMassOnly && NL && L.dzp) L.dzp[g] = s.DZ(i, j);
     }
     if (MODE != kMassOnly) {
@@ -640,8 +655,8 @@ __device__ __forceinline__ void tile_body(const DevLayer &L, int t, int ti0, int
           n2v = n2v + cy * L.dt * s.N(i, j);
         }
       }
-      L.m2[g] = m2v;
-      L.n2[g] = n2v;
+      st_plane<ST>(L.m2 + g, m2v);
+      st_plane<ST>(L.n2 + g, n2v);
       if (L.zmax && (IN || r || (i >= 2 && j >= 2))) {
         const float z2 = s.Z2(i, j);
         if (s.H(i, j) + z2 > kGx) {
@@ -667,7 +682,7 @@ __device__ __forceinline__ void tile_body(const DevLayer &L, int t, int ti0, int
 // the divergence read the fluxes of the cell and of its left and lower
 // neighbours only.  Each value comes from the same functions and operands
 // as in the momentum-only and continuity-only launches.
-template <bool SPH, int TX, int TY, bool IN>
+template <bool SPH, int TX, int TY, bool IN, bool ST>
 __device__ __forceinline__ void tile_body_mm(const DevLayer &L, int t, int ti0, int tj0,
                                              float *sm, const gcdet::Tables *stab) {
   constexpr int P = TX + 4, R = TY + 4, A = P * R;
@@ -729,8 +744,8 @@ __device__ __forceinline__ void tile_body_mm(const DevLayer &L, int t, int ti0, 
     const int i = ti0 + idx % TX, j = tj0 + idx / TX;
     if (!IN && (i > nx || j > ny)) continue;
     const size_t g = (size_t)(j - 1) * nx + (i - 1);
-    L.m2[g] = sb.M(i, j);
-    L.n2[g] = sb.N(i, j);
+    st_plane<ST>(L.m2 + g, sb.M(i, j));
+    st_plane<ST>(L.n2 + g, sb.N(i, j));
     const float zk = sa.Z2(i, j);
     if (L.zmax && sa.H(i, j) + zk > kGx) {
       const float zm = L.zmax[g];
@@ -740,11 +755,11 @@ __device__ __forceinline__ void tile_body_mm(const DevLayer &L, int t, int ti0, 
     if (L.zraw && !(L.zraw[g] >= zk)) L.zraw[g] = zk;
     float z2, dz2;
     mass_cell<false, SPH, IN>(L, sb, i, j, z2, dz2);
-    L.z2[g] = z2;
+    st_plane<ST>(L.z2 + g, z2);
   }
 }
 
-template <int MODE, bool NL, bool SPH, int TX, int TY>
+template <int MODE, bool NL, bool SPH, int TX, int TY, bool ST = false>
 __device__ __forceinline__ void tile_step(const DevLayer &L, int t, int tiles_x, float *sm,
                                           const gcdet::Tables *stab) {
   const int ti0 = (t % tiles_x) * TX + 1, tj0 = (t / tiles_x) * TY + 1;
@@ -752,11 +767,11 @@ __device__ __forceinline__ void tile_step(const DevLayer &L, int t, int tiles_x,
                         tj0 - 2 > kSpan + 2 && tj0 + TY + 2 < L.ny - kSpan;
   if constexpr (MODE == kMomentMass) {
     static_assert(!NL, "momentum + next continuity is built for linear layers");
-    if (interior) tile_body_mm<SPH, TX, TY, true>(L, t, ti0, tj0, sm, stab);
-    else tile_body_mm<SPH, TX, TY, false>(L, t, ti0, tj0, sm, stab);
+    if (interior) tile_body_mm<SPH, TX, TY, true, ST>(L, t, ti0, tj0, sm, stab);
+    else tile_body_mm<SPH, TX, TY, false, false>(L, t, ti0, tj0, sm, stab);
   } else {
-    if (interior) tile_body<MODE, NL, SPH, TX, TY, true>(L, t, ti0, tj0, sm, stab);
-    else tile_body<MODE, NL, SPH, TX, TY, false>(L, t, ti0, tj0, sm, stab);
+    if (interior) tile_body<MODE, NL, SPH, TX, TY, true, ST>(L, t, ti0, tj0, sm, stab);
+    else tile_body<MODE, NL, SPH, TX, TY, false, false>(L, t, ti0, tj0, sm, stab);
   }
 }
 
@@ -790,7 +805,7 @@ constexpr int tile_lds_bytes() {
                sizeof(gcdet::Tables) + 2 * sizeof(int));
 }
 
-template <int MODE, bool NL, bool SPH, int TX, int TY>
+template <int MODE, bool NL, bool SPH, int TX, int TY, bool ST>
 __global__ void __launch_bounds__(kThreads)
     __attribute__((amdgpu_waves_per_eu(StepWaves<NL, SPH, tile_lds_bytes<TX, TY>()>::value)))
     k_step(DevLayer L, int tiles_x, int ntiles, int per_xcd) {
@@ -799,7 +814,7 @@ __global__ void __launch_bounds__(kThreads)
   const int t = tile_of_block(per_xcd);
   if (t >= ntiles) return;
   if (NL || !SPH) stage_tables(stab, threadIdx.x);
-  tile_step<MODE, NL, SPH, TX, TY>(L, t, tiles_x, sm, &stab);
+  tile_step<MODE, NL, SPH, TX, TY, ST>(L, t, tiles_x, sm, &stab);
 }
 
 // ------------------------------------------------ two steps per launch
@@ -813,7 +828,7 @@ __global__ void __launch_bounds__(kThreads)
 // bitwise the same; HBM traffic and launches per step halve, at the cost
 // of recomputing the halo ring (cheap for the linear equations).  The wet
 // maximum and arrival see both steps in order.
-template <bool SPH, int TX, int TY, bool IN, int NT>
+template <bool SPH, int TX, int TY, bool IN, int NT, bool ST>
 __device__ __forceinline__ void tile_body2(const DevLayer &L, int t, int ti0, int tj0,
                                            float *sm, const gcdet::Tables *stab) {
   constexpr int HW = 4, P = TX + 2 * HW, R = TY + 2 * HW, A = P * R;
@@ -896,11 +911,11 @@ __device__ __forceinline__ void tile_body2(const DevLayer &L, int t, int ti0, in
     if (!IN && (i > nx || j > ny)) continue;
     const size_t g = (size_t)(j - 1) * nx + (i - 1);
     const float za = pz1[s1.o(i, j)], zb = pz0[s2.o(i, j)];
-    L.z2[g] = zb;
+    st_plane<ST>(L.z2 + g, zb);
     const bool mx = IN || (i <= nx - 1);
     const bool my = IN || (j <= ny - 1);
-    L.m2[g] = mx ? lin_x<SPH, IN>(L, s2, i, j) : 0.0f;
-    L.n2[g] = my ? lin_y<SPH, IN>(L, s2, i, j) : 0.0f;
+    st_plane<ST>(L.m2 + g, mx ? lin_x<SPH, IN>(L, s2, i, j) : 0.0f);
+    st_plane<ST>(L.n2 + g, my ? lin_y<SPH, IN>(L, s2, i, j) : 0.0f);
     if (L.zmax) {
       const float h = ph[s1.o(i, j)];
       const bool wa = h + za > kGx, wb = h + zb > kGx;
@@ -934,7 +949,7 @@ __device__ __forceinline__ void tile_body2(const DevLayer &L, int t, int ti0, in
 // CU, and more threads on the same LDS tile shorten each phase's chain of
 // dependent LDS and memory accesses there; the arithmetic of every cell is
 // the same whatever NT is.
-template <bool SPH, int TX, int TY, int NT>
+template <bool SPH, int TX, int TY, int NT, bool ST>
 __global__ void __launch_bounds__(NT)
     k_step2(DevLayer L, int tiles_x, int ntiles, int per_xcd) {
   constexpr int HW = 4;
@@ -946,8 +961,8 @@ __global__ void __launch_bounds__(NT)
   const int ti0 = (t % tiles_x) * TX + 1, tj0 = (t / tiles_x) * TY + 1;
   const bool interior = ti0 - HW > kSpan + 2 && ti0 + TX + HW < L.nx - kSpan &&
                         tj0 - HW > kSpan + 2 && tj0 + TY + HW < L.ny - kSpan;
-  if (interior) tile_body2<SPH, TX, TY, true, NT>(L, t, ti0, tj0, sm, &stab);
-  else tile_body2<SPH, TX, TY, false, NT>(L, t, ti0, tj0, sm, &stab);
+  if (interior) tile_body2<SPH, TX, TY, true, NT, ST>(L, t, ti0, tj0, sm, &stab);
+  else tile_body2<SPH, TX, TY, false, NT, false>(L, t, ti0, tj0, sm, &stab);
 }
 
 // ---------------------------------------------------------- strip kernel
@@ -1067,7 +1082,7 @@ __global__ void __launch_bounds__(kThreads)
       else if (i == 1 && j == 1) zo = r.Z2(2, 2);  // UPDATE ghosts, all_grids.f90:323-324
       else if (i == 1) zo = r.Z2(2, j);
       else zo = r.Z2(i, 2);
-      L.z2[g] = zo;
+      st_plane<false>(L.z2 + g, zo);
       r.z1own = zq0;
       float m2v, n2v;
       if (NL) {
@@ -1090,8 +1105,8 @@ __global__ void __launch_bounds__(kThreads)
           n2v = n2v + cy * L.dt * r.N(i, j);
         }
       }
-      L.m2[g] = m2v;
-      L.n2[g] = n2v;
+      st_plane<false>(L.m2 + g, m2v);
+      st_plane<false>(L.n2 + g, n2v);
       if (L.zmax && (rt || (i >= 2 && j >= 2))) {
         const float z2 = r.Z2(i, j);
         if (r.H(i, j) + z2 > kGx) {
@@ -1855,16 +1870,20 @@ __global__ void __launch_bounds__(NT)
   const int ti0 = (t % tiles_x) * TX + 1, tj0 = (t / tiles_x) * TY + 1;
   const bool interior = ti0 - HW > kSpan + 2 && ti0 + TX + HW < L.nx - kSpan &&
                         tj0 - HW > kSpan + 2 && tj0 + TY + HW < L.ny - kSpan;
-  if (interior) tile_body2<SPH, TX, TY, true, NT>(L, t, ti0, tj0, sm, &stab);
-  else tile_body2<SPH, TX, TY, false, NT>(L, t, ti0, tj0, sm, &stab);
+  if (interior) tile_body2<SPH, TX, TY, true, NT, false>(L, t, ti0, tj0, sm, &stab);
+  else tile_body2<SPH, TX, TY, false, NT, false>(L, t, ti0, tj0, sm, &stab);
 }
 
 template <int MODE, bool NL, bool SPH, int TX, int TY>
 hipError_t launch_t(const DevLayer &L, hipStream_t st) {
   const int tx = (L.nx + TX - 1) / TX, ty = (L.ny + TY - 1) / TY;
   const int nt = tx * ty, per = (nt + 7) / 8;
-  hipLaunchKernelGGL((k_step<MODE, NL, SPH, TX, TY>), dim3(8 * per), dim3(kThreads), 0, st, L,
-                     tx, nt, per);
+  if (L.stream_out)
+    hipLaunchKernelGGL((k_step<MODE, NL, SPH, TX, TY, true>), dim3(8 * per), dim3(kThreads), 0,
+                       st, L, tx, nt, per);
+  else
+    hipLaunchKernelGGL((k_step<MODE, NL, SPH, TX, TY, false>), dim3(8 * per), dim3(kThreads), 0,
+                       st, L, tx, nt, per);
   return hipGetLastError();
 }
 
@@ -1886,8 +1905,12 @@ template <bool SPH, int TX, int TY, int NT = kThreads>
 hipError_t launch2_t(const DevLayer &L, hipStream_t st) {
   const int tx = (L.nx + TX - 1) / TX, ty = (L.ny + TY - 1) / TY;
   const int nt = tx * ty, per = (nt + 7) / 8;
-  hipLaunchKernelGGL((k_step2<SPH, TX, TY, NT>), dim3(8 * per), dim3(NT), 0, st, L, tx, nt,
-                     per);
+  if (L.stream_out)
+    hipLaunchKernelGGL((k_step2<SPH, TX, TY, NT, true>), dim3(8 * per), dim3(NT), 0, st, L, tx,
+                       nt, per);
+  else
+    hipLaunchKernelGGL((k_step2<SPH, TX, TY, NT, false>), dim3(8 * per), dim3(NT), 0, st, L, tx,
+                       nt, per);
   return hipGetLastError();
 }
 

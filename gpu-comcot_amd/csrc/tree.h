@@ -91,6 +91,15 @@ int take_last_code();
                         __LINE__, hipGetErrorString(e_));                        \
   } while (0)
 
+// How step launches store the new level-2 planes (DevLayer::stream_out):
+// 0 through the caches, 2 streamed (nontemporal) for every layer, 1 (the
+// default) streamed for layers of more than kStreamCells cells, whose one
+// step (about 36 bytes per cell) moves more than three quarters of the
+// 256 MB last-level cache.  The bits are the same every way.
+extern int g_store_policy;
+constexpr long long kStreamCells = (192ll << 20) / 36;
+bool streams_planes(const LayerRec &L);
+
 // Bytes of device memory a layer of nx x ny cells takes (planes + tables).
 size_t layer_bytes(int nx, int ny, bool nl, bool r24);
 

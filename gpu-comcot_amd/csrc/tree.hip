@@ -265,6 +265,13 @@ int tree_alloc_r24(LayerRec &L, const float *R2, const float *R4) {
 }
 
 // ---------------------------------------------------------- device views
+int g_store_policy = 1;
+
+bool streams_planes(const LayerRec &L) {
+  if (g_store_policy != 1) return g_store_policy == 2;
+  return (long long)L.nx * L.ny > kStreamCells;
+}
+
 DevLayer dev_view(const Tree &T, const LayerRec &L, bool with_max, bool advance) {
   DevLayer d{};
   d.nx = L.nx;
@@ -298,6 +305,7 @@ DevLayer dev_view(const Tree &T, const LayerRec &L, bool with_max, bool advance)
   d.zraw = (T.raw_max && with_max && advance && L.id == 1) ? L.zmax_raw : nullptr;
   d.sponge_cx = (L.sponge && L.id == 1) ? L.sponge : nullptr;
   d.sponge_cy = d.sponge_cx ? L.sponge + L.nx : nullptr;
+  d.stream_out = streams_planes(L) ? 1 : 0;
   return d;
 }
 

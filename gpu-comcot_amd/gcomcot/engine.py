@@ -112,6 +112,12 @@ class DeviceRun:
                                       ctypes.byref(ms)), "autotune")
         return v.value, b.value, ms.value
 
+    def set_store_policy(self, mode=1):
+        """How step launches store the new time level (process-wide): 1 =
+        streamed past the caches for layers over ~5.6 M cells, 0 = always
+        through the caches, 2 = always streamed; same bits."""
+        check(lib().gcomcot_set_store_policy_(ip(mode)), "store policy")
+
     def set_fused_halves(self, on=True):
         """For a linear outermost layer with nested layers: the momentum
         launch of each root step but the last of a run also computes the

@@ -370,6 +370,17 @@ int gcomcot_set_temporal_blocking_(int *mode);
  * Returns 0. */
 int gcomcot_set_fused_halves_(int *on);
 
+/* How the step launches of the device loop, the drop-in entry points and
+ * the batch engine store the new time level: *mode = 1 (the default)
+ * streams it past the caches (nontemporal stores) for layers of more than
+ * about 5.6 M cells, whose one step moves more bytes than the 256 MB
+ * last-level cache holds, and stores it through the caches for smaller
+ * layers, whose next step or next phase finds it there; 0 = always
+ * through the caches, 2 = always streamed.  The same bits every way.
+ * The batch engine rebuilds its plan at its next run call.  Returns 0,
+ * or 46 for another *mode. */
+int gcomcot_set_store_policy_(int *mode);
+
 /* Measurement aid: *reps copies of a *mib MiB buffer with a plain
  * streaming kernel; *ms_per_copy = mean duration (HIP events).  Gives the
  * chip's practical HBM bandwidth and the scale of rocprofv3's memory

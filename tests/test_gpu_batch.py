@@ -66,6 +66,27 @@ def test_mixed_batch_matches_oracle(two_step, graphs, streams, fused):
     b.close()
 
 
+def test_streamed_stores_in_the_batch():
+    """The batch with every layer's new time level streamed past the caches,
+    switched on after the first plan was built (the plan is rebuilt)."""
+    from gcomcot._lib import lib, ip
+    steps = [s + (i % 3) for i, (_, _, s) in enumerate(CASES)]
+    print(f"[gpu] batch with streamed stores, steps {steps}", flush=True)
+    b = Batch()
+    sids = [b.add(sc, n) for (_, sc, _), n in zip(CASES, steps)]
+    b.run(7)
+    plans = b.info()["plans"]
+    assert lib().gcomcot_set_store_policy_(ip(2)) == 0
+    try:
+        assert b.run(10 ** 6) == len(CASES)
+        assert b.info()["plans"] > plans
+    finally:
+        lib().gcomcot_set_store_policy_(ip(1))
+    for (name, sc, _), sid, n in zip(CASES, sids, steps):
+        _check_vs_oracle(b, sid, sc, _oracle(sc, n), f"{name} streamed")
+    b.close()
+
+
 @pytest.mark.parametrize("shape", range(7))
 def test_two_step_shapes_in_the_batch(shape):
     """Every two-step tile shape on single-layer linear roots, with step

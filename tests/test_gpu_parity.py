@@ -85,6 +85,31 @@ def test_nest_root_halves_fused_or_separate(name, sc, steps, fused):
         dev.close()
 
 
+@pytest.mark.parametrize("name,sc,steps", [CASES[0], CASES[2], CASES[3], CASES[5], CASES[6]],
+                         ids=[c[0] for c in (CASES[0], CASES[2], CASES[3], CASES[5], CASES[6])])
+def test_streamed_time_level_stores(name, sc, steps):
+    """Every layer's new time level written with nontemporal stores (what
+    layers over ~5.6 M cells do by default), through the device loop with
+    graphs (two steps per launch on the linear single layer), the
+    momentum + next continuity launches of the nests and the drop-in entry
+    points: the oracle's bits."""
+    from gcomcot._lib import GcomcotError
+    orc = Oracle(sc, 32, pow_mode=1)
+    orc.step(steps)
+    dev = DeviceRun(sc)
+    with pytest.raises(GcomcotError, match="code 46"):
+        dev.set_store_policy(3)
+    dev.set_store_policy(2)
+    try:
+        print(f"[gpu] {name} streamed stores", flush=True)
+        dev.run(steps - 1, graph=True)
+        dev.dropin_step()
+        _compare(sc, dev, orc, f"{name} streamed")
+    finally:
+        dev.set_store_policy(1)
+        dev.close()
+
+
 def test_reference_entry_points_match_oracle():
     """The Fortran driver's call sequence through the reference symbols,
     with layers registered by gcomcot_init_layer_ from full PARA planes."""
