@@ -517,8 +517,13 @@ template <int NT = kThreads>
 __device__ __forceinline__ void stage_tables(gcdet::Tables &dst, int tid) {
   const double *src = reinterpret_cast<const double *>(&gcdet::kTab);
   double *d = reinterpret_cast<double *>(&dst);
-  constexpr int n = sizeof(gcdet::Tables) / sizeof(double);
-  for (int k = tid; k < n; k += NT) d[k] = src[k];
+  constexpr int n = sizeof(gcdet::Tables) / sizeof(double), K = (n + NT - 1) / NT;
+  double v[K];  // all loads in flight before the first LDS write
+#pragma unroll
+  for (int q = 0; q < K; ++q) v[q] = tid + q * NT < n ? src[tid + q * NT] : 0.0;
+#pragma unroll
+  for (int q = 0; q < K; ++q)
+    if (tid + q * NT < n) d[tid + q * NT] = v[q];
 }
 
 // Tile -> workgroup order: workgroup b runs on XCD b%8; XCD x gets the
@@ -544,45 +549,70 @@ __device__ __forceinline__ void tile_body(const DevLayer &L, int t, int ti0, int
            tj0 - 2, stab, sm + 6 * A, R};
   const int nx = L.nx, ny = L.ny, tid = threadIdx.x;
 
-  // ---- per-row coefficients of the tile's rows (spherical layers)
-  if (SPH) {
-    float *crow = sm + 6 * A;
-    for (int idx = tid; idx < kNumCoef * R; idx += kThreads) {
-      const int k = idx / R, j = tj0 - 2 + idx % R;
-      crow[idx] = (IN || (j >= 1 && j <= ny)) ? L.coef[k * ny + (j - 1)] : 0.0f;
+  // ---- per-row coefficients of the tile's rows (spherical layers) and
+  // the inputs with a two-cell halo: every load of the thread is issued
+  // before its first LDS write, so the tile waits for memory once rather
+  // than once per pass of the thread over the staged rectangle
+  {
+    constexpr int KS = (A + kThreads - 1) / kThreads;
+    constexpr int KC = SPH ? (kNumCoef * R + kThreads - 1) / kThreads : 0;
+    constexpr bool Z1 = MODE != kMomentOnly || NL, Z2 = MODE == kMomentOnly;
+    float vh[KS], vm[KS], vn[KS], vz1[KS], vz2[KS], vdz[KS], vc[KC > 0 ? KC : 1];
+#pragma unroll
+    for (int k = 0; k < KC; ++k) {
+      const int idx = tid + k * kThreads, c = idx / R, j = tj0 - 2 + idx % R;
+      vc[k] = idx < kNumCoef * R && (IN || (j >= 1 && j <= ny)) ? L.coef[c * ny + (j - 1)] : 0.0f;
     }
-  }
-
-  // ---- stage inputs with a two-cell halo
-  for (int idx = tid; idx < A; idx += kThreads) {
-    const int gi = ti0 - 2 + idx % P, gj = tj0 - 2 + idx / P;
-    const bool in = IN || (gi >= 1 && gi <= nx && gj >= 1 && gj <= ny);
-    const size_t g = in ? (size_t)(gj - 1) * nx + (gi - 1) : 0;
-    const float h = in ? L.h[g] : 0.0f;
-    s.h[idx] = h;
-    s.m1[idx] = in ? L.m1[g] : 0.0f;
-    s.n1[idx] = in ? L.n1[g] : 0.0f;
-    if (MODE != kMomentOnly || NL) s.z1[idx] = in ? L.z1[g] : 0.0f;
-    if (MODE == kMomentOnly) {
-      const float z2 = in ? L.z2[g] : 0.0f;
-      s.z2[idx] = z2;
-      if (NL) {
-        float dz = 0.0f;
-        if (in) {
-          if (L.dzp) {
-            dz = L.dzp[g];
-          } else {
-            const bool massed =
-                IN || (L.is_root ? (gi >= 2 && gi <= nx - 1 && gj >= 2 && gj <= ny - 1)
-                                 : (gi >= 2 && gj >= 2));
-            dz = massed ? z2 + h : 0.0f;
+#pragma unroll
+    for (int k = 0; k < KS; ++k) {
+      const int idx = tid + k * kThreads;
+      const int gi = ti0 - 2 + idx % P, gj = tj0 - 2 + idx / P;
+      const bool in = (KS * kThreads == A || idx < A) &&
+                      (IN || (gi >= 1 && gi <= nx && gj >= 1 && gj <= ny));
+      const size_t g = in ? (size_t)(gj - 1) * nx + (gi - 1) : 0;
+      vh[k] = in ? L.h[g] : 0.0f;
+      vm[k] = in ? L.m1[g] : 0.0f;
+      vn[k] = in ? L.n1[g] : 0.0f;
+      if (Z1) vz1[k] = in ? L.z1[g] : 0.0f;
+      if (Z2) vz2[k] = in ? L.z2[g] : 0.0f;
+      if (Z2 && NL) vdz[k] = in && L.dzp ? L.dzp[g] : 0.0f;
+    }
+    if (SPH) {
+      float *crow = sm + 6 * A;
+#pragma unroll
+      for (int k = 0; k < KC; ++k)
+        if (tid + k * kThreads < kNumCoef * R) crow[tid + k * kThreads] = vc[k];
+    }
+#pragma unroll
+    for (int k = 0; k < KS; ++k) {
+      const int idx = tid + k * kThreads;
+      if (!(KS * kThreads == A || idx < A)) continue;
+      const int gi = ti0 - 2 + idx % P, gj = tj0 - 2 + idx / P;
+      const bool in = IN || (gi >= 1 && gi <= nx && gj >= 1 && gj <= ny);
+      s.h[idx] = vh[k];
+      s.m1[idx] = vm[k];
+      s.n1[idx] = vn[k];
+      if (Z1) s.z1[idx] = vz1[k];
+      if (Z2) {
+        s.z2[idx] = vz2[k];
+        if (NL) {
+          float dz = 0.0f;
+          if (in) {
+            if (L.dzp) {
+              dz = vdz[k];
+            } else {
+              const bool massed =
+                  IN || (L.is_root ? (gi >= 2 && gi <= nx - 1 && gj >= 2 && gj <= ny - 1)
+                                   : (gi >= 2 && gj >= 2));
+              dz = massed ? vz2[k] + vh[k] : 0.0f;
+            }
           }
+          s.dz[idx] = dz;
         }
-        s.dz[idx] = dz;
+      } else {
+        s.z2[idx] = 0.0f;
+        s.dz[idx] = 0.0f;
       }
-    } else {
-      s.z2[idx] = 0.0f;
-      s.dz[idx] = 0.0f;
     }
   }
   __syncthreads();
@@ -693,20 +723,42 @@ __device__ __forceinline__ void tile_body_mm(const DevLayer &L, int t, int ti0, 
   Lds<P> sa{pz, pm0, pn0, ph, pz, nullptr, ti0 - 2, tj0 - 2, stab, crow, R};
   Lds<P> sb{pz, pm, pn, ph, pz, nullptr, ti0 - 2, tj0 - 2, stab, crow, R};
   const int nx = L.nx, ny = L.ny, tid = threadIdx.x;
-  if (SPH) {
-    for (int idx = tid; idx < kNumCoef * R; idx += kThreads) {
-      const int k = idx / R, j = tj0 - 2 + idx % R;
-      crow[idx] = (IN || (j >= 1 && j <= ny)) ? L.coef[k * ny + (j - 1)] : 0.0f;
+  {  // every load of the thread in flight before its first LDS write
+    constexpr int KS = (A + kThreads - 1) / kThreads;
+    constexpr int KC = SPH ? (kNumCoef * R + kThreads - 1) / kThreads : 0;
+    float vh[KS], vz[KS], vm[KS], vn[KS], vc[KC > 0 ? KC : 1];
+#pragma unroll
+    for (int k = 0; k < KC; ++k) {
+      const int idx = tid + k * kThreads, c = idx / R, j = tj0 - 2 + idx % R;
+      vc[k] = idx < kNumCoef * R && (IN || (j >= 1 && j <= ny)) ? L.coef[c * ny + (j - 1)] : 0.0f;
     }
-  }
-  for (int idx = tid; idx < A; idx += kThreads) {
-    const int gi = ti0 - 2 + idx % P, gj = tj0 - 2 + idx / P;
-    const bool in = IN || (gi >= 1 && gi <= nx && gj >= 1 && gj <= ny);
-    const size_t g = in ? (size_t)(gj - 1) * nx + (gi - 1) : 0;
-    ph[idx] = in ? L.h[g] : 0.0f;
-    pz[idx] = in ? L.z1[g] : 0.0f;
-    pm0[idx] = in ? L.m1[g] : 0.0f;
-    pn0[idx] = in ? L.n1[g] : 0.0f;
+#pragma unroll
+    for (int k = 0; k < KS; ++k) {
+      const int idx = tid + k * kThreads;
+      const int gi = ti0 - 2 + idx % P, gj = tj0 - 2 + idx / P;
+      const bool in = (KS * kThreads == A || idx < A) &&
+                      (IN || (gi >= 1 && gi <= nx && gj >= 1 && gj <= ny));
+      const size_t g = in ? (size_t)(gj - 1) * nx + (gi - 1) : 0;
+      vh[k] = in ? L.h[g] : 0.0f;
+      vz[k] = in ? L.z1[g] : 0.0f;
+      vm[k] = in ? L.m1[g] : 0.0f;
+      vn[k] = in ? L.n1[g] : 0.0f;
+    }
+    if (SPH) {
+#pragma unroll
+      for (int k = 0; k < KC; ++k)
+        if (tid + k * kThreads < kNumCoef * R) crow[tid + k * kThreads] = vc[k];
+    }
+#pragma unroll
+    for (int k = 0; k < KS; ++k) {
+      const int idx = tid + k * kThreads;
+      if (KS * kThreads == A || idx < A) {
+        ph[idx] = vh[k];
+        pz[idx] = vz[k];
+        pm0[idx] = vm[k];
+        pn0[idx] = vn[k];
+      }
+    }
   }
   __syncthreads();
 
@@ -842,20 +894,41 @@ __device__ __forceinline__ void tile_body2(const DevLayer &L, int t, int ti0, in
   const int nx = L.nx, ny = L.ny, tid = threadIdx.x;
   auto inside = [&](int i, int j) { return IN || (i >= 1 && i <= nx && j >= 1 && j <= ny); };
 
-  if (SPH) {
-    for (int idx = tid; idx < kNumCoef * R; idx += NT) {
-      const int k = idx / R, j = gj0 + idx % R;
-      crow[idx] = (IN || (j >= 1 && j <= ny)) ? L.coef[k * ny + (j - 1)] : 0.0f;
+  // Staging: every load of the thread is issued before its first LDS
+  // write, so the tile waits for memory once rather than once per pass of
+  // the thread over the staged rectangle.
+  {
+    constexpr int KS = (A + NT - 1) / NT, KC = SPH ? (kNumCoef * R + NT - 1) / NT : 0;
+    float vh[KS], vz[KS], vm[KS], vn[KS], vc[KC > 0 ? KC : 1];
+#pragma unroll
+    for (int k = 0; k < KC; ++k) {
+      const int idx = tid + k * NT, c = idx / R, j = gj0 + idx % R;
+      vc[k] = idx < kNumCoef * R && (IN || (j >= 1 && j <= ny)) ? L.coef[c * ny + (j - 1)] : 0.0f;
     }
-  }
-  for (int idx = tid; idx < A; idx += NT) {
-    const int gi = gi0 + idx % P, gj = gj0 + idx / P;
-    const bool in = inside(gi, gj);
-    const size_t g = in ? (size_t)(gj - 1) * nx + (gi - 1) : 0;
-    ph[idx] = in ? L.h[g] : 0.0f;
-    pz0[idx] = in ? L.z1[g] : 0.0f;
-    pm0[idx] = in ? L.m1[g] : 0.0f;
-    pn0[idx] = in ? L.n1[g] : 0.0f;
+#pragma unroll
+    for (int k = 0; k < KS; ++k) {
+      const int idx = tid + k * NT;
+      const int gi = gi0 + idx % P, gj = gj0 + idx / P;
+      const bool in = idx < A && inside(gi, gj);
+      const size_t g = in ? (size_t)(gj - 1) * nx + (gi - 1) : 0;
+      vh[k] = in ? L.h[g] : 0.0f;
+      vz[k] = in ? L.z1[g] : 0.0f;
+      vm[k] = in ? L.m1[g] : 0.0f;
+      vn[k] = in ? L.n1[g] : 0.0f;
+    }
+#pragma unroll
+    for (int k = 0; k < KC; ++k)
+      if (tid + k * NT < kNumCoef * R) crow[tid + k * NT] = vc[k];
+#pragma unroll
+    for (int k = 0; k < KS; ++k) {
+      const int idx = tid + k * NT;
+      if (KS * NT == A || idx < A) {
+        ph[idx] = vh[k];
+        pz0[idx] = vz[k];
+        pm0[idx] = vm[k];
+        pn0[idx] = vn[k];
+      }
+    }
   }
   __syncthreads();
 
