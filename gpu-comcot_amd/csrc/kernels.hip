@@ -513,17 +513,30 @@ __device__ float nl_y(const DevLayer &L, const A &s, int i, int j, float brk) {
 }
 
 // Copy the pow/exp tables into LDS (read before the first __syncthreads).
+// The two halves are split so that a tile body can issue the table loads
+// with its own staging loads (TabStage), before any LDS write.
+template <int NT>
+struct TabStage {
+  static constexpr int n = sizeof(gcdet::Tables) / sizeof(double), K = (n + NT - 1) / NT;
+  double v[K];
+  __device__ __forceinline__ void load(int tid) {
+    const double *src = reinterpret_cast<const double *>(&gcdet::kTab);
+#pragma unroll
+    for (int q = 0; q < K; ++q) v[q] = tid + q * NT < n ? src[tid + q * NT] : 0.0;
+  }
+  // dst: the workgroup's LDS copy, which the tile reads after its barrier
+  __device__ __forceinline__ void store(const gcdet::Tables *dst, int tid) const {
+    double *d = reinterpret_cast<double *>(const_cast<gcdet::Tables *>(dst));
+#pragma unroll
+    for (int q = 0; q < K; ++q)
+      if (tid + q * NT < n) d[tid + q * NT] = v[q];
+  }
+};
 template <int NT = kThreads>
 __device__ __forceinline__ void stage_tables(gcdet::Tables &dst, int tid) {
-  const double *src = reinterpret_cast<const double *>(&gcdet::kTab);
-  double *d = reinterpret_cast<double *>(&dst);
-  constexpr int n = sizeof(gcdet::Tables) / sizeof(double), K = (n + NT - 1) / NT;
-  double v[K];  // all loads in flight before the first LDS write
-#pragma unroll
-  for (int q = 0; q < K; ++q) v[q] = tid + q * NT < n ? src[tid + q * NT] : 0.0;
-#pragma unroll
-  for (int q = 0; q < K; ++q)
-    if (tid + q * NT < n) d[tid + q * NT] = v[q];
+  TabStage<NT> t;  // all loads in flight before the first LDS write
+  t.load(tid);
+  t.store(&dst, tid);
 }
 
 // Tile -> workgroup order: workgroup b runs on XCD b%8; XCD x gets the
@@ -541,7 +554,7 @@ __device__ __forceinline__ int tile_of_block(int per_xcd) {
 // publishes them).  IN: the tile and its halo lie more than SPAN+2 cells
 // inside the grid, so no index clamp, boundary ring, ghost row or range
 // limit of the Fortran loops can apply and all of them compile away.
-template <int MODE, bool NL, bool SPH, int TX, int TY, bool IN, bool ST>
+template <int MODE, bool NL, bool SPH, int TX, int TY, bool IN, bool ST, bool TAB>
 __device__ __forceinline__ void tile_body(const DevLayer &L, int t, int ti0, int tj0, float *sm,
                                           const gcdet::Tables *stab) {
   constexpr int P = TX + 4, R = TY + 4, A = P * R;
@@ -558,6 +571,8 @@ __device__ __forceinline__ void tile_body(const DevLayer &L, int t, int ti0, int
     constexpr int KC = SPH ? (kNumCoef * R + kThreads - 1) / kThreads : 0;
     constexpr bool Z1 = MODE != kMomentOnly || NL, Z2 = MODE == kMomentOnly;
     float vh[KS], vm[KS], vn[KS], vz1[KS], vz2[KS], vdz[KS], vc[KC > 0 ? KC : 1];
+    TabStage<kThreads> tb;
+    if constexpr (TAB) tb.load(tid);
 #pragma unroll
     for (int k = 0; k < KC; ++k) {
       const int idx = tid + k * kThreads, c = idx / R, j = tj0 - 2 + idx % R;
@@ -577,6 +592,7 @@ __device__ __forceinline__ void tile_body(const DevLayer &L, int t, int ti0, int
       if (Z2) vz2[k] = in ? L.z2[g] : 0.0f;
       if (Z2 && NL) vdz[k] = in && L.dzp ? L.dzp[g] : 0.0f;
     }
+    if constexpr (TAB) tb.store(stab, tid);
     if (SPH) {
       float *crow = sm + 6 * A;
 #pragma unroll
@@ -712,7 +728,7 @@ __device__ __forceinline__ void tile_body(const DevLayer &L, int t, int ti0, int
 // the divergence read the fluxes of the cell and of its left and lower
 // neighbours only.  Each value comes from the same functions and operands
 // as in the momentum-only and continuity-only launches.
-template <bool SPH, int TX, int TY, bool IN, bool ST>
+template <bool SPH, int TX, int TY, bool IN, bool ST, bool TAB>
 __device__ __forceinline__ void tile_body_mm(const DevLayer &L, int t, int ti0, int tj0,
                                              float *sm, const gcdet::Tables *stab) {
   constexpr int P = TX + 4, R = TY + 4, A = P * R;
@@ -727,6 +743,8 @@ __device__ __forceinline__ void tile_body_mm(const DevLayer &L, int t, int ti0, 
     constexpr int KS = (A + kThreads - 1) / kThreads;
     constexpr int KC = SPH ? (kNumCoef * R + kThreads - 1) / kThreads : 0;
     float vh[KS], vz[KS], vm[KS], vn[KS], vc[KC > 0 ? KC : 1];
+    TabStage<kThreads> tb;
+    if constexpr (TAB) tb.load(tid);
 #pragma unroll
     for (int k = 0; k < KC; ++k) {
       const int idx = tid + k * kThreads, c = idx / R, j = tj0 - 2 + idx % R;
@@ -744,6 +762,7 @@ __device__ __forceinline__ void tile_body_mm(const DevLayer &L, int t, int ti0, 
       vm[k] = in ? L.m1[g] : 0.0f;
       vn[k] = in ? L.n1[g] : 0.0f;
     }
+    if constexpr (TAB) tb.store(stab, tid);
     if (SPH) {
 #pragma unroll
       for (int k = 0; k < KC; ++k)
@@ -811,7 +830,9 @@ __device__ __forceinline__ void tile_body_mm(const DevLayer &L, int t, int ti0, 
   }
 }
 
-template <int MODE, bool NL, bool SPH, int TX, int TY, bool ST = false>
+// TAB: the tile body also fills the workgroup's LDS copy of the pow/exp
+// tables (its loads issued with the tile's staging loads).
+template <int MODE, bool NL, bool SPH, int TX, int TY, bool ST = false, bool TAB = false>
 __device__ __forceinline__ void tile_step(const DevLayer &L, int t, int tiles_x, float *sm,
                                           const gcdet::Tables *stab) {
   const int ti0 = (t % tiles_x) * TX + 1, tj0 = (t / tiles_x) * TY + 1;
@@ -819,11 +840,11 @@ __device__ __forceinline__ void tile_step(const DevLayer &L, int t, int tiles_x,
                         tj0 - 2 > kSpan + 2 && tj0 + TY + 2 < L.ny - kSpan;
   if constexpr (MODE == kMomentMass) {
     static_assert(!NL, "momentum + next continuity is built for linear layers");
-    if (interior) tile_body_mm<SPH, TX, TY, true, ST>(L, t, ti0, tj0, sm, stab);
-    else tile_body_mm<SPH, TX, TY, false, false>(L, t, ti0, tj0, sm, stab);
+    if (interior) tile_body_mm<SPH, TX, TY, true, ST, TAB>(L, t, ti0, tj0, sm, stab);
+    else tile_body_mm<SPH, TX, TY, false, false, TAB>(L, t, ti0, tj0, sm, stab);
   } else {
-    if (interior) tile_body<MODE, NL, SPH, TX, TY, true, ST>(L, t, ti0, tj0, sm, stab);
-    else tile_body<MODE, NL, SPH, TX, TY, false, false>(L, t, ti0, tj0, sm, stab);
+    if (interior) tile_body<MODE, NL, SPH, TX, TY, true, ST, TAB>(L, t, ti0, tj0, sm, stab);
+    else tile_body<MODE, NL, SPH, TX, TY, false, false, TAB>(L, t, ti0, tj0, sm, stab);
   }
 }
 
@@ -865,8 +886,7 @@ __global__ void __launch_bounds__(kThreads)
   __shared__ gcdet::Tables stab;
   const int t = tile_of_block(per_xcd);
   if (t >= ntiles) return;
-  if (NL || !SPH) stage_tables(stab, threadIdx.x);
-  tile_step<MODE, NL, SPH, TX, TY, ST>(L, t, tiles_x, sm, &stab);
+  tile_step<MODE, NL, SPH, TX, TY, ST, NL || !SPH>(L, t, tiles_x, sm, &stab);
 }
 
 // ------------------------------------------------ two steps per launch
@@ -880,7 +900,7 @@ __global__ void __launch_bounds__(kThreads)
 // bitwise the same; HBM traffic and launches per step halve, at the cost
 // of recomputing the halo ring (cheap for the linear equations).  The wet
 // maximum and arrival see both steps in order.
-template <bool SPH, int TX, int TY, bool IN, int NT, bool ST>
+template <bool SPH, int TX, int TY, bool IN, int NT, bool ST, bool TAB>
 __device__ __forceinline__ void tile_body2(const DevLayer &L, int t, int ti0, int tj0,
                                            float *sm, const gcdet::Tables *stab) {
   constexpr int HW = 4, P = TX + 2 * HW, R = TY + 2 * HW, A = P * R;
@@ -900,6 +920,8 @@ __device__ __forceinline__ void tile_body2(const DevLayer &L, int t, int ti0, in
   {
     constexpr int KS = (A + NT - 1) / NT, KC = SPH ? (kNumCoef * R + NT - 1) / NT : 0;
     float vh[KS], vz[KS], vm[KS], vn[KS], vc[KC > 0 ? KC : 1];
+    TabStage<NT> tb;
+    if constexpr (TAB) tb.load(tid);
 #pragma unroll
     for (int k = 0; k < KC; ++k) {
       const int idx = tid + k * NT, c = idx / R, j = gj0 + idx % R;
@@ -916,6 +938,7 @@ __device__ __forceinline__ void tile_body2(const DevLayer &L, int t, int ti0, in
       vm[k] = in ? L.m1[g] : 0.0f;
       vn[k] = in ? L.n1[g] : 0.0f;
     }
+    if constexpr (TAB) tb.store(stab, tid);
 #pragma unroll
     for (int k = 0; k < KC; ++k)
       if (tid + k * NT < kNumCoef * R) crow[tid + k * NT] = vc[k];
@@ -1030,12 +1053,11 @@ __global__ void __launch_bounds__(NT)
   __shared__ gcdet::Tables stab;
   const int t = tile_of_block(per_xcd);
   if (t >= ntiles) return;
-  if (!SPH) stage_tables<NT>(stab, threadIdx.x);
   const int ti0 = (t % tiles_x) * TX + 1, tj0 = (t / tiles_x) * TY + 1;
   const bool interior = ti0 - HW > kSpan + 2 && ti0 + TX + HW < L.nx - kSpan &&
                         tj0 - HW > kSpan + 2 && tj0 + TY + HW < L.ny - kSpan;
-  if (interior) tile_body2<SPH, TX, TY, true, NT, ST>(L, t, ti0, tj0, sm, &stab);
-  else tile_body2<SPH, TX, TY, false, NT, false>(L, t, ti0, tj0, sm, &stab);
+  if (interior) tile_body2<SPH, TX, TY, true, NT, ST, !SPH>(L, t, ti0, tj0, sm, &stab);
+  else tile_body2<SPH, TX, TY, false, NT, false, !SPH>(L, t, ti0, tj0, sm, &stab);
 }
 
 // ---------------------------------------------------------- strip kernel
@@ -1913,12 +1935,11 @@ __global__ void __launch_bounds__(kThreads)
   __shared__ gcdet::Tables stab;
   const int it = tile_of_block(per_xcd);
   if (it >= nitems) return;
-  if (NL || !SPH) stage_tables(stab, threadIdx.x);
   const BatchItem w = items[it];
   const int slot = __builtin_amdgcn_readfirstlane(w.slot);
   const int tile = __builtin_amdgcn_readfirstlane(w.tile);
   const DevLayer L = descs[slot].L;
-  tile_step<MODE, NL, SPH, TX, TY>(L, tile, descs[slot].tiles_x, sm, &stab);
+  tile_step<MODE, NL, SPH, TX, TY, false, NL || !SPH>(L, tile, descs[slot].tiles_x, sm, &stab);
 }
 
 
@@ -1934,7 +1955,6 @@ __global__ void __launch_bounds__(NT)
   __shared__ gcdet::Tables stab;
   const int it = tile_of_block(per_xcd);
   if (it >= nitems) return;
-  if (!SPH) stage_tables<NT>(stab, threadIdx.x);
   const BatchItem w = items[it];
   const int slot = __builtin_amdgcn_readfirstlane(w.slot);
   const int t = __builtin_amdgcn_readfirstlane(w.tile);
@@ -1943,8 +1963,8 @@ __global__ void __launch_bounds__(NT)
   const int ti0 = (t % tiles_x) * TX + 1, tj0 = (t / tiles_x) * TY + 1;
   const bool interior = ti0 - HW > kSpan + 2 && ti0 + TX + HW < L.nx - kSpan &&
                         tj0 - HW > kSpan + 2 && tj0 + TY + HW < L.ny - kSpan;
-  if (interior) tile_body2<SPH, TX, TY, true, NT, false>(L, t, ti0, tj0, sm, &stab);
-  else tile_body2<SPH, TX, TY, false, NT, false>(L, t, ti0, tj0, sm, &stab);
+  if (interior) tile_body2<SPH, TX, TY, true, NT, false, !SPH>(L, t, ti0, tj0, sm, &stab);
+  else tile_body2<SPH, TX, TY, false, NT, false, !SPH>(L, t, ti0, tj0, sm, &stab);
 }
 
 template <int MODE, bool NL, bool SPH, int TX, int TY>
