@@ -513,6 +513,24 @@ __device__ float nl_y(const DevLayer &L, const A &s, int i, int j, float brk) {
 }
 
 // Copy the pow/exp tables into LDS (read before the first __syncthreads).
+// A thread's walk over a region CX cells wide in steps of NT cells of the
+// flat order (idx = tid, tid + NT, ...): column i and row j advance by
+// constants with one wrap test instead of a division per cell.  idx <
+// CX * CY exactly when j < CY.
+template <int CX, int NT>
+struct Walk {
+  int i, j;
+  __device__ __forceinline__ explicit Walk(int tid) : i(tid % CX), j(tid / CX) {}
+  __device__ __forceinline__ void next() {
+    i += NT % CX;
+    j += NT / CX;
+    if (i >= CX) {
+      i -= CX;
+      ++j;
+    }
+  }
+};
+
 // The two halves are split so that a tile body can issue the table loads
 // with its own staging loads (TabStage), before any LDS write.
 template <int NT>
@@ -578,10 +596,11 @@ __device__ __forceinline__ void tile_body(const DevLayer &L, int t, int ti0, int
       const int idx = tid + k * kThreads, c = idx / R, j = tj0 - 2 + idx % R;
       vc[k] = idx < kNumCoef * R && (IN || (j >= 1 && j <= ny)) ? L.coef[c * ny + (j - 1)] : 0.0f;
     }
+    Walk<P, kThreads> w(tid);
 #pragma unroll
-    for (int k = 0; k < KS; ++k) {
+    for (int k = 0; k < KS; ++k, w.next()) {
       const int idx = tid + k * kThreads;
-      const int gi = ti0 - 2 + idx % P, gj = tj0 - 2 + idx / P;
+      const int gi = ti0 - 2 + w.i, gj = tj0 - 2 + w.j;
       const bool in = (KS * kThreads == A || idx < A) &&
                       (IN || (gi >= 1 && gi <= nx && gj >= 1 && gj <= ny));
       const size_t g = in ? (size_t)(gj - 1) * nx + (gi - 1) : 0;
@@ -639,8 +658,8 @@ __device__ __forceinline__ void tile_body(const DevLayer &L, int t, int ti0, int
     constexpr int CY = MODE == kFused ? TY + 3 : TY;
     const int ci0 = MODE == kFused ? ti0 - 1 : ti0;
     const int cj0 = MODE == kFused ? tj0 - 1 : tj0;
-    for (int idx = tid; idx < CX * CY; idx += kThreads) {
-      const int i = ci0 + idx % CX, j = cj0 + idx / CX;
+    for (Walk<CX, kThreads> w(tid); w.j < CY; w.next()) {
+      const int i = ci0 + w.i, j = cj0 + w.j;
       if (!IN && (i < 1 || i > nx || j < 1 || j > ny)) continue;
       float z2, dz2;
       mass_cell<NL, SPH, IN>(L, s, i, j, z2, dz2);
@@ -750,10 +769,11 @@ __device__ __forceinline__ void tile_body_mm(const DevLayer &L, int t, int ti0, 
       const int idx = tid + k * kThreads, c = idx / R, j = tj0 - 2 + idx % R;
       vc[k] = idx < kNumCoef * R && (IN || (j >= 1 && j <= ny)) ? L.coef[c * ny + (j - 1)] : 0.0f;
     }
+    Walk<P, kThreads> w(tid);
 #pragma unroll
-    for (int k = 0; k < KS; ++k) {
+    for (int k = 0; k < KS; ++k, w.next()) {
       const int idx = tid + k * kThreads;
-      const int gi = ti0 - 2 + idx % P, gj = tj0 - 2 + idx / P;
+      const int gi = ti0 - 2 + w.i, gj = tj0 - 2 + w.j;
       const bool in = (KS * kThreads == A || idx < A) &&
                       (IN || (gi >= 1 && gi <= nx && gj >= 1 && gj <= ny));
       const size_t g = in ? (size_t)(gj - 1) * nx + (gi - 1) : 0;
@@ -785,8 +805,8 @@ __device__ __forceinline__ void tile_body_mm(const DevLayer &L, int t, int ti0, 
   // of the outermost layer: M on I <= NX-1, N on J <= NY-1; 0 elsewhere)
   {
     constexpr int CX = TX + 1, CY = TY + 1;
-    for (int idx = tid; idx < CX * CY; idx += kThreads) {
-      const int i = ti0 - 1 + idx % CX, j = tj0 - 1 + idx / CX;
+    for (Walk<CX, kThreads> w(tid); w.j < CY; w.next()) {
+      const int i = ti0 - 1 + w.i, j = tj0 - 1 + w.j;
       float m = 0.0f, n = 0.0f;
       if (IN || (i >= 1 && i <= nx && j >= 1 && j <= ny)) {
         if (IN || i <= nx - 1) m = lin_x<SPH, IN>(L, sa, i, j);
@@ -927,10 +947,11 @@ __device__ __forceinline__ void tile_body2(const DevLayer &L, int t, int ti0, in
       const int idx = tid + k * NT, c = idx / R, j = gj0 + idx % R;
       vc[k] = idx < kNumCoef * R && (IN || (j >= 1 && j <= ny)) ? L.coef[c * ny + (j - 1)] : 0.0f;
     }
+    Walk<P, NT> w(tid);
 #pragma unroll
-    for (int k = 0; k < KS; ++k) {
+    for (int k = 0; k < KS; ++k, w.next()) {
       const int idx = tid + k * NT;
-      const int gi = gi0 + idx % P, gj = gj0 + idx / P;
+      const int gi = gi0 + w.i, gj = gj0 + w.j;
       const bool in = idx < A && inside(gi, gj);
       const size_t g = in ? (size_t)(gj - 1) * nx + (gi - 1) : 0;
       vh[k] = in ? L.h[g] : 0.0f;
@@ -958,8 +979,8 @@ __device__ __forceinline__ void tile_body2(const DevLayer &L, int t, int ti0, in
   // step one, continuity on the tile + 3
   {
     constexpr int CX = TX + 6, CY = TY + 6;
-    for (int idx = tid; idx < CX * CY; idx += NT) {
-      const int i = ti0 - 3 + idx % CX, j = tj0 - 3 + idx / CX;
+    for (Walk<CX, NT> w(tid); w.j < CY; w.next()) {
+      const int i = ti0 - 3 + w.i, j = tj0 - 3 + w.j;
       float z2 = 0.0f, dz2;
       if (inside(i, j)) mass_cell<false, SPH, IN>(L, s1, i, j, z2, dz2);
       pz1[s1.o(i, j)] = z2;
@@ -969,8 +990,8 @@ __device__ __forceinline__ void tile_body2(const DevLayer &L, int t, int ti0, in
   // step one, momentum on the tile + 2 (0 outside the MOMT ranges and the grid)
   {
     constexpr int CX = TX + 4, CY = TY + 4;
-    for (int idx = tid; idx < CX * CY; idx += NT) {
-      const int i = ti0 - 2 + idx % CX, j = tj0 - 2 + idx / CX;
+    for (Walk<CX, NT> w(tid); w.j < CY; w.next()) {
+      const int i = ti0 - 2 + w.i, j = tj0 - 2 + w.j;
       float m = 0.0f, n = 0.0f;
       if (inside(i, j)) {
         const bool mx = IN || (i <= nx - 1);
@@ -986,8 +1007,8 @@ __device__ __forceinline__ void tile_body2(const DevLayer &L, int t, int ti0, in
   // step two, continuity on the tile + 1
   {
     constexpr int CX = TX + 2, CY = TY + 2;
-    for (int idx = tid; idx < CX * CY; idx += NT) {
-      const int i = ti0 - 1 + idx % CX, j = tj0 - 1 + idx / CX;
+    for (Walk<CX, NT> w(tid); w.j < CY; w.next()) {
+      const int i = ti0 - 1 + w.i, j = tj0 - 1 + w.j;
       float z2 = 0.0f, dz2;
       if (inside(i, j)) mass_cell<false, SPH, IN>(L, s2, i, j, z2, dz2);
       pz0[s2.o(i, j)] = z2;
