@@ -153,11 +153,13 @@ __device__ float open_cell(const DevLayer &L, const A &s, int i, int j) {
 
 // Continuity for one cell: MASS_S (mass.f90:61-100) / MASS_C (:103-157),
 // CONMASS_S (:227-277) / CONMASS (:162-224) when NL.  dz2 is the total
-// depth DZ(:,:,2) of the nonlinear forms (0 where not computed).
-template <bool NL, bool SPH, bool IN, class A>
+// depth DZ(:,:,2) of the nonlinear forms (0 where not computed).  RING =
+// false: the caller leaves the outermost layer's open-boundary ring to
+// ring_cells below.
+template <bool NL, bool SPH, bool IN, bool RING = true, class A>
 __device__ __forceinline__ void mass_cell(const DevLayer &L, const A &s, int i, int j,
                                           float &z2, float &dz2) {
-  if (!IN && L.is_root && (i == 1 || j == 1 || i == L.nx || j == L.ny)) {
+  if (RING && !IN && L.is_root && (i == 1 || j == 1 || i == L.nx || j == L.ny)) {
     z2 = open_cell(L, s, i, j);
     dz2 = 0.0f;
     return;
@@ -196,6 +198,47 @@ __device__ __forceinline__ void mass_cell(const DevLayer &L, const A &s, int i, 
       dz2 = 0.0f;
       z2 = -h;
     }
+  }
+}
+
+// On the outermost layer's open-boundary ring (I = 1 or NX, J = 1 or NY)?
+__device__ __forceinline__ bool on_ring(const DevLayer &L, int i, int j) {
+  return L.is_root && (i == 1 || j == 1 || i == L.nx || j == L.ny);
+}
+
+// The ring cells of the rectangle [x0, x0 + CX) x [y0, y0 + CY) of the
+// outermost layer, one per lane: the columns I = 1 and I = NX, then the
+// rows J = 1 and J = NY without those columns.  The tile loops skip the
+// ring and this pass computes OPEN there (put(i, j) stores a cell), so
+// only the ring's lanes run OPEN's square roots and division: spread over
+// a tile's flat order, every wave of an edge tile would.
+template <int NT, class F>
+__device__ __forceinline__ void ring_cells(const DevLayer &L, int x0, int y0, int CX, int CY,
+                                           int tid, F put) {
+  const int nx = L.nx, ny = L.ny;
+  const int xa = max(x0, 1), xb = min(x0 + CX - 1, nx);
+  const int ya = max(y0, 1), yb = min(y0 + CY - 1, ny);
+  if (!L.is_root || xa > xb || ya > yb) return;
+  const bool c0 = xa == 1, c1 = xb == nx && nx > 1, r0 = ya == 1, r1 = yb == ny && ny > 1;
+  const int ia = xa + (c0 ? 1 : 0), ib = xb - (c1 ? 1 : 0);
+  const int rows = yb - ya + 1, cols = ib >= ia ? ib - ia + 1 : 0;
+  const int n0 = c0 ? rows : 0, n1 = c1 ? rows : 0, n2 = r0 ? cols : 0, n3 = r1 ? cols : 0;
+  for (int q = tid; q < n0 + n1 + n2 + n3; q += NT) {
+    int r = q, i, j;
+    if (r < n0) {
+      i = 1;
+      j = ya + r;
+    } else if ((r -= n0) < n1) {
+      i = nx;
+      j = ya + r;
+    } else if ((r -= n1) < n2) {
+      i = ia + r;
+      j = 1;
+    } else {
+      i = ia + (r - n2);
+      j = ny;
+    }
+    put(i, j);
   }
 }
 
@@ -660,12 +703,17 @@ __device__ __forceinline__ void tile_body(const DevLayer &L, int t, int ti0, int
     const int cj0 = MODE == kFused ? tj0 - 1 : tj0;
     for (Walk<CX, kThreads> w(tid); w.j < CY; w.next()) {
       const int i = ci0 + w.i, j = cj0 + w.j;
-      if (!IN && (i < 1 || i > nx || j < 1 || j > ny)) continue;
+      if (!IN && (i < 1 || i > nx || j < 1 || j > ny || on_ring(L, i, j))) continue;
       float z2, dz2;
-      mass_cell<NL, SPH, IN>(L, s, i, j, z2, dz2);
+      mass_cell<NL, SPH, IN, false>(L, s, i, j, z2, dz2);
       s.z2[s.o(i, j)] = z2;
       if (NL) s.dz[s.o(i, j)] = dz2;
     }
+    if (!IN)
+      ring_cells<kThreads>(L, ci0, cj0, CX, CY, tid, [&](int i, int j) {
+        s.z2[s.o(i, j)] = open_cell(L, s, i, j);
+        if (NL) s.dz[s.o(i, j)] = 0.0f;
+      });
     __syncthreads();
   }
 
@@ -844,10 +892,15 @@ __device__ __forceinline__ void tile_body_mm(const DevLayer &L, int t, int ti0, 
       if (zk > zm) L.zmax[g] = zk;
     }
     if (L.zraw && !(L.zraw[g] >= zk)) L.zraw[g] = zk;
+    if (!IN && on_ring(L, i, j)) continue;  // the ring pass below
     float z2, dz2;
-    mass_cell<false, SPH, IN>(L, sb, i, j, z2, dz2);
+    mass_cell<false, SPH, IN, false>(L, sb, i, j, z2, dz2);
     st_plane<ST>(L.z2 + g, z2);
   }
+  if (!IN)
+    ring_cells<kThreads>(L, ti0, tj0, TX, TY, tid, [&](int i, int j) {
+      st_plane<ST>(L.z2 + (size_t)(j - 1) * nx + (i - 1), open_cell(L, sb, i, j));
+    });
 }
 
 // TAB: the tile body also fills the workgroup's LDS copy of the pow/exp
@@ -981,10 +1034,14 @@ __device__ __forceinline__ void tile_body2(const DevLayer &L, int t, int ti0, in
     constexpr int CX = TX + 6, CY = TY + 6;
     for (Walk<CX, NT> w(tid); w.j < CY; w.next()) {
       const int i = ti0 - 3 + w.i, j = tj0 - 3 + w.j;
+      if (!IN && inside(i, j) && on_ring(L, i, j)) continue;  // the ring pass below
       float z2 = 0.0f, dz2;
-      if (inside(i, j)) mass_cell<false, SPH, IN>(L, s1, i, j, z2, dz2);
+      if (inside(i, j)) mass_cell<false, SPH, IN, false>(L, s1, i, j, z2, dz2);
       pz1[s1.o(i, j)] = z2;
     }
+    if (!IN)
+      ring_cells<NT>(L, ti0 - 3, tj0 - 3, CX, CY, tid,
+                     [&](int i, int j) { pz1[s1.o(i, j)] = open_cell(L, s1, i, j); });
   }
   __syncthreads();
   // step one, momentum on the tile + 2 (0 outside the MOMT ranges and the grid)
@@ -1009,10 +1066,14 @@ __device__ __forceinline__ void tile_body2(const DevLayer &L, int t, int ti0, in
     constexpr int CX = TX + 2, CY = TY + 2;
     for (Walk<CX, NT> w(tid); w.j < CY; w.next()) {
       const int i = ti0 - 1 + w.i, j = tj0 - 1 + w.j;
+      if (!IN && inside(i, j) && on_ring(L, i, j)) continue;  // the ring pass below
       float z2 = 0.0f, dz2;
-      if (inside(i, j)) mass_cell<false, SPH, IN>(L, s2, i, j, z2, dz2);
+      if (inside(i, j)) mass_cell<false, SPH, IN, false>(L, s2, i, j, z2, dz2);
       pz0[s2.o(i, j)] = z2;
     }
+    if (!IN)
+      ring_cells<NT>(L, ti0 - 1, tj0 - 1, CX, CY, tid,
+                     [&](int i, int j) { pz0[s2.o(i, j)] = open_cell(L, s2, i, j); });
   }
   __syncthreads();
 
