@@ -199,6 +199,12 @@ module gcomcot_hip
       integer(c_int) :: on
     end function
 
+    integer(c_int) function gcomcot_set_store_policy(mode) &
+        bind(C, name="gcomcot_set_store_policy_")
+      import :: c_int
+      integer(c_int) :: mode
+    end function
+
     ! Exact checkpoints of the device state
     integer(c_int) function gcomcot_checkpoint_save(path, pathlen) &
         bind(C, name="gcomcot_checkpoint_save_")
