@@ -560,8 +560,9 @@ def main():
                      "traffic_per_cell_update": round(traffic / cells, 2) if traffic else None,
                      "practical_peak": copy["read_plus_write_GBs"],
                      "frac_of_practical": round(achieved / copy["read_plus_write_GBs"], 4),
-                     "limit": "VALU issue: the nonlinear step's instruction stream at one "
-                              "wave64 instruction per SIMD every 4 cycles"},
+                     "limit": "VALU issue: ~560 VALU instructions per cell, whose stream at "
+                              "one wave64 instruction per SIMD every 4 cycles would take ~179 us "
+                              "per launch; the launch runs close to that, not at HBM bandwidth"},
         "cpu_baseline": cpu,
         "kernel_ms_mean": round(kern_ms, 5),
         "event_ms_total": round(ms_total, 3),
