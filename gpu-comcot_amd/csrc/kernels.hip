@@ -555,7 +555,6 @@ __device__ float nl_y(const DevLayer &L, const A &s, int i, int j, float brk) {
   return XQ;
 }
 
-// Copy the pow/exp tables into LDS (read before the first __syncthreads).
 // A thread's walk over a region CX cells wide in steps of NT cells of the
 // flat order (idx = tid, tid + NT, ...): column i and row j advance by
 // constants with one wrap test instead of a division per cell.  idx <
@@ -574,6 +573,7 @@ struct Walk {
   }
 };
 
+// Copy the pow/exp tables into LDS (read after the first __syncthreads).
 // The two halves are split so that a tile body can issue the table loads
 // with its own staging loads (TabStage), before any LDS write.
 template <int NT>
