@@ -39,8 +39,10 @@ struct DevLayer {
   // new level-2 planes Z2, M2, N2 of interior tiles with nontemporal
   // stores.  For a layer whose one step moves more bytes than the chip's
   // last-level cache holds, the next step finds none of them in a cache
-  // anyway, and streaming them keeps the depth and maximum planes there.
-  // The batch kernels always store through the caches.
+  // anyway, and streaming them keeps the depth and maximum planes there;
+  // for a layer small enough to stay in L2, streaming spares each kernel
+  // end the write-back of the whole new level.  The batch kernels always
+  // store through the caches.
   int stream_out;
 };
 

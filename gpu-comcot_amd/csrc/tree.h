@@ -95,9 +95,13 @@ int take_last_code();
 // 0 through the caches, 2 streamed (nontemporal) for every layer, 1 (the
 // default) streamed for layers of more than kStreamCells cells, whose one
 // step (about 36 bytes per cell) moves more than three quarters of the
-// 256 MB last-level cache.  The bits are the same every way.
+// 256 MB last-level cache, and for layers of at most kL2Cells cells, whose
+// step fits in the 32 MB of the XCDs' L2s together: cached stores there
+// leave the whole new level dirty in L2, written back at every kernel end.
+// The bits are the same every way.
 extern int g_store_policy;
 constexpr long long kStreamCells = (192ll << 20) / 36;
+constexpr long long kL2Cells = (32ll << 20) / 36;
 bool streams_planes(const LayerRec &L);
 
 // Bytes of device memory a layer of nx x ny cells takes (planes + tables).

@@ -269,7 +269,8 @@ int g_store_policy = 1;
 
 bool streams_planes(const LayerRec &L) {
   if (g_store_policy != 1) return g_store_policy == 2;
-  return (long long)L.nx * L.ny > kStreamCells;
+  const long long cells = (long long)L.nx * L.ny;
+  return cells > kStreamCells || cells <= kL2Cells;
 }
 
 DevLayer dev_view(const Tree &T, const LayerRec &L, bool with_max, bool advance) {

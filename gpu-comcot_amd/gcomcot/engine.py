@@ -114,8 +114,9 @@ class DeviceRun:
 
     def set_store_policy(self, mode=1):
         """How step launches store the new time level (process-wide): 1 =
-        streamed past the caches for layers over ~5.6 M cells, 0 = always
-        through the caches, 2 = always streamed; same bits."""
+        streamed past the caches for layers over ~5.6 M or under ~0.93 M
+        cells, 0 = always through the caches, 2 = always streamed; same
+        bits."""
         check(lib().gcomcot_set_store_policy_(ip(mode)), "store policy")
 
     def set_fused_halves(self, on=True):

@@ -370,13 +370,16 @@ int gcomcot_set_temporal_blocking_(int *mode);
  * Returns 0. */
 int gcomcot_set_fused_halves_(int *on);
 
-/* How the step launches of the device loop, the drop-in entry points and
- * the batch engine store the new time level: *mode = 1 (the default)
- * streams it past the caches (nontemporal stores) for layers of more than
- * about 5.6 M cells, whose one step moves more bytes than the 256 MB
- * last-level cache holds, and stores it through the caches for smaller
- * layers, whose next step or next phase finds it there; 0 = always
- * through the caches, 2 = always streamed.  The same bits every way.
+/* How the step launches of the device loop and the drop-in entry points
+ * store the new time level (the batch engine's launches always store
+ * through the caches): *mode = 1 (the default) streams it past the caches
+ * (nontemporal stores) for layers of more than about 5.6 M cells, whose
+ * one step moves more bytes than the 256 MB last-level cache holds, and
+ * for layers of at most about 0.93 M cells, whose step fits in the L2s
+ * (streaming spares every kernel end the write-back of the new level),
+ * and stores it through the caches for the layers between, whose next
+ * step or next phase finds it in the last-level cache; 0 = always through
+ * the caches, 2 = always streamed.  The same bits every way.
  * The batch engine rebuilds its plan at its next run call.  Returns 0,
  * or 46 for another *mode. */
 int gcomcot_set_store_policy_(int *mode);

@@ -67,8 +67,9 @@ def test_mixed_batch_matches_oracle(two_step, graphs, streams, fused):
 
 
 def test_streamed_stores_in_the_batch():
-    """The batch with every layer's new time level streamed past the caches,
-    switched on after the first plan was built (the plan is rebuilt)."""
+    """The store policy changed after the first plan was built: the plan is
+    rebuilt (the batch kernels store through the caches whatever the
+    policy), and every scenario still lands on the oracle's bits."""
     from gcomcot._lib import lib, ip
     steps = [s + (i % 3) for i, (_, _, s) in enumerate(CASES)]
     print(f"[gpu] batch with streamed stores, steps {steps}", flush=True)

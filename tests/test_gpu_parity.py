@@ -87,9 +87,11 @@ def test_nest_root_halves_fused_or_separate(name, sc, steps, fused):
 
 @pytest.mark.parametrize("name,sc,steps", [CASES[0], CASES[2], CASES[3], CASES[5], CASES[6]],
                          ids=[c[0] for c in (CASES[0], CASES[2], CASES[3], CASES[5], CASES[6])])
-def test_streamed_time_level_stores(name, sc, steps):
+@pytest.mark.parametrize("policy", [2, 0], ids=["streamed", "cached"])
+def test_time_level_store_policies(name, sc, steps, policy):
     """Every layer's new time level written with nontemporal stores (what
-    layers over ~5.6 M cells do by default), through the device loop with
+    the smallest and the largest layers do by default) or through the
+    caches (what layers in between do), through the device loop with
     graphs (two steps per launch on the linear single layer), the
     momentum + next continuity launches of the nests and the drop-in entry
     points: the oracle's bits."""
@@ -99,12 +101,12 @@ def test_streamed_time_level_stores(name, sc, steps):
     dev = DeviceRun(sc)
     with pytest.raises(GcomcotError, match="code 46"):
         dev.set_store_policy(3)
-    dev.set_store_policy(2)
+    dev.set_store_policy(policy)
     try:
-        print(f"[gpu] {name} streamed stores", flush=True)
+        print(f"[gpu] {name} store policy {policy}", flush=True)
         dev.run(steps - 1, graph=True)
         dev.dropin_step()
-        _compare(sc, dev, orc, f"{name} streamed")
+        _compare(sc, dev, orc, f"{name} store policy {policy}")
     finally:
         dev.set_store_policy(1)
         dev.close()
