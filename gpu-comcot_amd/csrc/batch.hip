@@ -389,6 +389,10 @@ void emit_lanes(const std::vector<Scenario *> &act, std::vector<Group> &out, int
 
 size_t al(size_t x) { return (x + 255) & ~(size_t)255; }
 
+// engine_arena: the plan's descriptors go to the engine's arena, reused
+// from plan to plan (the tuner's probes, the continuity prologue and the
+// tail step run there too, each after the previous user has finished and
+// before the next plan is built), instead of an allocation of its own.
 int upload_plan(Plan &P, bool engine_arena = false) {
   size_t bytes = 0;
   int heads = 0;
@@ -572,7 +576,7 @@ int tune(const std::vector<Scenario *> &act) {
       for (auto &tr : g.recs) add_layer_items(d, dev_view(*tr.first, *tr.second, false, false), tx, ty);
       split_queues(d);
       T.step[0].push_back(std::move(d));
-      if (int r = upload_plan(T)) return r;
+      if (int r = upload_plan(T, true)) return r;
       float tot = 0;
       for (int rep = 0; rep < 4; ++rep) {
         if (rep == 1) HIPCHK(hipEventRecord(e0, g_eng.st));
@@ -634,7 +638,7 @@ int tune(const std::vector<Scenario *> &act) {
           split_queues(d);
           T.step[0].push_back(std::move(d));
         }
-        if (int r = upload_plan(T)) return r;
+        if (int r = upload_plan(T, true)) return r;
         float tot = 0;
         for (int rep = 0; rep < 4; ++rep) {
           if (rep == 1) HIPCHK(hipEventRecord(e0, g_eng.st));
@@ -685,7 +689,7 @@ int build_plan(const std::vector<Scenario *> &act) {
       }
     flush_steps(pend, pre.step[0]);
     if (!pre.step[0].empty()) {
-      int r = upload_plan(pre);
+      int r = upload_plan(pre, true);
       if (!r) r = enqueue_plan(pre, 0, 1);
       if (!r) r = hipStreamSynchronize(g_eng.st) == hipSuccess ? 0 : fail(211, "batch continuity");
       free_plan(pre);
@@ -782,7 +786,7 @@ int run(long long n) {
           Plan one;
           one.nsteps = 1;
           emit_step(act, one.step[0]);
-          int r = upload_plan(one);
+          int r = upload_plan(one, true);
           if (!r) r = enqueue_plan(one, 0, 1);
           if (!r) r = hipStreamSynchronize(g_eng.st) == hipSuccess ? 0 : fail(211, "batch tail step");
           free_plan(one);
